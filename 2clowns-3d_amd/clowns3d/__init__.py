@@ -1,0 +1,8 @@
+"""clowns3d -- Python side of the MI355X pressure-Poisson solve of 2CLOWNS-3D.
+
+The compute path is the gfx950 library ``lib/libclowns_pcg.so`` (C-ABI in
+include/clowns_pcg.h); this package only binds it (ctypes), generates the
+synthetic workloads and exposes the adjoint as a torch.autograd.Function.
+"""
+from ._lib import ClownsError, Info, LIB_PATH, lib  # noqa: F401
+from .pcgpme import Solver, pcgpme  # noqa: F401

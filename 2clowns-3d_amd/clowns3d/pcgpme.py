@@ -1,0 +1,145 @@
+"""Host-side mirror of the reference's PCGPME interface over the C-ABI.
+
+``pcgpme`` keeps the argument list and in/out semantics of
+``subroutine PCGPME(AL,NL,NA,LL,D,N,N2,B,X,EPS,PARM,ITER,IER)``
+(src2LOWNS/pcgpme_para.F90:1-58, 250-257) on numpy arrays laid out as the
+Fortran caller holds them: AL/LL column-major (NA, NL), LL 1-based.  Like the
+reference it mutates AL, LL, D, B, X and EPS in place and returns
+(ITER, IER).  ``Solver`` is the device-resident form on torch tensors.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _check_host(name, a, dtype, shape=None, fortran=False):
+    if not isinstance(a, np.ndarray) or a.dtype != dtype:
+        raise TypeError(f"{name} must be a numpy {np.dtype(dtype).name} array")
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"{name} has shape {a.shape}, expected {shape}")
+    if fortran and not a.flags.f_contiguous:
+        raise ValueError(f"{name} must be Fortran-ordered (column-major)")
+    if not fortran and not a.flags.c_contiguous:
+        raise ValueError(f"{name} must be contiguous")
+
+
+def pcgpme(AL, NL, NA, LL, D, N, N2, B, X, EPS, PARM, ITER):
+    """Drop-in of PCGPME through ``clowns_pcgpme``.  Returns (ITER, IER)."""
+    _check_host("AL", AL, np.float64, (NA, NL), fortran=True)
+    _check_host("LL", LL, np.int32, (NA, NL), fortran=True)
+    _check_host("D", D, np.float64, (N,))
+    _check_host("B", B, np.float64, (N,))
+    _check_host("X", X, np.float64, (N + N2,))
+    _check_host("EPS", EPS, np.float64, (2,))
+    it = ctypes.c_int32(int(ITER))
+    ier = ctypes.c_int32(0)
+    i32 = lambda v: ctypes.byref(ctypes.c_int32(int(v)))
+    L.lib().clowns_pcgpme(_ptr(AL), i32(NL), i32(NA), _ptr(LL), _ptr(D), i32(N), i32(N2),
+                          _ptr(B), _ptr(X), _ptr(EPS), ctypes.byref(ctypes.c_double(PARM)),
+                          ctypes.byref(it), ctypes.byref(ier))
+    return it.value, ier.value
+
+
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Solver:
+    """Device-resident PCGPME: the scaled/sorted operator stays in HBM.
+
+    Tensors are torch CUDA tensors on the solver's device.  Work runs on the
+    current torch stream at construction (``stream=None``) so torch events
+    and later torch work order correctly around it.
+    """
+
+    def __init__(self, device=0, stream=None):
+        import torch
+        self._torch = torch
+        if stream is None:
+            stream = torch.cuda.current_stream(device).cuda_stream
+        h = ctypes.c_void_p()
+        L.check(L.lib().clowns_solver_create(ctypes.byref(h), int(device), ctypes.c_void_p(stream)),
+                "clowns_solver_create")
+        self._h = h
+        self.device = device
+        self.N = 0
+        self.lda = 0
+        self.NL = 0
+
+    def close(self):
+        if self._h:
+            L.lib().clowns_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_matrix(self, AL, LL, D, parm=5.0):
+        """AL (NL, lda) float64, LL (NL, lda) int32 1-based, D (N,) -- i.e.
+        the reference's column-major AL(NA,NL) viewed row-major as (NL, NA)."""
+        torch = self._torch
+        NL, lda = AL.shape
+        N = D.shape[0]
+        for name, t, dt in (("AL", AL, torch.float64), ("LL", LL, torch.int32), ("D", D, torch.float64)):
+            if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
+                raise TypeError(f"{name} must be a contiguous CUDA {dt} tensor")
+        if LL.shape != AL.shape:
+            raise ValueError("LL and AL shapes differ")
+        L.check(L.lib().clowns_solver_set_matrix(self._h, _dptr(AL), NL, lda, _dptr(LL), _dptr(D), N,
+                                                 float(parm)), "clowns_solver_set_matrix")
+        self.N, self.lda, self.NL = N, lda, NL
+
+    def solve(self, b, x=None, eps=1e-10, maxiter=10000, mixed=False, warm=False):
+        torch = self._torch
+        if x is None:
+            x = torch.zeros_like(b)
+        flags = (L.CLOWNS_MIXED if mixed else 0) | (L.CLOWNS_WARM_START if warm else 0)
+        info = L.Info()
+        L.check(L.lib().clowns_solver_solve(self._h, _dptr(b), _dptr(x), float(eps), int(maxiter), flags,
+                                            ctypes.byref(info)), "clowns_solver_solve")
+        return x, info.as_dict()
+
+    def solve_transpose(self, g, lam=None, eps=1e-10, maxiter=10000, warm=False):
+        torch = self._torch
+        if lam is None:
+            lam = torch.zeros_like(g)
+        info = L.Info()
+        L.check(L.lib().clowns_solver_solve_transpose(self._h, _dptr(g), _dptr(lam), float(eps), int(maxiter),
+                                                      L.CLOWNS_WARM_START if warm else 0, ctypes.byref(info)),
+                "clowns_solver_solve_transpose")
+        return lam, info.as_dict()
+
+    def grad_coefficients(self, lam, x):
+        torch = self._torch
+        gAL = torch.zeros((self.NL, self.lda), dtype=torch.float64, device=lam.device)
+        gD = torch.zeros(self.N, dtype=torch.float64, device=lam.device)
+        L.check(L.lib().clowns_grad_coefficients(self._h, _dptr(lam), _dptr(x), _dptr(gAL), _dptr(gD)),
+                "clowns_grad_coefficients")
+        return gAL, gD
+
+    def get_scaled(self):
+        torch = self._torch
+        dev = f"cuda:{self.device}"
+        AL = torch.empty((self.NL, self.lda), dtype=torch.float64, device=dev)
+        LL = torch.empty((self.NL, self.lda), dtype=torch.int32, device=dev)
+        D = torch.empty(self.N, dtype=torch.float64, device=dev)
+        W = torch.empty(self.N, dtype=torch.float64, device=dev)
+        B = torch.empty(self.N, dtype=torch.float64, device=dev)
+        L.check(L.lib().clowns_solver_get_scaled(self._h, _dptr(AL), _dptr(LL), _dptr(D), _dptr(W), _dptr(B)),
+                "clowns_solver_get_scaled")
+        return AL, LL, D, W, B
+
+    def time_kernel(self, kernel, reps=50):
+        ms = ctypes.c_double()
+        L.check(L.lib().clowns_solver_time_kernel(self._h, int(kernel), int(reps), ctypes.byref(ms)),
+                "clowns_solver_time_kernel")
+        return ms.value

@@ -1,0 +1,520 @@
+// pcg_kernels.hpp -- gfx950 device kernels of the PCGPME solve.
+//
+// Reference: src2LOWNS/pcgpme_para.F90 (PCGPME, LDUSUB, AXSUB).  Every pass
+// of the reference's OpenMP loops maps to one streaming kernel here:
+//   k_scale_sort   row scaling + ghost indices + per-row exchange sort (:77-112)
+//   k_spmv         AXSUB, Y = A*X in sorted slot order (:323-352), with the
+//                  preceding vector update fused (p = r + beta(p - omega q),
+//                  :244; e = r - alpha q, :173) and recomputed at the gather
+//                  points, so the updated vector is never re-read from HBM
+//   k_pm           one stage of LDUSUB (:279-318); the last stage carries the
+//                  dot products that follow it ((r^,q) :157-160,
+//                  (e,v),(v,v) :186-191)
+//   k_update       x += alpha p + omega e; r = e - omega v; (r,r), (r^,r)
+//                  (:203-211)
+//   k_fin_*        the scalar steps (alpha, omega, ERR, exits, beta), one
+//                  block, deterministic order
+// Layout: matrix columns are structure-of-arrays with column stride lda
+// (identical to the reference's AL(NA,NL)/LL(NA,NL)); LL stays 1-based, so a
+// gather reads x[LL-1]; every vector has N+64 entries whose last 64 (the
+// ghost rows N+1+mod(I,64), :80) are zero.
+// Arithmetic: expressions keep the Fortran's left-to-right order; the
+// library is compiled with -ffp-contract=off so single FP64 kernels agree
+// bit for bit with the reference.  Reductions: per-thread sums in row order,
+// a fixed wavefront/block tree, per-block partials combined by one block in
+// fixed order, so a solve is deterministic run to run.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace clowns {
+
+constexpr int BS = 256;        // threads per block (4 wavefronts of 64)
+constexpr int FIN_BS = 1024;   // threads of the one-block finalize kernels
+constexpr int PAD = 64;        // ghost rows after N (pcgpme_para.F90:80)
+
+// Scalars of one Bi-CGSTAB run, resident in device memory.  Names follow
+// pcgpme_para.F90:47.
+struct Scal {
+    double BN, C1, RQ, ALP, AMU, SS, RN, ERR, ERRP, BETA, EPS;
+    double RN0;        // refinement: ||r|| of the outer residual (scale)
+    int K;             // last completed iteration
+    int itermax;
+    int done;          // 1 = converged / exited: every later kernel returns
+    int ier;
+    int kout;          // ITER returned (the reference returns ITER+1 on 3000)
+    int pad_[3];
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Giving XCD x the contiguous logical blocks [x*G/8, (x+1)*G/8) keeps the
+// +-1, +-row and +-plane neighbours of a row in the L2 that streamed them.
+// G is a multiple of 8.
+__device__ __forceinline__ int logical_block() {
+    const int b = blockIdx.x;
+    return (b & 7) * (gridDim.x >> 3) + (b >> 3);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Sum over the block, fixed order; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NT / 64; ++k) s += lds[k];
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------- setup --
+// a3 + a4: scale row i by W = P2/(max(sum|AL|, D) + P1*D), substitute ghost
+// indices for LL = 0, then the reference's (unstable) exchange sort by |AL|.
+template <int NL>
+__global__ void __launch_bounds__(BS)
+k_scale_sort(int N, int lda, double *__restrict__ AL, int *__restrict__ LL,
+             double *__restrict__ D, double *__restrict__ Wsc, double P1,
+             double P2) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= N) return;
+    const int I = i + 1;
+    double a[NL];
+    int l[NL];
+    double w = 0.0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        a[k] = AL[(size_t)k * lda + i];
+        l[k] = LL[(size_t)k * lda + i];
+        if (l[k] == 0) l[k] = N + 1 + I % 64;
+        w = w + fabs(a[k]);
+    }
+    const double temp = D[i];
+    const double m = (w > temp) ? w : temp;
+    w = P2 / (m + temp * P1);
+#pragma unroll
+    for (int k = 0; k < NL; ++k) a[k] = a[k] * w;
+    D[i] = temp * w;
+    Wsc[i] = w;
+#pragma unroll
+    for (int k = 0; k < NL - 1; ++k) {
+#pragma unroll
+        for (int kk = k + 1; kk < NL; ++kk) {
+            if (fabs(a[k]) < fabs(a[kk])) {
+                const double t = a[k];
+                const int it = l[k];
+                a[k] = a[kk];
+                l[k] = l[kk];
+                a[kk] = t;
+                l[kk] = it;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        AL[(size_t)k * lda + i] = a[k];
+        LL[(size_t)k * lda + i] = l[k];
+    }
+}
+
+// Caller LL must name rows 1..N or 0 (no neighbour); anything else would be
+// gathered out of bounds.  Sets *bad.
+__global__ void __launch_bounds__(BS)
+k_check_ll(int N, int lda, int NL, const int *__restrict__ LL, int *__restrict__ bad) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= N) return;
+    int ok = 1;
+    for (int k = 0; k < NL; ++k) {
+        const int v = LL[(size_t)k * lda + i];
+        ok &= (v >= 0 && v <= N) ? 1 : 0;
+    }
+    if (!ok) atomicOr(bad, 1);
+}
+
+// FP32 copy of the scaled, sorted FP64 operator (the mixed solve's inner
+// matrix); LL is shared.
+__global__ void __launch_bounds__(BS)
+k_to_f32(int N, int lda, int NL, const double *__restrict__ AL,
+         const double *__restrict__ D, float *__restrict__ AL32,
+         float *__restrict__ D32) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= N) return;
+    for (int k = 0; k < NL; ++k)
+        AL32[(size_t)k * lda + i] = (float)AL[(size_t)k * lda + i];
+    D32[i] = (float)D[i];
+}
+
+// b_s = W o b (pcgpme_para.F90:92); x0 into the padded solve vector.
+__global__ void __launch_bounds__(BS)
+k_scale_b(int N, const double *__restrict__ b, const double *__restrict__ Wsc,
+          double *__restrict__ bs, const double *__restrict__ x_in,
+          double *__restrict__ x) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= N) return;
+    bs[i] = b[i] * Wsc[i];
+    if (x) x[i] = x_in ? x_in[i] : 0.0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_fill(size_t n, T *__restrict__ v, T val) {
+    const size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
+    if (i < n) v[i] = val;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_copy(int N, const T *__restrict__ a, T *__restrict__ b) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i < N) b[i] = a[i];
+}
+
+// ------------------------------------------------------------ SpMV (a6) --
+// MODE 0: x = v0.
+// MODE 1: x = v1 + beta*(v0 - omega*v2)   (p update, r = v1, p = v0, q = v2)
+// MODE 2: x = v1 - alpha*v2                 (e = r - alpha q)
+// MODE 3: y = v1 - A*v0                     (true residual b_s - A x)
+// MODE 1/2 also store x_i in vout.
+template <typename T, int NL, int MODE>
+__global__ void __launch_bounds__(BS)
+k_spmv(const Scal *__restrict__ sc, int N, int lda, int R,
+       const T *__restrict__ AL, const int *__restrict__ LL,
+       const T *__restrict__ D, const T *__restrict__ v0,
+       const T *__restrict__ v1, const T *__restrict__ v2,
+       T *__restrict__ vout, T *__restrict__ y) {
+    if (sc->done) return;
+    T c0 = T(0), c1 = T(0);
+    if (MODE == 1) {
+        c0 = (T)sc->BETA;
+        c1 = (T)sc->AMU;
+    } else if (MODE == 2) {
+        c0 = (T)sc->ALP;
+    }
+    auto X = [&](int j) -> T {
+        if (MODE == 1) return v1[j] + c0 * (v0[j] - c1 * v2[j]);
+        if (MODE == 2) return v1[j] - c0 * v2[j];
+        return v0[j];
+    };
+    const int base = logical_block() * BS * R + threadIdx.x;
+    for (int rr = 0; rr < R; ++rr) {
+        const int i = base + rr * BS;
+        if (i >= N) return;
+        int l[NL];
+        T a[NL];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            l[k] = LL[(size_t)k * lda + i] - 1;
+            a[k] = AL[(size_t)k * lda + i];
+        }
+        const T xi = X(i);
+        T acc = xi * D[i];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) acc = acc + a[k] * X(l[k]);
+        if (MODE == 1 || MODE == 2) vout[i] = xi;
+        if (MODE == 3)
+            y[i] = v1[i] - acc;
+        else
+            y[i] = acc;
+    }
+}
+
+// -------------------------------------------------------------- PM (a7) --
+// out_i = in_i - a0_i*in[l0_i] (- a1_i*in[l1_i] when NT == 2).
+// DOT 0: none.  DOT 1: part0 += w_i*out_i.  DOT 2: part0 += w_i*out_i,
+// part1 += out_i*out_i.  DOT 3: part0 += out_i*out_i.
+template <typename T, int NT, int DOT>
+__global__ void __launch_bounds__(BS)
+k_pm(const Scal *__restrict__ sc, int N, int R, const T *__restrict__ a0,
+     const int *__restrict__ l0, const T *__restrict__ a1,
+     const int *__restrict__ l1, const T *__restrict__ in,
+     T *__restrict__ out, const T *__restrict__ w, double *__restrict__ part) {
+    if (sc->done) return;
+    double s0 = 0.0, s1 = 0.0;
+    const int lb = logical_block();
+    const int base = lb * BS * R + threadIdx.x;
+    for (int rr = 0; rr < R; ++rr) {
+        const int i = base + rr * BS;
+        if (i >= N) break;
+        T o;
+        if (NT == 2) {
+            const int j0 = l0[i] - 1, j1 = l1[i] - 1;
+            const T c0 = a0[i], c1 = a1[i];
+            o = in[i] - c0 * in[j0] - c1 * in[j1];
+        } else {
+            const int j0 = l0[i] - 1;
+            const T c0 = a0[i];
+            o = in[i] - c0 * in[j0];
+        }
+        out[i] = o;
+        if (DOT == 1 || DOT == 2) s0 += (double)w[i] * (double)o;
+        if (DOT == 2) s1 += (double)o * (double)o;
+        if (DOT == 3) s0 += (double)o * (double)o;
+    }
+    if (DOT != 0) {
+        __shared__ double lds[BS / 64];
+        const double t0 = block_sum<BS>(s0, lds);
+        if (DOT == 2) {
+            const double t1 = block_sum<BS>(s1, lds);
+            if (threadIdx.x == 0) part[gridDim.x + lb] = t1;
+        }
+        if (threadIdx.x == 0) part[lb] = t0;
+    }
+}
+
+// ---------------------------------------------------------- update (a11) --
+// X = X + ALP*P + AMU*E;  R = E - AMU*V;  RN += R*R;  C1 += R*R^.
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_update(const Scal *__restrict__ sc, int N, int R, T *__restrict__ x,
+         const T *__restrict__ p, const T *__restrict__ e,
+         const T *__restrict__ v, T *__restrict__ r,
+         const T *__restrict__ rh, double *__restrict__ part) {
+    if (sc->done) return;
+    const T alp = (T)sc->ALP, amu = (T)sc->AMU;
+    double rn = 0.0, c1 = 0.0;
+    const int lb = logical_block();
+    const int base = lb * BS * R + threadIdx.x;
+    for (int rr = 0; rr < R; ++rr) {
+        const int i = base + rr * BS;
+        if (i >= N) break;
+        T t = e[i];
+        x[i] = x[i] + alp * p[i] + amu * t;
+        t = t - amu * v[i];
+        r[i] = t;
+        rn += (double)t * (double)t;
+        c1 += (double)t * (double)rh[i];
+    }
+    __shared__ double lds[BS / 64];
+    const double t0 = block_sum<BS>(rn, lds);
+    const double t1 = block_sum<BS>(c1, lds);
+    if (threadIdx.x == 0) {
+        part[lb] = t0;
+        part[gridDim.x + lb] = t1;
+    }
+}
+
+// Initial residual (a5, pcgpme_para.F90:133-142): wk1 holds PM*W*b, wk3 holds
+// PM*A*x0 (nullptr when x0 = 0, where it is exactly zero).
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_init(const Scal *__restrict__ sc, int N, int R, T *__restrict__ wk1,
+       const T *__restrict__ wk3, T *__restrict__ r, T *__restrict__ rh,
+       double *__restrict__ part) {
+    if (sc->done) return;
+    double bn = 0.0, c1 = 0.0;
+    const int lb = logical_block();
+    const int base = lb * BS * R + threadIdx.x;
+    for (int rr = 0; rr < R; ++rr) {
+        const int i = base + rr * BS;
+        if (i >= N) break;
+        T t = wk1[i];
+        bn += (double)t * (double)t;
+        if (wk3) t = t - wk3[i];
+        r[i] = t;
+        c1 += (double)t * (double)t;
+        wk1[i] = t;
+        rh[i] = t;
+    }
+    __shared__ double lds[BS / 64];
+    const double t0 = block_sum<BS>(bn, lds);
+    const double t1 = block_sum<BS>(c1, lds);
+    if (threadIdx.x == 0) {
+        part[lb] = t0;
+        part[gridDim.x + lb] = t1;
+    }
+}
+
+// ------------------------------------------------------------ finalize --
+__device__ __forceinline__ double fin_sum(const double *__restrict__ p, int G) {
+    __shared__ double lds[FIN_BS / 64];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < G; k += FIN_BS) s += p[k];
+    return block_sum<FIN_BS>(s, lds);
+}
+
+// Reset the run's scalars (also makes a stale `done` from an earlier run go).
+__global__ void k_reset(Scal *sc, double eps, int itermax) {
+    if (threadIdx.x != 0) return;
+    Scal s = {};
+    s.EPS = eps;
+    s.itermax = itermax;
+    *sc = s;
+}
+
+// After k_init: BN, C1.  A zero right-hand side with a zero residual returns
+// at once with X unchanged (the reference divides 0/0 there).
+__global__ void __launch_bounds__(FIN_BS)
+k_fin_init(Scal *sc, const double *__restrict__ part, int G) {
+    if (sc->done) return;
+    const double bn = fin_sum(part, G);
+    const double c1 = fin_sum(part + G, G);
+    if (threadIdx.x == 0) {
+        sc->BN = bn;
+        sc->C1 = c1;
+        sc->ERRP = 0.0;
+        sc->K = 0;
+        if (bn == 0.0 && c1 == 0.0) {
+            sc->ERR = 0.0;
+            sc->kout = 0;
+            sc->ier = 0;
+            sc->done = 1;
+        }
+    }
+}
+
+// ALP = C1/RQ (pcgpme_para.F90:167).
+__global__ void __launch_bounds__(FIN_BS)
+k_fin_alpha(Scal *sc, const double *__restrict__ part, int G) {
+    if (sc->done) return;
+    const double rq = fin_sum(part, G);
+    if (threadIdx.x == 0) {
+        sc->RQ = rq;
+        sc->ALP = sc->C1 / rq;
+    }
+}
+
+// AMU = (E,V)/(V,V) (pcgpme_para.F90:197).
+__global__ void __launch_bounds__(FIN_BS)
+k_fin_omega(Scal *sc, const double *__restrict__ part, int G) {
+    if (sc->done) return;
+    const double amu = fin_sum(part, G);
+    const double ss = fin_sum(part + G, G);
+    if (threadIdx.x == 0) {
+        sc->SS = ss;
+        sc->AMU = amu / ss;
+    }
+}
+
+// ERR and the exits in the reference's order (pcgpme_para.F90:215-251).
+__global__ void __launch_bounds__(FIN_BS)
+k_fin_beta(Scal *sc, const double *__restrict__ part, int G, int K) {
+    if (sc->done) return;
+    const double rn = fin_sum(part, G);
+    const double c1 = fin_sum(part + G, G);
+    if (threadIdx.x != 0) return;
+    sc->RN = rn;
+    sc->C1 = c1;
+    sc->K = K;
+    const double err = sqrt(rn / sc->BN);
+    sc->ERR = err;
+    int stop = 0;
+    if (!(err == err) || err > 1.7976931348623157e308) {
+        sc->ier = 4001; // CLOWNS_ENONFINITE
+        stop = 1;
+    } else if (K >= 10 && fabs(err - sc->ERRP) <= err * 1.0e-6) {
+        stop = 1;
+    } else if (err <= sc->EPS) {
+        stop = 1;
+    } else {
+        sc->ERRP = err;
+        if (sc->AMU == 0.0 || c1 == 0.0) {
+            stop = 1;
+        } else {
+            sc->BETA = c1 / sc->AMU / sc->RQ;
+            if (K >= sc->itermax) {
+                sc->ier = 3000;
+                sc->kout = K + 1;
+                sc->done = 1;
+                return;
+            }
+        }
+    }
+    if (stop) {
+        sc->kout = K;
+        sc->done = 1;
+    }
+}
+
+// Mixed precision, outer step: ERR64 = sqrt(RN/BN) of the true preconditioned
+// residual; RN0 = ||r|| scales the FP32 inner right-hand side.
+__global__ void __launch_bounds__(FIN_BS)
+k_fin_true(Scal *outer, const double *__restrict__ part, int G) {
+    const double rn = fin_sum(part, G);
+    if (threadIdx.x == 0) {
+        outer->RN = rn;
+        outer->RN0 = sqrt(rn);
+        outer->ERR = sqrt(rn / outer->BN);
+    }
+}
+
+// Mixed: FP32 inner right-hand side = r/||r||.
+__global__ void __launch_bounds__(BS)
+k_rhs32(int N, const Scal *__restrict__ outer, const double *__restrict__ r,
+        float *__restrict__ r32) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= N) return;
+    const double s = outer->RN0 > 0.0 ? 1.0 / outer->RN0 : 0.0;
+    r32[i] = (float)(r[i] * s);
+}
+
+// Mixed: x += ||r|| * delta (FP64 accumulate of the FP32 correction).
+__global__ void __launch_bounds__(BS)
+k_correct(int N, const Scal *__restrict__ outer, const float *__restrict__ d,
+          double *__restrict__ x) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= N) return;
+    x[i] = x[i] + outer->RN0 * (double)d[i];
+}
+
+// ------------------------------------------------------------- adjoint --
+// A^T in the reference layout, from the unscaled caller-layout operator.
+// The 7-point pattern is structurally symmetric (a face is open for both of
+// its cells, src2LOWNS/1sub-matrix.F90:41-45, 1sub-momentum.F90:496-512), so
+// row j of A^T keeps row j's slots: A^T(j, LL(j,k)) = A(LL(j,k), j), found by
+// searching row LL(j,k)'s slots for j.  A missing partner sets *bad.
+__global__ void __launch_bounds__(BS)
+k_transpose(int N, int lda, int NL, const double *__restrict__ AL,
+            const int *__restrict__ LL, const double *__restrict__ D,
+            double *__restrict__ ALt, int *__restrict__ LLt,
+            double *__restrict__ Dt, int *__restrict__ bad) {
+    const int j = blockIdx.x * BS + threadIdx.x;
+    if (j >= N) return;
+    for (int k = 0; k < NL; ++k) {
+        const int i = LL[(size_t)k * lda + j];
+        double val = 0.0;
+        int idx = 0;
+        if (i >= 1 && i <= N) {
+            int found = 0;
+            for (int kk = 0; kk < NL; ++kk) {
+                if (LL[(size_t)kk * lda + (i - 1)] == j + 1) {
+                    val = AL[(size_t)kk * lda + (i - 1)];
+                    found = 1;
+                    break;
+                }
+            }
+            if (!found) atomicOr(bad, 1);
+            idx = i;
+        } else if (i != 0) {
+            atomicOr(bad, 2);
+        }
+        ALt[(size_t)k * lda + j] = val;
+        LLt[(size_t)k * lda + j] = idx;
+    }
+    Dt[j] = D[j];
+}
+
+// dL/dAL(j,k) = -lambda_j * x(LL(j,k)),  dL/dD_j = -lambda_j * x_j, in the
+// caller's unscaled layout (LL = 0 slots get 0).
+__global__ void __launch_bounds__(BS)
+k_grad_coef(int N, int lda, int NL, const int *__restrict__ LL,
+            const double *__restrict__ lam, const double *__restrict__ x,
+            double *__restrict__ gAL, double *__restrict__ gD) {
+    const int j = blockIdx.x * BS + threadIdx.x;
+    if (j >= N) return;
+    const double l = lam[j];
+    for (int k = 0; k < NL; ++k) {
+        const int i = LL[(size_t)k * lda + j];
+        if (gAL) gAL[(size_t)k * lda + j] = (i >= 1 && i <= N) ? -l * x[i - 1] : 0.0;
+    }
+    if (gD) gD[j] = -l * x[j];
+}
+
+} // namespace clowns

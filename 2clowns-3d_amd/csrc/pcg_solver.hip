@@ -1,0 +1,789 @@
+// pcg_solver.hip -- host runtime of the gfx950 PCGPME solve and its C-ABI.
+//
+// The reference (src2LOWNS/pcgpme_para.F90) allocates its work arrays on the
+// stack per call and rescales/re-sorts the matrix on every call.  Here a
+// solver object owns grow-only device buffers; the scaled, sorted operator
+// stays resident in HBM between solves (the SMAC pass loop of
+// src2LOWNS/1sub-momentum.F90:42-60 changes only the right-hand side), and a
+// solve runs entirely on one HIP stream.  Convergence is decided on the
+// device (k_fin_beta); the host only polls a pinned copy of the scalars every
+// few iterations, so iterations run back to back without host round trips.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "clowns_pcg.h"
+#include "pcg_kernels.hpp"
+
+using namespace clowns;
+
+#define HIPCHK(expr)                                                         \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess) {                                              \
+            fprintf(stderr, "clowns_pcg: %s failed: %s (%s:%d)\n", #expr,    \
+                    hipGetErrorString(e_), __FILE__, __LINE__);              \
+            return e_ == hipErrorOutOfMemory ? CLOWNS_ENOMEM : CLOWNS_EHIP;  \
+        }                                                                    \
+    } while (0)
+
+namespace {
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+template <typename T> struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t want) {
+        if (want <= n) return CLOWNS_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e == hipErrorOutOfMemory ? CLOWNS_ENOMEM : CLOWNS_EHIP;
+        }
+        n = want;
+        return CLOWNS_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Vectors of one precision for one Bi-CGSTAB run.
+template <typename T> struct VecSet {
+    DevBuf<T> r, rh, p0, p1, q, e, v, y, t, x;
+    int ensure(size_t n) {
+        DevBuf<T> *all[] = {&r, &rh, &p0, &p1, &q, &e, &v, &y, &t, &x};
+        for (auto *b : all) {
+            int rc = b->ensure(n);
+            if (rc) return rc;
+        }
+        return CLOWNS_OK;
+    }
+    void release() {
+        DevBuf<T> *all[] = {&r, &rh, &p0, &p1, &q, &e, &v, &y, &t, &x};
+        for (auto *b : all) b->release();
+    }
+};
+
+} // namespace
+
+struct clowns_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    // Resident operator (reference layout after PCGPME's scale + sort).
+    int N = 0, NL = 0, lda = 0;
+    double parm = 5.0;
+    DevBuf<double> AL, D, W;
+    DevBuf<int> LL;
+    DevBuf<float> AL32, D32;
+    bool have32 = false;
+    // Transposed operator (adjoint), scaled + sorted the same way.
+    DevBuf<double> ALt, Dt, Wt;
+    DevBuf<int> LLt;
+    bool have_t = false;
+    // Caller-layout copy of the unscaled operator (gradients).
+    DevBuf<double> AL0, D0;
+    DevBuf<int> LL0;
+
+    VecSet<double> v64;
+    VecSet<float> v32;
+    DevBuf<double> bs;     // scaled right-hand side
+    DevBuf<double> stage_b, stage_x;  // clowns_pcgpme host staging
+    DevBuf<double> part;   // per-block partial sums
+    DevBuf<int> flag;      // device error flag
+    DevBuf<Scal> sc;       // [0] inner/FP64 run, [1] outer (mixed)
+    Scal *h_sc = nullptr;  // pinned mirror
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+
+    int R = 1;             // rows per thread in the streaming kernels
+    int poll = 8;          // iterations between host polls of `done`
+};
+
+namespace {
+
+std::mutex g_mu;
+clowns_solver *g_default[64] = {};
+
+int n_pad(int N) { return N + PAD; }
+
+struct Geo {
+    int G, R;
+};
+inline Geo geo_of(int N, int R) {
+    int G = cdiv(N, (long)BS * R);
+    G = (G + 7) & ~7;
+    return {G, R};
+}
+
+int ensure_common(clowns_solver *h, int N) {
+    int rc;
+    Geo g = geo_of(N, h->R);
+    if ((rc = h->part.ensure((size_t)2 * g.G))) return rc;
+    if ((rc = h->sc.ensure(2))) return rc;
+    if (!h->h_sc) HIPCHK(hipHostMalloc((void **)&h->h_sc, 2 * sizeof(Scal), hipHostMallocDefault));
+    if (!h->ev0) {
+        HIPCHK(hipEventCreate(&h->ev0));
+        HIPCHK(hipEventCreate(&h->ev1));
+        HIPCHK(hipEventCreate(&h->ev2));
+    }
+    return CLOWNS_OK;
+}
+
+template <typename T> void fill(hipStream_t s, T *p, size_t n, T val) {
+    if (n == 0) return;
+    k_fill<T><<<cdiv((long)n, BS), BS, 0, s>>>(n, p, val);
+}
+
+// Operator views for one run.
+template <typename T> struct Op {
+    int N, NL, lda;
+    const T *AL;
+    const int *LL;
+    const T *D;
+};
+
+template <typename T, int NL>
+void launch_spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
+                 const T *v0, const T *v1, const T *v2, T *vout, T *y) {
+    switch (mode) {
+    case 0:
+        k_spmv<T, NL, 0><<<g.G, BS, 0, s>>>(sc, A.N, A.lda, g.R, A.AL, A.LL, A.D, v0, v1, v2, vout, y);
+        break;
+    case 1:
+        k_spmv<T, NL, 1><<<g.G, BS, 0, s>>>(sc, A.N, A.lda, g.R, A.AL, A.LL, A.D, v0, v1, v2, vout, y);
+        break;
+    case 2:
+        k_spmv<T, NL, 2><<<g.G, BS, 0, s>>>(sc, A.N, A.lda, g.R, A.AL, A.LL, A.D, v0, v1, v2, vout, y);
+        break;
+    default:
+        k_spmv<T, NL, 3><<<g.G, BS, 0, s>>>(sc, A.N, A.lda, g.R, A.AL, A.LL, A.D, v0, v1, v2, vout, y);
+        break;
+    }
+}
+
+template <typename T, int NT>
+void launch_pm_stage(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j,
+                     const T *in, T *out, int dot, const T *w, double *part) {
+    const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
+    const T *a0 = A.AL + o0, *a1 = NT == 2 ? A.AL + o1 : nullptr;
+    const int *l0 = A.LL + o0, *l1 = NT == 2 ? A.LL + o1 : nullptr;
+    switch (dot) {
+    case 0: k_pm<T, NT, 0><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
+    case 1: k_pm<T, NT, 1><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
+    case 2: k_pm<T, NT, 2><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
+    default: k_pm<T, NT, 3><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
+    }
+}
+
+// LDUSUB (pcgpme_para.F90:263-320): out = PM*in through scratch y, t.
+// `in` may be `out`; neither may be y or t except in == y.
+template <typename T>
+void apply_pm(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, const T *in,
+              T *out, T *y, T *t, int dot, const T *w, double *part) {
+    if (A.NL == 6) {
+        launch_pm_stage<T, 1>(s, g, sc, A, 0, in, t, 0, w, part);
+        launch_pm_stage<T, 1>(s, g, sc, A, 1, t, y, 0, w, part);
+        launch_pm_stage<T, 1>(s, g, sc, A, 2, y, t, 0, w, part);
+        launch_pm_stage<T, 1>(s, g, sc, A, 3, t, y, 0, w, part);
+        launch_pm_stage<T, 2>(s, g, sc, A, 4, y, out, dot, w, part);
+    } else {
+        launch_pm_stage<T, 1>(s, g, sc, A, 0, in, t, 0, w, part);
+        launch_pm_stage<T, 1>(s, g, sc, A, 1, t, y, 0, w, part);
+        launch_pm_stage<T, 1>(s, g, sc, A, 2, y, t, 0, w, part);
+        launch_pm_stage<T, 1>(s, g, sc, A, 3, t, out, dot, w, part);
+    }
+}
+
+template <typename T>
+void spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
+          const T *v0, const T *v1, const T *v2, T *vout, T *y) {
+    if (A.NL == 6)
+        launch_spmv<T, 6>(s, g, sc, A, mode, v0, v1, v2, vout, y);
+    else
+        launch_spmv<T, 4>(s, g, sc, A, mode, v0, v1, v2, vout, y);
+}
+
+// The PCGPME iteration (pcgpme_para.F90:148-247) on already initialised
+// vectors: V.p0 = V.r = V.rh = r0, V.x = x0, scalars BN/C1 set by k_fin_init.
+// Returns after the device has set `done`.
+template <typename T>
+int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc,
+                 int itermax, int sc_index) {
+    hipStream_t s = h->stream;
+    const Geo g = geo_of(A.N, h->R);
+    double *part = h->part.p;
+    T *pc = V.p0.p, *pn = V.p1.p;
+    int poll = h->poll;
+    int K = 1;
+    while (true) {
+        int kend = std::min(itermax, K + poll - 1);
+        for (; K <= kend; ++K) {
+            // Q = PM*A*P, with P = R + BETA*(P - AMU*Q) fused for K > 1.
+            if (K == 1) {
+                spmv<T>(s, g, sc, A, 0, pc, nullptr, nullptr, nullptr, V.y.p);
+            } else {
+                spmv<T>(s, g, sc, A, 1, pc, V.r.p, V.q.p, pn, V.y.p);
+                std::swap(pc, pn);
+            }
+            apply_pm<T>(s, g, sc, A, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
+            k_fin_alpha<<<1, FIN_BS, 0, s>>>(sc, part, g.G);
+            // V = PM*A*E with E = R - ALP*Q fused.
+            spmv<T>(s, g, sc, A, 2, nullptr, V.r.p, V.q.p, V.e.p, V.y.p);
+            apply_pm<T>(s, g, sc, A, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
+            k_fin_omega<<<1, FIN_BS, 0, s>>>(sc, part, g.G);
+            k_update<T><<<g.G, BS, 0, s>>>(sc, A.N, g.R, V.x.p, pc, V.e.p, V.v.p,
+                                          V.r.p, V.rh.p, part);
+            k_fin_beta<<<1, FIN_BS, 0, s>>>(sc, part, g.G, K);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h->h_sc + sc_index, sc, sizeof(Scal), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const Scal &hs = h->h_sc[sc_index];
+        if (hs.done) break;
+        if (K > itermax) break; // cannot happen: the device sets done at itermax
+        // Poll less often while far from convergence.
+        poll = std::min(32, poll * 2);
+    }
+    // Leave the current search direction in p0 for callers that inspect it.
+    if (pc != V.p0.p) std::swap(V.p0, V.p1);
+    return CLOWNS_OK;
+}
+
+int check_nl(int NL) { return (NL == 6 || NL == 4) ? CLOWNS_OK : CLOWNS_EBADNL; }
+
+// Load AL/LL/D (reference layout, stride lda) from `kind` memory and scale +
+// sort in place on the device.
+int load_matrix(clowns_solver *h, const double *AL, int NL, int lda, const int *LL,
+                const double *D, int N, double parm, hipMemcpyKind kind, bool keep_raw) {
+    int rc;
+    if ((rc = check_nl(NL))) return rc;
+    if (N < 1 || lda < N || !AL || !LL || !D) return CLOWNS_EBADARG;
+    const size_t nm = (size_t)lda * NL;
+    if ((rc = h->AL.ensure(nm)) || (rc = h->LL.ensure(nm)) || (rc = h->D.ensure(N)) ||
+        (rc = h->W.ensure(N)))
+        return rc;
+    if ((rc = ensure_common(h, N))) return rc;
+    hipStream_t s = h->stream;
+    HIPCHK(hipMemcpyAsync(h->AL.p, AL, nm * sizeof(double), kind, s));
+    HIPCHK(hipMemcpyAsync(h->LL.p, LL, nm * sizeof(int), kind, s));
+    HIPCHK(hipMemcpyAsync(h->D.p, D, (size_t)N * sizeof(double), kind, s));
+    if (keep_raw) {
+        if ((rc = h->AL0.ensure(nm)) || (rc = h->LL0.ensure(nm)) || (rc = h->D0.ensure(N)))
+            return rc;
+        HIPCHK(hipMemcpyAsync(h->AL0.p, h->AL.p, nm * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(h->LL0.p, h->LL.p, nm * sizeof(int), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(h->D0.p, h->D.p, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    // Reject out-of-range neighbour indices before any gather uses them.
+    if ((rc = h->flag.ensure(1))) return rc;
+    fill<int>(s, h->flag.p, 1, 0);
+    k_check_ll<<<cdiv(N, BS), BS, 0, s>>>(N, lda, NL, h->LL.p, h->flag.p);
+    int hbad = 0;
+    HIPCHK(hipMemcpyAsync(&hbad, h->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hbad) {
+        h->N = 0;
+        return CLOWNS_EBADARG;
+    }
+    double P1 = parm, P2;
+    if (parm < 1.0) P1 = 2.0 * 2.0;
+    P2 = P1 + 1.0;
+    if (NL == 6)
+        k_scale_sort<6><<<cdiv(N, BS), BS, 0, s>>>(N, lda, h->AL.p, h->LL.p, h->D.p, h->W.p, P1, P2);
+    else
+        k_scale_sort<4><<<cdiv(N, BS), BS, 0, s>>>(N, lda, h->AL.p, h->LL.p, h->D.p, h->W.p, P1, P2);
+    HIPCHK(hipGetLastError());
+    h->N = N;
+    h->NL = NL;
+    h->lda = lda;
+    h->parm = parm;
+    h->have32 = false;
+    h->have_t = false;
+    return CLOWNS_OK;
+}
+
+int ensure_f32(clowns_solver *h) {
+    if (h->have32) return CLOWNS_OK;
+    int rc;
+    const size_t nm = (size_t)h->lda * h->NL;
+    if ((rc = h->AL32.ensure(nm)) || (rc = h->D32.ensure(h->N))) return rc;
+    k_to_f32<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->NL, h->AL.p, h->D.p,
+                                                   h->AL32.p, h->D32.p);
+    HIPCHK(hipGetLastError());
+    h->have32 = true;
+    return CLOWNS_OK;
+}
+
+template <typename T> int ensure_vecs(clowns_solver *h, VecSet<T> &V, int N) {
+    const size_t n = (size_t)n_pad(N);
+    const bool grew = V.r.n < n;
+    int rc = V.ensure(n);
+    if (rc) return rc;
+    if (grew) {
+        // Ghost rows must read as zero in every vector.
+        DevBuf<T> *all[] = {&V.r, &V.rh, &V.p0, &V.p1, &V.q, &V.e, &V.v, &V.y, &V.t, &V.x};
+        for (auto *b : all) fill<T>(h->stream, b->p, b->n, T(0));
+    }
+    return CLOWNS_OK;
+}
+
+// FP64 PCGPME on the resident operator `A` with row scales `Wsc`: exactly the
+// reference's sequence (setup :115-145, iteration :148-247).
+int solve_fp64(clowns_solver *h, const Op<double> &A, const double *Wsc, const double *d_b,
+               double *d_x, double eps, int itermax, bool warm, clowns_info *info) {
+    hipStream_t s = h->stream;
+    int rc;
+    VecSet<double> &V = h->v64;
+    if ((rc = ensure_vecs(h, V, A.N)) || (rc = h->bs.ensure(n_pad(A.N)))) return rc;
+    const int N = A.N;
+    const Geo g = geo_of(N, h->R);
+    Scal *sc = h->sc.p;
+    double *part = h->part.p;
+    HIPCHK(hipEventRecord(h->ev0, s));
+    k_reset<<<1, 64, 0, s>>>(sc, eps, itermax);
+    // WK1 = W*B; X = x0.
+    k_scale_b<<<cdiv(N, BS), BS, 0, s>>>(N, d_b, Wsc, V.p0.p, warm ? d_x : nullptr, V.x.p);
+    // WK1 = PM*WK1 (:123).
+    apply_pm<double>(s, g, sc, A, V.p0.p, V.p0.p, V.y.p, V.t.p, 0, nullptr, part);
+    const double *wk3 = nullptr;
+    if (warm) {
+        // WK3 = PM*A*X (:126-127).
+        spmv<double>(s, g, sc, A, 0, V.x.p, nullptr, nullptr, nullptr, V.y.p);
+        apply_pm<double>(s, g, sc, A, V.y.p, V.q.p, V.y.p, V.t.p, 0, nullptr, part);
+        wk3 = V.q.p;
+    }
+    k_init<double><<<g.G, BS, 0, s>>>(sc, N, g.R, V.p0.p, wk3, V.r.p, V.rh.p, part);
+    k_fin_init<<<1, FIN_BS, 0, s>>>(sc, part, g.G);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(h->ev2, s));
+    if ((rc = run_bicgstab<double>(h, A, V, sc, itermax, 0))) return rc;
+    HIPCHK(hipMemcpyAsync(d_x, V.x.p, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipEventRecord(h->ev1, s));
+    HIPCHK(hipMemcpyAsync(h->h_sc, sc, sizeof(Scal), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const Scal &hs = h->h_sc[0];
+    if (info) {
+        float ms = 0.f, ms0 = 0.f;
+        (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+        (void)hipEventElapsedTime(&ms0, h->ev0, h->ev2);
+        info->iters = hs.kout;
+        info->ier = hs.ier;
+        info->refinements = 0;
+        info->err = hs.ERR;
+        info->ms = ms;
+        info->setup_ms = ms0;
+    }
+    return CLOWNS_OK;
+}
+
+// Mixed precision: FP32 inner PM-Bi-CGSTAB on (M A) d = r, FP64 refinement
+// x += d with the true preconditioned residual r = M (W b - W A x) recomputed
+// in FP64 after each inner solve; the outer stopping value is the
+// reference's sqrt(RN/BN) evaluated on that true residual.
+int solve_mixed(clowns_solver *h, const Op<double> &A, const Op<float> &A32, const double *Wsc,
+                const double *d_b, double *d_x, double eps, int itermax, bool warm,
+                clowns_info *info) {
+    hipStream_t s = h->stream;
+    int rc;
+    VecSet<double> &V = h->v64;
+    VecSet<float> &F = h->v32;
+    if ((rc = ensure_vecs(h, V, A.N)) || (rc = ensure_vecs(h, F, A.N)) ||
+        (rc = h->bs.ensure(n_pad(A.N))))
+        return rc;
+    const int N = A.N;
+    const Geo g = geo_of(N, h->R);
+    Scal *sc = h->sc.p, *outer = h->sc.p + 1;
+    double *part = h->part.p;
+    HIPCHK(hipEventRecord(h->ev0, s));
+    k_reset<<<1, 64, 0, s>>>(outer, eps, itermax);
+    k_reset<<<1, 64, 0, s>>>(sc, eps, itermax);
+    // bs = W*b ; BN = ||PM*bs||^2.
+    k_scale_b<<<cdiv(N, BS), BS, 0, s>>>(N, d_b, Wsc, h->bs.p, warm ? d_x : nullptr, V.x.p);
+    apply_pm<double>(s, g, sc, A, h->bs.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
+    k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, g.G);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h->h_sc + 1, outer, sizeof(Scal), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double BN = h->h_sc[1].RN;
+    HIPCHK(hipEventRecord(h->ev2, s));
+    double err = 1.0;
+    int total_inner = 0, refinements = 0, ier = CLOWNS_NOCONV;
+    if (BN == 0.0 && !warm) {
+        err = 0.0;
+        ier = CLOWNS_OK;
+    } else {
+        // outer->BN for k_fin_true; cold start: r = PM*bs, so ERR = 1.
+        Scal &init = h->h_sc[1];
+        init.BN = BN;
+        init.ERR = std::sqrt(init.RN / BN);
+        HIPCHK(hipMemcpyAsync(outer, &init, sizeof(Scal), hipMemcpyHostToDevice, s));
+        if (warm) {
+            // r = PM*(bs - A x).
+            spmv<double>(s, g, sc, A, 3, V.x.p, h->bs.p, nullptr, nullptr, V.y.p);
+            apply_pm<double>(s, g, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
+            k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, g.G);
+        }
+        const float tau_min = 1e-4f;
+        for (int it = 0; it < 60; ++it) {
+            HIPCHK(hipMemcpyAsync(h->h_sc + 1, outer, sizeof(Scal), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            err = h->h_sc[1].ERR;
+            if (!(err == err) || std::isinf(err)) {
+                ier = CLOWNS_ENONFINITE;
+                break;
+            }
+            if (err <= eps) {
+                ier = CLOWNS_OK;
+                break;
+            }
+            if (total_inner >= itermax) break;
+            // Inner solve of (M A) d = r/||r|| to relative tolerance tau.
+            double tau = std::max((double)tau_min, 0.3 * eps / err);
+            k_reset<<<1, 64, 0, s>>>(sc, tau, std::max(1, itermax - total_inner));
+            k_rhs32<<<cdiv(N, BS), BS, 0, s>>>(N, outer, V.r.p, F.p0.p);
+            fill<float>(s, F.x.p, (size_t)N, 0.f);
+            k_init<float><<<g.G, BS, 0, s>>>(sc, N, g.R, F.p0.p, nullptr, F.r.p, F.rh.p, part);
+            k_fin_init<<<1, FIN_BS, 0, s>>>(sc, part, g.G);
+            if ((rc = run_bicgstab<float>(h, A32, F, sc, std::max(1, itermax - total_inner), 0)))
+                return rc;
+            total_inner += std::max(0, std::min(h->h_sc[0].kout, itermax));
+            ++refinements;
+            // x += ||r|| d ; r = PM*(bs - A x) ; ERR.
+            k_correct<<<cdiv(N, BS), BS, 0, s>>>(N, outer, F.x.p, V.x.p);
+            k_reset<<<1, 64, 0, s>>>(sc, eps, itermax);
+            spmv<double>(s, g, sc, A, 3, V.x.p, h->bs.p, nullptr, nullptr, V.y.p);
+            apply_pm<double>(s, g, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
+            k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, g.G);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    HIPCHK(hipMemcpyAsync(d_x, V.x.p, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipEventRecord(h->ev1, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (info) {
+        float ms = 0.f, ms0 = 0.f;
+        (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+        (void)hipEventElapsedTime(&ms0, h->ev0, h->ev2);
+        info->iters = total_inner;
+        info->ier = ier;
+        info->refinements = refinements;
+        info->err = err;
+        info->ms = ms;
+        info->setup_ms = ms0;
+    }
+    return CLOWNS_OK;
+}
+
+int set_device(clowns_solver *h) {
+    HIPCHK(hipSetDevice(h->device));
+    return CLOWNS_OK;
+}
+
+clowns_solver *default_solver(int dev) {
+    if (dev < 0 || dev >= 64) return nullptr;
+    if (!g_default[dev]) {
+        clowns_solver *h = nullptr;
+        if (clowns_solver_create(&h, dev, nullptr) != CLOWNS_OK) return nullptr;
+        g_default[dev] = h;
+    }
+    return g_default[dev];
+}
+
+} // namespace
+
+// ======================================================================= C-ABI
+
+extern "C" int clowns_solver_create(clowns_solver **out, int device, void *stream) {
+    if (!out) return CLOWNS_EBADARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CLOWNS_EHIP;
+    if (device < 0) {
+        if (hipGetDevice(&device) != hipSuccess) return CLOWNS_EHIP;
+    }
+    if (device >= ndev) return CLOWNS_EBADARG;
+    clowns_solver *h = new clowns_solver();
+    h->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete h;
+        return CLOWNS_EHIP;
+    }
+    if (stream) {
+        h->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete h;
+            return CLOWNS_EHIP;
+        }
+        h->own_stream = true;
+    }
+    if (const char *e = getenv("CLOWNS_ROWS_PER_THREAD")) h->R = std::max(1, atoi(e));
+    if (const char *e = getenv("CLOWNS_POLL")) h->poll = std::max(1, atoi(e));
+    *out = h;
+    return CLOWNS_OK;
+}
+
+extern "C" int clowns_solver_destroy(clowns_solver *h) {
+    if (!h) return CLOWNS_OK;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    h->AL.release(); h->D.release(); h->W.release(); h->LL.release();
+    h->AL32.release(); h->D32.release();
+    h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
+    h->AL0.release(); h->D0.release(); h->LL0.release();
+    h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->sc.release();
+    if (h->h_sc) (void)hipHostFree(h->h_sc);
+    if (h->ev0) { (void)hipEventDestroy(h->ev0); (void)hipEventDestroy(h->ev1); (void)hipEventDestroy(h->ev2); }
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return CLOWNS_OK;
+}
+
+extern "C" int clowns_solver_set_stream(clowns_solver *h, void *stream) {
+    if (!h) return CLOWNS_EBADARG;
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
+    h->own_stream = false;
+    h->stream = (hipStream_t)stream;
+    if (!stream) {
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return CLOWNS_EHIP;
+        h->own_stream = true;
+    }
+    return CLOWNS_OK;
+}
+
+extern "C" int clowns_solver_set_matrix(clowns_solver *h, const double *d_AL, int32_t NL,
+                                        int32_t lda, const int32_t *d_LL, const double *d_D,
+                                        int32_t N, double parm) {
+    if (!h) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    rc = load_matrix(h, d_AL, NL, lda, d_LL, d_D, N, parm, hipMemcpyDeviceToDevice, true);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CLOWNS_OK;
+}
+
+extern "C" int clowns_solver_solve(clowns_solver *h, const double *d_b, double *d_x, double eps,
+                                   int32_t maxiter, int32_t flags, clowns_info *info) {
+    if (!h || !d_b || !d_x || h->N < 1) return CLOWNS_EBADARG;
+    if (flags & ~(CLOWNS_MIXED | CLOWNS_WARM_START)) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    if (eps <= 1e-15) eps = 1e-9;                                        // :51
+    if (maxiter <= 0) maxiter = (int)(10.0 * (std::sqrt((double)h->N) + 1.0)); // :52
+    Op<double> A{h->N, h->NL, h->lda, h->AL.p, h->LL.p, h->D.p};
+    const bool warm = (flags & CLOWNS_WARM_START) != 0;
+    if (flags & CLOWNS_MIXED) {
+        if ((rc = ensure_f32(h))) return rc;
+        Op<float> A32{h->N, h->NL, h->lda, h->AL32.p, h->LL.p, h->D32.p};
+        return solve_mixed(h, A, A32, h->W.p, d_b, d_x, eps, maxiter, warm, info);
+    }
+    return solve_fp64(h, A, h->W.p, d_b, d_x, eps, maxiter, warm, info);
+}
+
+extern "C" int clowns_solver_get_scaled(clowns_solver *h, double *d_AL, int32_t *d_LL,
+                                        double *d_D, double *d_W, double *d_B) {
+    if (!h || h->N < 1) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    const size_t nm = (size_t)h->lda * h->NL;
+    if (d_AL) HIPCHK(hipMemcpyAsync(d_AL, h->AL.p, nm * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (d_LL) HIPCHK(hipMemcpyAsync(d_LL, h->LL.p, nm * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (d_D) HIPCHK(hipMemcpyAsync(d_D, h->D.p, (size_t)h->N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (d_W) HIPCHK(hipMemcpyAsync(d_W, h->W.p, (size_t)h->N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (d_B) {
+        if (h->v64.r.n < (size_t)h->N) return CLOWNS_EBADARG;
+        HIPCHK(hipMemcpyAsync(d_B, h->v64.r.p, (size_t)h->N * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return CLOWNS_OK;
+}
+
+extern "C" void clowns_pcgpme(double *AL, const int32_t *NL_, const int32_t *NA_, int32_t *LL,
+                              double *D, const int32_t *N_, const int32_t *N2_, double *B,
+                              double *X, double *EPS, const double *PARM, int32_t *ITER,
+                              int32_t *IER) {
+    if (!IER) return;
+    if (!AL || !NL_ || !NA_ || !LL || !D || !N_ || !N2_ || !B || !X || !EPS || !PARM || !ITER) {
+        *IER = CLOWNS_EBADARG;
+        return;
+    }
+    const int NL = *NL_, NA = *NA_, N = *N_, N2 = *N2_;
+    if (check_nl(NL)) {
+        *IER = CLOWNS_EBADNL;
+        return;
+    }
+    if (N < 1 || NA < N || N2 < 64) {
+        *IER = CLOWNS_EBADARG;
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        *IER = CLOWNS_EHIP;
+        return;
+    }
+    clowns_solver *h = default_solver(dev);
+    if (!h) {
+        *IER = CLOWNS_EHIP;
+        return;
+    }
+    if (set_device(h)) {
+        *IER = CLOWNS_EHIP;
+        return;
+    }
+    double eps = EPS[0];
+    int itermax = *ITER;
+    if (eps <= 1e-15) eps = 1e-9;                                            // :51
+    if (itermax <= 0) itermax = (int)(10.0 * (std::sqrt((double)N) + 1.0));  // :52
+    // The caller normally passes X = 0 (1sub-momentum.F90:267-271); then the
+    // reference's PM*A*X term is exactly zero and is skipped.
+    bool warm = false;
+    for (int i = 0; i < N && !warm; ++i) warm = X[i] != 0.0;
+    hipStream_t s = h->stream;
+    int rc = load_matrix(h, AL, NL, NA, LL, D, N, *PARM, hipMemcpyHostToDevice, false);
+    if (!rc) rc = h->stage_b.ensure(N);
+    if (!rc) rc = h->stage_x.ensure(N);
+    if (!rc && hipMemcpyAsync(h->stage_b.p, B, (size_t)N * sizeof(double), hipMemcpyHostToDevice, s))
+        rc = CLOWNS_EHIP;
+    if (!rc && warm &&
+        hipMemcpyAsync(h->stage_x.p, X, (size_t)N * sizeof(double), hipMemcpyHostToDevice, s))
+        rc = CLOWNS_EHIP;
+    clowns_info info{};
+    if (!rc) {
+        Op<double> A{N, NL, NA, h->AL.p, h->LL.p, h->D.p};
+        rc = solve_fp64(h, A, h->W.p, h->stage_b.p, h->stage_x.p, eps, itermax, warm, &info);
+    }
+    if (!rc) {
+        const size_t nm = (size_t)NA * NL;
+        if (hipMemcpyAsync(AL, h->AL.p, nm * sizeof(double), hipMemcpyDeviceToHost, s) ||
+            hipMemcpyAsync(LL, h->LL.p, nm * sizeof(int), hipMemcpyDeviceToHost, s) ||
+            hipMemcpyAsync(D, h->D.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s) ||
+            hipMemcpyAsync(B, h->v64.r.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s) ||
+            hipMemcpyAsync(X, h->stage_x.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s) ||
+            hipStreamSynchronize(s))
+            rc = CLOWNS_EHIP;
+    }
+    if (rc) {
+        *IER = rc;
+        return;
+    }
+    *ITER = info.iters;
+    EPS[0] = info.err;
+    *IER = info.ier;
+}
+
+extern "C" int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32_t reps,
+                                         double *avg_ms) {
+    if (!h || h->N < 1 || reps < 1 || !avg_ms) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    const int N = h->N;
+    if ((rc = ensure_vecs(h, h->v64, N))) return rc;
+    if (kernel >= 2) {
+        if ((rc = ensure_f32(h)) || (rc = ensure_vecs(h, h->v32, N))) return rc;
+    }
+    Scal *sc = h->sc.p;
+    k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1);
+    const Geo g = geo_of(N, h->R);
+    Op<double> A{N, h->NL, h->lda, h->AL.p, h->LL.p, h->D.p};
+    Op<float> A32{N, h->NL, h->lda, h->AL32.p, h->LL.p, h->D32.p};
+    auto one = [&]() {
+        switch (kernel) {
+        case 0: spmv<double>(s, g, sc, A, 1, h->v64.p0.p, h->v64.r.p, h->v64.q.p, h->v64.p1.p, h->v64.y.p); break;
+        case 1: launch_pm_stage<double, 1>(s, g, sc, A, 0, h->v64.y.p, h->v64.t.p, 0, nullptr, h->part.p); break;
+        case 2: spmv<float>(s, g, sc, A32, 1, h->v32.p0.p, h->v32.r.p, h->v32.q.p, h->v32.p1.p, h->v32.y.p); break;
+        default: launch_pm_stage<float, 1>(s, g, sc, A32, 0, h->v32.y.p, h->v32.t.p, 0, nullptr, h->part.p); break;
+        }
+    };
+    one();
+    HIPCHK(hipEventRecord(h->ev0, s));
+    for (int k = 0; k < reps; ++k) one();
+    HIPCHK(hipEventRecord(h->ev1, s));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    *avg_ms = (double)ms / reps;
+    return CLOWNS_OK;
+}
+
+namespace {
+// Build and scale/sort A^T from the caller-layout copy kept by set_matrix.
+int ensure_transpose(clowns_solver *h) {
+    if (h->have_t) return CLOWNS_OK;
+    if (h->AL0.n < (size_t)h->lda * h->NL) return CLOWNS_EBADARG;
+    int rc;
+    const int N = h->N, NL = h->NL, lda = h->lda;
+    const size_t nm = (size_t)lda * NL;
+    if ((rc = h->ALt.ensure(nm)) || (rc = h->LLt.ensure(nm)) || (rc = h->Dt.ensure(N)) ||
+        (rc = h->Wt.ensure(N)) || (rc = h->flag.ensure(1)))
+        return rc;
+    hipStream_t s = h->stream;
+    int *bad = h->flag.p;
+    fill<int>(s, bad, 1, 0);
+    k_transpose<<<cdiv(N, BS), BS, 0, s>>>(N, lda, NL, h->AL0.p, h->LL0.p, h->D0.p, h->ALt.p,
+                                          h->LLt.p, h->Dt.p, bad);
+    int hbad = 0;
+    HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hbad) return CLOWNS_ESTRUCT;
+    double P1 = h->parm, P2;
+    if (h->parm < 1.0) P1 = 2.0 * 2.0;
+    P2 = P1 + 1.0;
+    if (NL == 6)
+        k_scale_sort<6><<<cdiv(N, BS), BS, 0, s>>>(N, lda, h->ALt.p, h->LLt.p, h->Dt.p, h->Wt.p, P1, P2);
+    else
+        k_scale_sort<4><<<cdiv(N, BS), BS, 0, s>>>(N, lda, h->ALt.p, h->LLt.p, h->Dt.p, h->Wt.p, P1, P2);
+    HIPCHK(hipGetLastError());
+    h->have_t = true;
+    return CLOWNS_OK;
+}
+} // namespace
+
+extern "C" int clowns_solver_solve_transpose(clowns_solver *h, const double *d_g,
+                                             double *d_lambda, double eps, int32_t maxiter,
+                                             int32_t flags, clowns_info *info) {
+    if (!h || !d_g || !d_lambda || h->N < 1) return CLOWNS_EBADARG;
+    if (flags & ~CLOWNS_WARM_START) return CLOWNS_EBADARG;  // FP64 only
+    int rc = set_device(h);
+    if (rc) return rc;
+    if ((rc = ensure_transpose(h))) return rc;
+    if (eps <= 1e-15) eps = 1e-9;
+    if (maxiter <= 0) maxiter = (int)(10.0 * (std::sqrt((double)h->N) + 1.0));
+    Op<double> At{h->N, h->NL, h->lda, h->ALt.p, h->LLt.p, h->Dt.p};
+    return solve_fp64(h, At, h->Wt.p, d_g, d_lambda, eps, maxiter,
+                      (flags & CLOWNS_WARM_START) != 0, info);
+}
+
+extern "C" int clowns_grad_coefficients(clowns_solver *h, const double *d_lambda,
+                                        const double *d_x, double *d_gAL, double *d_gD) {
+    if (!h || !d_lambda || !d_x || h->N < 1) return CLOWNS_EBADARG;
+    if (h->LL0.n < (size_t)h->lda * h->NL) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    k_grad_coef<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->NL, h->LL0.p, d_lambda,
+                                                      d_x, d_gAL, d_gD);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CLOWNS_OK;
+}
+
+extern "C" const char *clowns_version(void) {
+    return "clowns_pcg gfx950 (PM-Bi-CGSTAB, FP64 + mixed FP32/FP64)";
+}

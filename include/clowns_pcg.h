@@ -1,0 +1,128 @@
+/*
+ * clowns_pcg.h -- C-ABI of the MI355X (gfx950) pressure-Poisson solver that
+ * replaces 2CLOWNS-3D's PCGPME.
+ *
+ * Two layers:
+ *  1. clowns_pcgpme(): the drop-in.  Same argument list, layouts, in/out
+ *     semantics and IER codes as the reference's external subroutine
+ *     PCGPME(AL,NL,NA,LL,D,N,N2,B,X,EPS,PARM,ITER,IER)
+ *     (src2LOWNS/pcgpme_para.F90:1-58, 250-257), every argument passed by
+ *     reference so that a Fortran ISO_C_BINDING interface binds it directly.
+ *     Host arrays in, host arrays out; synchronous.
+ *  2. clowns_solver_*(): the device-resident form of the same solve (matrix
+ *     scaled and sorted once and kept on the GPU across pressure-correction
+ *     passes and time steps, vectors already in HBM), the mixed-precision
+ *     solve and the adjoint (transpose) solve.  Pointers named d_* are device
+ *     pointers.
+ *
+ * Return values of int functions: 0 = success, otherwise a CLOWNS_E* code.
+ * All functions are synchronous on return unless stated otherwise.
+ */
+#ifndef CLOWNS_PCG_H
+#define CLOWNS_PCG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- IER / return codes ------------------------------------------------
+ * 0 and 3000 are the reference's (pcgpme_para.F90:53,251).  The others are
+ * new: the reference STOPs on a bad NL (pcgpme_para.F90:347) and runs on
+ * with NaN where these return early. */
+#define CLOWNS_OK 0
+#define CLOWNS_NOCONV 3000      /* ITER exhausted (reference IER=3000)      */
+#define CLOWNS_ENONFINITE 4001  /* residual became NaN/Inf; early exit       */
+#define CLOWNS_EBADNL 4002      /* NL not 4 or 6                             */
+#define CLOWNS_EBADARG 4003     /* N<1, NA<N, N2<64, null pointer, bad flag  */
+#define CLOWNS_EHIP 4004        /* HIP runtime error / no device             */
+#define CLOWNS_ENOMEM 4005      /* device allocation failed                  */
+#define CLOWNS_ESTRUCT 4006     /* transpose: pattern not structurally symmetric */
+
+/* Replaces PCGPME (src2LOWNS/pcgpme_para.F90:1, argument declarations
+ * :35-46).  AL(NA,NL), LL(NA,NL) column-major, LL 1-based (0 = none);
+ * D, B length N; X length N+N2 (entries past N are treated as zero and never
+ * read or written); EPS[0] tolerance in / achieved ERR out; ITER max in /
+ * iterations out; IER out.  As in the reference, AL and D return scaled, LL
+ * sorted with ghost indices N+1+mod(I,64), B the final scaled residual. */
+void clowns_pcgpme(double *AL, const int32_t *NL, const int32_t *NA,
+                   int32_t *LL, double *D, const int32_t *N, const int32_t *N2,
+                   double *B, double *X, double *EPS, const double *PARM,
+                   int32_t *ITER, int32_t *IER);
+
+/* ---- device-resident solver ------------------------------------------- */
+typedef struct clowns_solver clowns_solver;
+
+typedef struct clowns_info {
+    int32_t iters;        /* Bi-CGSTAB iterations (FP64), or inner total (mixed) */
+    int32_t ier;          /* CLOWNS_OK / CLOWNS_NOCONV / CLOWNS_ENONFINITE       */
+    int32_t refinements;  /* mixed: FP64 refinement steps                        */
+    int32_t reserved;
+    double err;           /* final sqrt(RN/BN), the reference's stopping value  */
+    double ms;            /* device time of the solve (HIP events), ms          */
+    double setup_ms;      /* part of ms spent before the first iteration        */
+} clowns_info;
+
+/* Solve flags */
+#define CLOWNS_MIXED 1       /* FP32 inner PM-Bi-CGSTAB + FP64 refinement   */
+#define CLOWNS_WARM_START 2  /* d_x holds the initial guess (else zero)      */
+
+/* Create a solver bound to `device` (-1 = current).  `stream` is a
+ * hipStream_t, or NULL for a library-owned stream. */
+int clowns_solver_create(clowns_solver **out, int device, void *stream);
+int clowns_solver_destroy(clowns_solver *h);
+int clowns_solver_set_stream(clowns_solver *h, void *stream);
+
+/* Load the operator from device arrays in the reference layout (AL/LL with
+ * column stride lda >= N, LL 1-based, D length N).  Scales and sorts a copy
+ * on the device (pcgpme_para.F90:77-112); the caller's arrays are not
+ * modified.  The scaled/sorted matrix stays resident until the next call. */
+int clowns_solver_set_matrix(clowns_solver *h, const double *d_AL, int32_t NL,
+                             int32_t lda, const int32_t *d_LL,
+                             const double *d_D, int32_t N, double parm);
+
+/* Solve A x = b for the resident operator to the reference's stopping rule
+ * sqrt(RN/BN) <= eps (pcgpme_para.F90:215-232).  d_b unscaled, length N;
+ * d_x length N (in: initial guess when CLOWNS_WARM_START).  maxiter <= 0
+ * gives the reference default 10*(sqrt(N)+1). */
+int clowns_solver_solve(clowns_solver *h, const double *d_b, double *d_x,
+                        double eps, int32_t maxiter, int32_t flags,
+                        clowns_info *info);
+
+/* Copy of the resident scaled/sorted operator (reference post-call layout,
+ * column stride lda of set_matrix), the row scales W, and the final scaled
+ * residual of the last solve.  Any pointer may be NULL. */
+int clowns_solver_get_scaled(clowns_solver *h, double *d_AL, int32_t *d_LL,
+                             double *d_D, double *d_W, double *d_B);
+
+/* Adjoint: solve A^T lambda = g for the resident operator's transpose
+ * (the operator is non-symmetric; its pattern must be structurally
+ * symmetric), with the same PM-Bi-CGSTAB and stopping rule.  d_g, d_lambda
+ * length N. */
+int clowns_solver_solve_transpose(clowns_solver *h, const double *d_g,
+                                  double *d_lambda, double eps,
+                                  int32_t maxiter, int32_t flags,
+                                  clowns_info *info);
+
+/* Gradients of a loss through x = A^{-1} b, given lambda = A^{-T} dL/dx and
+ * the solution x, in the caller's (unscaled, unsorted) layout of the operator
+ * last passed to set_matrix:
+ *   dL/db = lambda;  dL/dAL(i,k) = -lambda_i * x(LL(i,k));  dL/dD = -lambda_i x_i.
+ * d_gAL uses column stride lda; entries with LL = 0 get 0. */
+int clowns_grad_coefficients(clowns_solver *h, const double *d_lambda,
+                             const double *d_x, double *d_gAL, double *d_gD);
+
+/* Time `reps` back-to-back launches of one solver kernel over the resident
+ * operator with HIP events on the solver's stream: kernel 0 = FP64 SpMV,
+ * 1 = FP64 PM stage, 2 = FP32 SpMV, 3 = FP32 PM stage.  *avg_ms out. */
+int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32_t reps,
+                              double *avg_ms);
+
+/* Library build string, e.g. "clowns_pcg gfx950 ...". */
+const char *clowns_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLOWNS_PCG_H */

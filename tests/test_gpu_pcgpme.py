@@ -1,0 +1,158 @@
+"""GPU parity of the drop-in clowns_pcgpme against the C oracle of PCGPME.
+
+Bar (all FP64): the scaling/sort outputs AL, LL, D are bit-exact (the kernel
+evaluates the reference's expressions in the same order without FMA); after
+ITER = 1, 2, 5 the iterate X and residual B agree to 1e-12 relative (the only
+difference is the summation order of the dot products); at convergence the
+iteration count and exit code match and X agrees to 1e-9 relative L2
+(the north-star tolerance at the reference's 1e-10 stopping rule).
+"""
+import numpy as np
+import pytest
+
+from conftest import apply_A, call_oracle, copy_system
+from _systems import box_system
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(s, iters=10000, eps=1e-10, x0=None):
+    import clowns3d
+    c = copy_system(s)
+    N = c["N"]
+    X = np.zeros(N + 64) if x0 is None else np.concatenate([x0, np.zeros(64)])
+    EPS = np.array([eps, 0.0])
+    it, ier = clowns3d.pcgpme(c["AL"], c["NL"], c["NA"], c["LL"], c["D"], N, 64, c["B"], X, EPS, 5.0, iters)
+    c.update(X=X, EPS=EPS, ITER=it, IER=ier)
+    return c
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+SIZES = [(12, 10, 8), (33, 17, 9), (64, 48, 16)]
+
+
+@pytest.mark.parametrize("shape", SIZES)
+def test_scale_sort_bit_exact(shape):
+    s = box_system(*shape, seed=3)
+    g = run_gpu(s, iters=1)
+    o = call_oracle(s, iters=1)
+    N = s["N"]
+    assert np.array_equal(g["LL"][:N], o["LL"][:N])
+    assert np.array_equal(g["AL"][:N], o["AL"][:N])
+    assert np.array_equal(g["D"], o["D"])
+    # rows past N are never touched
+    assert np.array_equal(g["AL"][N:], s["AL"][N:])
+
+
+@pytest.mark.parametrize("iters", [1, 2, 5])
+def test_first_iterations_match(iters):
+    s = box_system(33, 17, 9, seed=4)
+    g = run_gpu(s, iters=iters)
+    o = call_oracle(s, iters=iters)
+    assert g["IER"] == o["IER"] == 3000
+    assert g["ITER"] == o["ITER"] == iters + 1
+    assert rel(g["X"][: s["N"]], o["X"][: s["N"]]) < 1e-12
+    assert rel(g["B"], o["B"]) < 1e-12
+    assert abs(g["EPS"][0] - o["EPS"][0]) <= 1e-12 * o["EPS"][0]
+
+
+@pytest.mark.parametrize("shape", SIZES)
+def test_converged_solution(shape):
+    s = box_system(*shape, seed=5)
+    g = run_gpu(s)
+    o = call_oracle(s)
+    N = s["N"]
+    assert g["IER"] == o["IER"] == 0
+    assert abs(g["ITER"] - o["ITER"]) <= 1
+    assert g["EPS"][0] <= 1e-10
+    assert rel(g["X"][:N], o["X"][:N]) < 1e-9
+    # and it solves the unscaled system
+    r = apply_A(s, g["X"]) - s["B"]
+    assert np.linalg.norm(r) / np.linalg.norm(s["B"]) < 1e-6
+
+
+def test_warm_start_and_reference_defaults():
+    s = box_system(20, 12, 6, seed=6)
+    o = call_oracle(s)
+    x0 = o["X"][: s["N"]] * (1 + 1e-3)
+    g = run_gpu(s, x0=x0)
+    o2 = call_oracle(s, x0=x0)
+    assert g["IER"] == o2["IER"] == 0
+    assert abs(g["ITER"] - o2["ITER"]) <= 1
+    assert rel(g["X"][: s["N"]], o2["X"][: s["N"]]) < 1e-9
+    # EPS <= 1e-15 -> 1e-9, ITER <= 0 -> 10*(sqrt(N)+1) (pcgpme_para.F90:51-52)
+    g0 = run_gpu(s, iters=0, eps=0.0)
+    o0 = call_oracle(s, iters=0, eps=0.0)
+    assert g0["IER"] == o0["IER"]
+    assert abs(g0["ITER"] - o0["ITER"]) <= 1
+    assert g0["EPS"][0] <= 1e-9
+
+
+def test_zero_rhs_returns_zero():
+    s = box_system(10, 8, 6, seed=7)
+    s["B"] = np.zeros_like(s["B"])
+    g = run_gpu(s)
+    assert g["IER"] == 0 and g["ITER"] == 0
+    assert np.all(g["X"] == 0.0)
+
+
+def test_bad_arguments():
+    import clowns3d
+    s = box_system(6, 5, 4, seed=8)
+    c = copy_system(s)
+    N = c["N"]
+    X = np.zeros(N + 64)
+    EPS = np.array([1e-10, 0.0])
+    # NL = 5: the reference STOPs (pcgpme_para.F90:347); here IER = 4002
+    AL5 = np.asfortranarray(c["AL"][:, :5].copy())
+    LL5 = np.asfortranarray(c["LL"][:, :5].copy())
+    _, ier = clowns3d.pcgpme(AL5, 5, c["NA"], LL5, c["D"], N, 64, c["B"], X, EPS, 5.0, 100)
+    assert ier == 4002
+    # an out-of-range neighbour index is rejected before any gather
+    bad = c["LL"].copy(order="F")
+    bad[3, 2] = N + 7
+    _, ier = clowns3d.pcgpme(c["AL"], 6, c["NA"], bad, c["D"], N, 64, c["B"], X, EPS, 5.0, 100)
+    assert ier == 4003
+    # N2 < 64 would let the ghost indices run past X
+    X2 = np.zeros(N + 10)
+    _, ier = clowns3d.pcgpme(c["AL"], 6, c["NA"], c["LL"], c["D"], N, 10, c["B"], X2, EPS, 5.0, 100)
+    assert ier == 4003
+
+
+def test_nl4_vertical_2d():
+    # a 2DV operator: drop the y slots (NL=4, slots x-,x+,z-,z+)
+    s = box_system(40, 1, 12, seed=9)
+    s4 = dict(s)
+    s4["AL"] = np.asfortranarray(s["AL"][:, [0, 1, 4, 5]])
+    s4["LL"] = np.asfortranarray(s["LL"][:, [0, 1, 4, 5]])
+    s4["NL"] = 4
+    g = run_gpu(s4)
+    o = call_oracle(s4)
+    assert g["IER"] == o["IER"] == 0
+    assert abs(g["ITER"] - o["ITER"]) <= 1
+    assert rel(g["X"][: s["N"]], o["X"][: s["N"]]) < 1e-9
+
+
+def test_resident_matches_boundary():
+    import torch
+    from clowns3d import Solver
+    s = box_system(33, 17, 9, seed=10)
+    g = run_gpu(s)
+    dev = torch.device("cuda:0")
+    AL = torch.from_numpy(np.ascontiguousarray(s["AL"].T)).to(dev)
+    LL = torch.from_numpy(np.ascontiguousarray(s["LL"].T)).to(dev)
+    D = torch.from_numpy(s["D"]).to(dev)
+    b = torch.from_numpy(s["B"]).to(dev)
+    sol = Solver(0)
+    sol.set_matrix(AL, LL, D)
+    x, info = sol.solve(b)
+    torch.cuda.synchronize()
+    assert info["ier"] == 0 and info["iters"] == g["ITER"]
+    assert np.array_equal(x.cpu().numpy(), g["X"][: s["N"]])
+    # repeated solves on the resident operator are deterministic
+    x2, _ = sol.solve(b)
+    assert torch.equal(x, x2)
+    sol.close()
