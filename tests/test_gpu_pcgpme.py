@@ -3,9 +3,9 @@
 Bar (all FP64): the scaling/sort outputs AL, LL, D are bit-exact (the kernel
 evaluates the reference's expressions in the same order without FMA); after
 ITER = 1, 2, 5 the iterate X and residual B agree to 1e-12 relative (the only
-difference is the summation order of the dot products); at convergence the
-iteration count and exit code match and X agrees to 1e-9 relative L2
-(the north-star tolerance at the reference's 1e-10 stopping rule).
+difference is the summation order of the dot products); at convergence
+the exit code matches and X and the iteration count sit within the spread the
+reference itself shows across OpenMP thread counts (test_converged_solution).
 """
 import numpy as np
 import pytest
@@ -61,14 +61,27 @@ def test_first_iterations_match(iters):
 
 @pytest.mark.parametrize("shape", SIZES)
 def test_converged_solution(shape):
+    """At the 1e-10 stopping rule the Bi-CGSTAB trajectory depends on the
+    summation order of its dot products: the reference itself, run at 1, 2, 4
+    and 8 OpenMP threads, ends at different iteration counts and different X.
+    So the GPU solve is judged against a tightly converged solution
+    (EPS = 2e-15, stagnation exit) and against that spread: its error must be
+    within 1e-9 relative L2 or no worse than 2x the reference's worst, and its
+    iteration count inside the reference's range widened by 10%."""
     s = box_system(*shape, seed=5)
-    g = run_gpu(s)
-    o = call_oracle(s)
     N = s["N"]
-    assert g["IER"] == o["IER"] == 0
-    assert abs(g["ITER"] - o["ITER"]) <= 1
+    g = run_gpu(s)
+    refs = {nt: call_oracle(s, nthr=nt) for nt in (1, 2, 4, 8)}
+    tight = call_oracle(s, eps=2e-15)["X"][:N]
+    err = lambda x: np.linalg.norm(x[:N] - tight) / np.linalg.norm(tight)
+    ref_err = max(err(r["X"]) for r in refs.values())
+    its = [r["ITER"] for r in refs.values()]
+    print(f"{shape}: gpu ITER {g['ITER']} err {err(g['X']):.2e} | ref ITER {its} worst err {ref_err:.2e} "
+          f"| gpu vs ref(1 thread) {rel(g['X'][:N], refs[1]['X'][:N]):.2e}")
+    assert g["IER"] == 0 and all(r["IER"] == 0 for r in refs.values())
     assert g["EPS"][0] <= 1e-10
-    assert rel(g["X"][:N], o["X"][:N]) < 1e-9
+    assert err(g["X"]) <= max(1e-9, 2.0 * ref_err)
+    assert 0.9 * min(its) - 1 <= g["ITER"] <= 1.1 * max(its) + 1
     # and it solves the unscaled system
     r = apply_A(s, g["X"]) - s["B"]
     assert np.linalg.norm(r) / np.linalg.norm(s["B"]) < 1e-6
