@@ -269,6 +269,47 @@ k_pm(const Scal *__restrict__ sc, int N, int R, const T *__restrict__ a0,
     }
 }
 
+// Two consecutive single-term stages in one pass:
+//   o1_i = in_i - a0_i*in[l0_i];  out_i = o1_i - a1_i*o1[l1_i]
+// with o1 at the gather point m = l1_i recomputed from its own row (m a ghost
+// row reads zero, as the reference's zeroed work array W does).  The values
+// are bit-identical to two separate passes; the intermediate never goes to
+// HBM, and row m's a0/l0 entries are L2 hits (m is a grid neighbour).
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_pm2(const Scal *__restrict__ sc, int N, int R, const T *__restrict__ a0,
+      const int *__restrict__ l0, const T *__restrict__ a1,
+      const int *__restrict__ l1, const T *__restrict__ in,
+      T *__restrict__ out) {
+    if (sc->done) return;
+    const int base = logical_block() * BS * R + threadIdx.x;
+    for (int rr = 0; rr < R; ++rr) {
+        const int i = base + rr * BS;
+        if (i >= N) return;
+        const int j0 = l0[i] - 1, m = l1[i] - 1;
+        const T c0 = a0[i], c1 = a1[i];
+        const T o1i = in[i] - c0 * in[j0];
+        T o1m = T(0);
+        if (m < N) {
+            const int jm = l0[m] - 1;
+            o1m = in[m] - a0[m] * in[jm];
+        }
+        out[i] = o1i - c1 * o1m;
+    }
+}
+
+// P = R + BETA*(P - AMU*Q) (pcgpme_para.F90:244), in place.
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_pupd(const Scal *__restrict__ sc, int N, T *__restrict__ p,
+       const T *__restrict__ r, const T *__restrict__ q) {
+    if (sc->done) return;
+    const int i = logical_block() * BS + threadIdx.x;
+    if (i >= N) return;
+    const T beta = (T)sc->BETA, amu = (T)sc->AMU;
+    p[i] = r[i] + beta * (p[i] - amu * q[i]);
+}
+
 // ---------------------------------------------------------- update (a11) --
 // X = X + ALP*P + AMU*E;  R = E - AMU*V;  RN += R*R;  C1 += R*R^.
 template <typename T>
@@ -333,10 +374,20 @@ k_init(const Scal *__restrict__ sc, int N, int R, T *__restrict__ wk1,
 }
 
 // ------------------------------------------------------------ finalize --
+// Loads are issued four at a time (independent) before the fixed-order adds:
+// the partials are few and L2-resident, so latency, not bandwidth, bounds this.
 __device__ __forceinline__ double fin_sum(const double *__restrict__ p, int G) {
     __shared__ double lds[FIN_BS / 64];
     double s = 0.0;
-    for (int k = threadIdx.x; k < G; k += FIN_BS) s += p[k];
+    int k = threadIdx.x;
+    for (; k + 3 * FIN_BS < G; k += 4 * FIN_BS) {
+        const double a = p[k], b = p[k + FIN_BS], c = p[k + 2 * FIN_BS], d = p[k + 3 * FIN_BS];
+        s += a;
+        s += b;
+        s += c;
+        s += d;
+    }
+    for (; k < G; k += FIN_BS) s += p[k];
     return block_sum<FIN_BS>(s, lds);
 }
 
