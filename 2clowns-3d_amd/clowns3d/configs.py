@@ -1,0 +1,186 @@
+"""Synthetic 2CLOWNS-3D grids of the benchmark configurations.
+
+A grid is a box of cells i=0..nx+1, j=0..ny+1, k=0..nz+1 whose outer ring is
+solid (closed walls), stored flat with index c = i + X*(j + Y*k), X = nx+2,
+Y = ny+2 -- the reference's k-major row order (src2LOWNS/nfd2015.F90:69-78).
+Fields, as the reference holds them per cell (src2LOWNS/0mod-arrays.F90):
+  solid  int8   1 = wall/object cell (nf = -1, nfb = 0, so mn = 0)
+  F      f64    VOF fill
+  fb     f64    cell open ratio
+  a      f64    (3, ncell) open ratio of the LOW face of the cell in x, y, z
+  un     f64    (3, ncell) velocity on that face
+  dx, xf f64    (3, stride) cell sizes and low-face coordinates per axis
+How F, fb and a follow from the water level and the bed follows the grid
+preprocessor (srcMakeGrid/set_landform.F90:25-92): a_x = min(fb_i, fb_{i-1})
+and a_z = 1 between open cells, 0 against a solid cell; in a partial bed cell
+F = (h - z_{k+1} + fb dz)/(dz fb).
+"""
+import math
+
+import numpy as np
+
+
+class Grid:
+    def __init__(self, nx, ny, nz, dx=1.0, dy=1.0, dz=0.5, dt=0.01):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.X, self.Y, self.Z = nx + 2, ny + 2, nz + 2
+        self.ncell = self.X * self.Y * self.Z
+        self.dt = dt
+        self.stride = max(nx, ny, nz) + 2
+        self.h = (dx, dy, dz)
+        self.dx = np.zeros((3, self.stride))
+        self.xf = np.zeros((3, self.stride))
+        for l, (n, h) in enumerate(zip((nx, ny, nz), self.h)):
+            self.dx[l, : n + 2] = h
+            # x(l, i) = (i-1)*h: low face of cell i; the ring cell 0 starts at -h
+            self.xf[l, : n + 2] = (np.arange(n + 2) - 1) * h
+        self.solid = np.zeros((self.Z, self.Y, self.X), np.int8)
+        self.solid[0, :, :] = self.solid[-1, :, :] = 1
+        self.solid[:, 0, :] = self.solid[:, -1, :] = 1
+        self.solid[:, :, 0] = self.solid[:, :, -1] = 1
+        self.F = np.zeros((self.Z, self.Y, self.X))
+        self.fb = np.ones((self.Z, self.Y, self.X))
+        self.a = np.zeros((3, self.Z, self.Y, self.X))
+        self.un = np.zeros((3, self.Z, self.Y, self.X))
+        self.name = ""
+
+    # cell-centre coordinates of the interior, shaped (Y-2, X-2)
+    def centres(self):
+        hx, hy, _ = self.h
+        xc = (np.arange(1, self.nx + 1) - 0.5) * hx
+        yc = (np.arange(1, self.ny + 1) - 0.5) * hy
+        return np.meshgrid(xc, yc, indexing="xy")
+
+    def set_faces(self):
+        """a on every low face: min of the two cells' fb in x and y, 1 in z,
+        0 where either side is solid (set_landform.F90:25-41)."""
+        open_ = self.solid == 0
+        fb = np.where(open_, self.fb, 0.0)
+        self.a[:] = 0.0
+        both = open_[:, :, 1:] & open_[:, :, :-1]
+        self.a[0][:, :, 1:] = np.where(both, np.minimum(fb[:, :, 1:], fb[:, :, :-1]), 0.0)
+        both = open_[:, 1:, :] & open_[:, :-1, :]
+        self.a[1][:, 1:, :] = np.where(both, np.minimum(fb[:, 1:, :], fb[:, :-1, :]), 0.0)
+        both = open_[1:, :, :] & open_[:-1, :, :]
+        self.a[2][1:, :, :] = np.where(both, 1.0, 0.0)
+
+    def set_water(self, zs):
+        """F from the water level zs(y, x) over the open part of each cell."""
+        dz = self.h[2]
+        k = np.arange(1, self.nz + 1)[:, None, None]
+        zlo, zhi = (k - 1) * dz, k * dz
+        zs3 = zs[None, :, :]
+        fb = self.fb[1:-1, 1:-1, 1:-1]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            part = (zs3 - zhi + fb * dz) / (dz * fb)
+        F = np.where(zlo >= zs3, 0.0, np.where(zhi <= zs3, 1.0,
+                     np.where(fb > (zhi - zs3) / dz, np.clip(part, 0.0, 1.0), 0.0)))
+        F = np.where(self.solid[1:-1, 1:-1, 1:-1] == 1, 0.0, F)
+        self.F[1:-1, 1:-1, 1:-1] = F
+
+    def set_velocity(self, seed, amp=0.1):
+        rng = np.random.default_rng(seed)
+        self.un[:] = rng.uniform(-amp, amp, size=self.un.shape)
+
+    def arrays(self):
+        """Flat C-contiguous copies in the box order c = i + X*(j + Y*k).
+        Solid cells carry F = fb = 0 and closed low faces, as the reference's
+        sentinel cell 0 does (src2LOWNS/data_io.F90:274-277)."""
+        sol = self.solid == 1
+        self.F[sol] = 0.0
+        self.fb[sol] = 0.0
+        self.a[:, sol] = 0.0
+        return dict(solid=np.ascontiguousarray(self.solid.ravel()),
+                    F=np.ascontiguousarray(self.F.ravel()),
+                    fb=np.ascontiguousarray(self.fb.ravel()),
+                    a=np.ascontiguousarray(self.a.reshape(3, -1)),
+                    un=np.ascontiguousarray(self.un.reshape(3, -1)),
+                    dx=np.ascontiguousarray(self.dx), xf=np.ascontiguousarray(self.xf))
+
+
+def wavy_surface(g):
+    """z_s = 0.6 H + 0.3 sin(2 pi x / Lx) cos(2 pi y / Ly) at cell centres."""
+    H = g.nz * g.h[2]
+    xc, yc = g.centres()
+    Lx, Ly = g.nx * g.h[0], g.ny * g.h[1]
+    return 0.6 * H + 0.3 * np.sin(2 * math.pi * xc / Lx) * np.cos(2 * math.pi * yc / Ly)
+
+
+def config_wavy_box(nx, ny, nz, box, seed=2024):
+    """Configurations 1 and 2: wavy free surface at 0.6 H, a solid box obstacle
+    over cells box = (i0, i1, j0, j1, k0, k1) (0-based, half-open), face open
+    ratios 1 elsewhere, RHS from a seeded U(-0.1, 0.1) face velocity field."""
+    g = Grid(nx, ny, nz)
+    i0, i1, j0, j1, k0, k1 = box
+    g.solid[1 + k0:1 + k1, 1 + j0:1 + j1, 1 + i0:1 + i1] = 1
+    g.fb[:] = 1.0
+    g.set_faces()
+    g.set_water(wavy_surface(g))
+    g.set_velocity(seed)
+    g.name = f"wavy{nx}x{ny}x{nz}"
+    return g
+
+
+def config1(seed=2024):
+    return config_wavy_box(64, 64, 32, (24, 40, 24, 40, 0, 12), seed)
+
+
+def config2(seed=2024):
+    return config_wavy_box(256, 256, 64, (96, 160, 96, 160, 0, 24), seed)
+
+
+def config_bathymetry(nx, ny, nz, nbumps=40, seed=2025, width=(5, 30), height=(0.1, 0.4)):
+    """Configuration 3: bed = sum of seeded Gaussian bumps (height U(0.1,0.4) H,
+    width U(5,30) cells); partial bed cells get fb = (z_top - z_b)/dz clipped
+    to [0.05, 1] (below 0.05 the cell is solid), a_x, a_y = min(fb) and a_z = 1
+    between open cells; wavy free surface at 0.6 H."""
+    g = Grid(nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    H = nz * g.h[2]
+    xc, yc = g.centres()
+    zb = np.zeros_like(xc)
+    for _ in range(nbumps):
+        cx, cy = rng.uniform(0, nx * g.h[0]), rng.uniform(0, ny * g.h[1])
+        hgt = rng.uniform(*height) * H
+        w = rng.uniform(*width) * g.h[0]
+        zb += hgt * np.exp(-((xc - cx) ** 2 + (yc - cy) ** 2) / (2 * w * w))
+    dz = g.h[2]
+    k = np.arange(1, nz + 1)[:, None, None]
+    frac = (k * dz - zb[None]) / dz
+    fb = np.clip(frac, 0.0, 1.0)
+    solid_in = fb < 0.05
+    g.solid[1:-1, 1:-1, 1:-1] |= solid_in.astype(np.int8)
+    g.fb[1:-1, 1:-1, 1:-1] = np.where(solid_in, 0.0, np.clip(fb, 0.05, 1.0))
+    g.fb[g.solid == 1] = 0.0
+    g.set_faces()
+    g.set_water(wavy_surface(g))
+    g.set_velocity(seed + 1)
+    g.name = f"bathy{nx}x{ny}x{nz}"
+    return g
+
+
+def config3(seed=2025):
+    return config_bathymetry(512, 512, 96, seed=seed)
+
+
+def config_solitary(nx, ny, nz, t, seed=2026, amp_frac=0.1, x0_frac=0.2, g_acc=9.80665):
+    """Configuration 5 at time t: flat bed, still-water depth H = 0.6 * height,
+    surface lifted by a solitary wave eta = A sech^2(kappa (x - x0 - c t)),
+    A = 0.1 H, c = sqrt(g H), kappa = sqrt(3 A / (4 H^3))."""
+    g = Grid(nx, ny, nz)
+    H = 0.6 * nz * g.h[2]
+    A = amp_frac * H
+    c = math.sqrt(g_acc * H)
+    kap = math.sqrt(3 * A / (4 * H ** 3))
+    xc, _ = g.centres()
+    x0 = x0_frac * nx * g.h[0]
+    eta = A / np.cosh(kap * (xc - x0 - c * t)) ** 2
+    g.fb[:] = 1.0
+    g.set_faces()
+    g.set_water(H + eta)
+    g.set_velocity(seed)
+    g.name = f"solitary{nx}x{ny}x{nz}@t={t:g}"
+    return g
+
+
+CONFIGS = {1: config1, 2: config2, 3: config3}

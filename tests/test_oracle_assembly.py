@@ -1,0 +1,122 @@
+"""Pins the C oracle of the system assembly (oracle/oracle_assembly.c):
+classification (set_nflag), row numbering, init_matrix/cal_matrix_element
+coefficients and the RHS, bit for bit against the flang build of the
+reference's own routines (oracle/_ref/libref_assembly.so, driven by
+oracle/ref_glue_assembly.F90) on small instances of the benchmark
+configurations; plus closed-form checks of the uniform-grid coefficients."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REF_ASM_SO, oracle, ptr
+from clowns3d import configs
+
+
+def build_oracle(g, lda=None):
+    A = g.arrays()
+    nc = g.ncell
+    lda = lda or nc
+    out = dict(nf=np.zeros(nc, np.int8), nb=np.zeros(nc, np.int8), row2cell=np.zeros(nc, np.int32),
+               mno=np.zeros(nc, np.int32), AL=np.zeros((6, lda)), LL=np.zeros((6, lda), np.int32),
+               D=np.zeros(lda), B=np.zeros(lda))
+    o = oracle()
+    o.oracle_build_system.restype = ctypes.c_int
+    N = o.oracle_build_system(g.nx, g.ny, g.nz, ptr(A["solid"]), ptr(A["F"]), ptr(A["fb"]), ptr(A["a"]),
+                              ptr(A["un"]), ptr(A["dx"]), ptr(A["xf"]), g.stride, ctypes.c_double(g.dt),
+                              ptr(out["nf"]), ptr(out["nb"]), ptr(out["row2cell"]), ptr(out["mno"]), lda,
+                              ptr(out["AL"]), ptr(out["LL"]), ptr(out["D"]), ptr(out["B"]))
+    assert N > 0
+    out["N"] = N
+    out["lda"] = lda
+    return out
+
+
+_refasm = None
+
+
+def ref_asm():
+    global _refasm
+    if _refasm is None:
+        if not os.path.exists(REF_ASM_SO):
+            pytest.skip("flang build of the reference assembly (oracle/_ref) not present")
+        _refasm = ctypes.CDLL(REF_ASM_SO, mode=os.RTLD_LAZY)
+    return _refasm
+
+
+def build_ref(g):
+    A = g.arrays()
+    nc = g.ncell
+    lda = nc
+    out = dict(nf=np.zeros(nc, np.int8), nb=np.zeros(nc, np.int8), row2cell=np.zeros(nc, np.int32),
+               AL=np.zeros((6, lda)), LL=np.zeros((6, lda), np.int32), D=np.zeros(lda), B=np.zeros(lda))
+    n = ctypes.c_int(0)
+    mp = ctypes.c_double(0)
+    ref_asm().ref_build_system(g.nx, g.ny, g.nz, ptr(A["solid"]), ptr(A["F"]), ptr(A["fb"]), ptr(A["a"]),
+                               ptr(A["un"]), ptr(A["dx"]), ptr(A["xf"]), g.stride, ctypes.c_double(g.dt),
+                               ptr(out["nf"]), ptr(out["nb"]), ptr(out["row2cell"]), lda, ptr(out["AL"]),
+                               ptr(out["LL"]), ptr(out["D"]), ptr(out["B"]), ctypes.byref(n), ctypes.byref(mp))
+    out["N"] = n.value
+    out["mp"] = mp.value
+    return out
+
+
+GRIDS = {
+    "wavy20x16x12": lambda: configs.config_wavy_box(20, 16, 12, (6, 11, 5, 10, 0, 4), seed=7),
+    "wavy32x24x16": lambda: configs.config_wavy_box(32, 24, 16, (8, 20, 8, 16, 0, 6), seed=8),
+    "bathy28x24x16": lambda: configs.config_bathymetry(28, 24, 16, nbumps=6, seed=9, width=(2, 5)),
+    "solitary40x8x12": lambda: configs.config_solitary(40, 8, 12, t=0.7, seed=10),
+}
+
+
+@pytest.mark.parametrize("name", sorted(GRIDS))
+def test_assembly_matches_reference(name):
+    g = GRIDS[name]()
+    r = build_ref(g)
+    o = build_oracle(g)
+    N = r["N"]
+    assert o["N"] == N
+    assert np.array_equal(o["nf"], r["nf"])
+    assert np.array_equal(o["nb"], r["nb"])
+    assert np.array_equal(o["row2cell"][:N], r["row2cell"][:N])
+    assert np.array_equal(o["LL"][:, :N], r["LL"][:, :N])
+    assert np.array_equal(o["AL"][:, :N], r["AL"][:, :N])
+    assert np.array_equal(o["D"][:N], r["D"][:N])
+    assert np.array_equal(o["B"][:N], r["B"][:N])
+    # every class of row occurs
+    nf = o["nf"][o["row2cell"][:N]]
+    assert (nf == 1).any() and (nf == 2).any()
+
+
+def test_uniform_coefficients_closed_form():
+    """dx=dy=1, dz=0.5, a=fb=1: interior fluid row ALU_x=-0.25, ALU_z=-1,
+    D=3 (1sub-matrix.F90:50-61, mp = 0.5^2); a surface row has D=1, the real
+    coefficient towards the fluid below -(fss-1/2)/(fss+1/2), the others -1e-20."""
+    g = configs.config_wavy_box(20, 16, 12, (6, 11, 5, 10, 0, 4), seed=7)
+    o = build_oracle(g)
+    N, X, Y = o["N"], g.X, g.Y
+    cells = o["row2cell"][:N]
+    nf = o["nf"][cells]
+    k = cells // (X * Y)
+    interior = (nf == 1) & (k >= 2) & (k <= 3)
+    rows = np.nonzero(interior)[0]
+    ok = 0
+    for r in rows:
+        if (o["LL"][:, r] > 0).all():
+            assert list(o["AL"][:, r]) == [-0.25, -0.25, -0.25, -0.25, -1.0, -1.0]
+            assert o["D"][r] == 3.0
+            ok += 1
+    assert ok > 100
+    surf = np.nonzero(nf == 2)[0]
+    for r in surf[:50]:
+        assert o["D"][r] == 1.0
+        c = cells[r]
+        b = o["nb"][c]
+        if b == -3 and o["LL"][4, r] > 0:
+            zs = g.arrays()["F"][c] * 0.5  # sp - x_low = F*dz for b = -3 with fb = 1
+            fss = zs / 0.5
+            expect = (fss - 0.5) * 0.5 / (fss * 0.5 + 0.5 * 0.5) * (-1.0)
+            assert abs(o["AL"][4, r] - expect) <= 1e-15
+            others = [o["AL"][s, r] for s in (0, 1, 2, 3) if o["LL"][s, r] > 0]
+            assert all(v == -1e-20 for v in others)
