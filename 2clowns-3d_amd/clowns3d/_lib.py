@@ -79,18 +79,26 @@ _lib = None
 def lib():
     """The loaded library; raises if it was not built."""
     global _lib
+    if _lib is not None and len(_lib._clowns_bound) != len(SIGNATURES):
+        _bind(_lib)
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it (python -c 'import __graft_entry__ as g; g.build()' "
                 "or make -C 2clowns-3d_amd); there is no CPU fallback")
         L = ctypes.CDLL(LIB_PATH)
-        for name, (res, args) in SIGNATURES.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
+        L._clowns_bound = set()
+        _bind(L)
         _lib = L
     return _lib
+
+
+def _bind(L):
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+        L._clowns_bound.add(name)
 
 
 def check(rc, what):

@@ -1,0 +1,120 @@
+"""Classification (host, C++) and device assembly of the pressure system
+through the C-ABI (include/clowns_pcg.h: clowns_classify, clowns_assemble,
+clowns_solver_assemble).
+
+``DeviceGrid(g)`` classifies a configs.Grid on the host (set_nflag, row
+numbering, surface positions), uploads the fields to the GPU once and can then
+assemble the PCGPME inputs there -- either in the reference layout
+(``assemble``) or fused with the solver's scaling/sort straight into a
+resident ``Solver`` (``assemble_into``).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("stride", ctypes.c_int32), ("dt", ctypes.c_double)]
+
+
+class GridDev(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("stride", ctypes.c_int32), ("nrows", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("dt", ctypes.c_double), ("mp", ctypes.c_double)] + [
+                   (n, ctypes.c_void_p) for n in ("nf", "nb", "mno", "row2cell", "sp_row", "F", "fb", "a", "un",
+                                                  "dx", "xf")]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+L.SIGNATURES.update({
+    "clowns_classify": (ctypes.c_int, [ctypes.POINTER(GridDesc)] + [_P] * 11 + [_I, ctypes.POINTER(_I),
+                                                                                 ctypes.POINTER(ctypes.c_double)]),
+    "clowns_assemble": (ctypes.c_int, [ctypes.POINTER(GridDev), _I, _P, _P, _P, _P, _P]),
+    "clowns_solver_assemble": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), ctypes.c_double, _P, _I]),
+})
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def classify(g):
+    """Host classification of a configs.Grid.  Returns a dict with nf, nb,
+    mno (per cell), row2cell, sp_row (per row), N, mp and the (possibly
+    clipped) F it classified."""
+    A = g.arrays()
+    nc = g.ncell
+    out = dict(nf=np.zeros(nc, np.int8), nb=np.zeros(nc, np.int8), mno=np.zeros(nc, np.int32),
+               row2cell=np.zeros(nc, np.int32), sp_row=np.zeros(nc), F=A["F"].copy())
+    desc = GridDesc(g.nx, g.ny, g.nz, g.stride, g.dt)
+    n = _I(0)
+    mp = ctypes.c_double(0)
+    L.check(L.lib().clowns_classify(ctypes.byref(desc), _p(A["solid"]), _p(out["F"]), _p(A["fb"]), _p(A["a"]),
+                                    _p(A["dx"]), _p(A["xf"]), _p(out["nf"]), _p(out["nb"]), _p(out["mno"]),
+                                    _p(out["row2cell"]), _p(out["sp_row"]), nc, ctypes.byref(n), ctypes.byref(mp)),
+            "clowns_classify")
+    out["N"] = n.value
+    out["mp"] = mp.value
+    out["row2cell"] = out["row2cell"][: n.value].copy()
+    out["sp_row"] = out["sp_row"][: n.value].copy()
+    out["arrays"] = A
+    return out
+
+
+class DeviceGrid:
+    def __init__(self, g, device=0, classified=None):
+        import torch
+        self._torch = torch
+        self.g = g
+        c = classified if classified is not None else classify(g)
+        self.N = c["N"]
+        self.mp = c["mp"]
+        self.host = c
+        dev = torch.device(f"cuda:{device}")
+        A = c["arrays"]
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        self.t = dict(nf=t(c["nf"]), nb=t(c["nb"]), mno=t(c["mno"]), row2cell=t(c["row2cell"]),
+                      sp_row=t(c["sp_row"]), F=t(c["F"]), fb=t(A["fb"]), a=t(A["a"]), un=t(A["un"]),
+                      dx=t(A["dx"]), xf=t(A["xf"]))
+        ptrs = {k: ctypes.c_void_p(v.data_ptr()) for k, v in self.t.items()}
+        self.dev = GridDev(g.nx, g.ny, g.nz, g.stride, self.N, 0, g.dt, self.mp, *[ptrs[k] for k in (
+            "nf", "nb", "mno", "row2cell", "sp_row", "F", "fb", "a", "un", "dx", "xf")])
+        self.device = dev
+
+    def set_velocity(self, un):
+        """Replace the face velocities (e.g. between pressure-correction passes)."""
+        self.t["un"].copy_(self._torch.as_tensor(un).reshape(self.t["un"].shape))
+
+    def assemble(self, lda=None, stream=None):
+        """Reference-layout PCGPME inputs: AL (6, lda), LL (6, lda) 1-based,
+        D (N,), B (N,) as torch tensors on the device."""
+        torch = self._torch
+        lda = lda or self.N
+        AL = torch.empty((6, lda), dtype=torch.float64, device=self.device)
+        LL = torch.empty((6, lda), dtype=torch.int32, device=self.device)
+        D = torch.empty(self.N, dtype=torch.float64, device=self.device)
+        B = torch.empty(self.N, dtype=torch.float64, device=self.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(L.lib().clowns_assemble(ctypes.byref(self.dev), lda, ctypes.c_void_p(AL.data_ptr()),
+                                        ctypes.c_void_p(LL.data_ptr()), ctypes.c_void_p(D.data_ptr()),
+                                        ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(stream)),
+                "clowns_assemble")
+        return AL, LL, D, B
+
+    def assemble_into(self, solver, parm=5.0, keep_raw=False, b=None):
+        """Fused assembly + scaling + sort into the solver's resident operator.
+        Returns the unscaled right-hand side b (N,)."""
+        torch = self._torch
+        if b is None:
+            b = torch.empty(self.N, dtype=torch.float64, device=self.device)
+        L.check(L.lib().clowns_solver_assemble(solver._h, ctypes.byref(self.dev), float(parm),
+                                               ctypes.c_void_p(b.data_ptr()), 1 if keep_raw else 0),
+                "clowns_solver_assemble")
+        solver.N, solver.NL = self.N, 6
+        solver.lda = (self.N + 63) // 64 * 64
+        return b

@@ -1,0 +1,171 @@
+// assembly_kernels.hpp -- per-row assembly of the pressure system on the GPU.
+//
+// One thread per matrix row builds what the reference builds in three
+// places: the static coefficients of init_matrix (src2LOWNS/1sub-matrix.F90:
+// 32-64, cal_alu_f :82-87), the per-pass row of cal_matrix_element
+// (src2LOWNS/1sub-momentum.F90:481-548, cal_alu 1sub-matrix.F90:75-80) and the
+// right-hand side of cal_err_of_continuity (1sub-momentum.F90:315-346,
+// renzoku_fv lib/src/math_func.F90:51-54).  Neighbours are fixed offsets on
+// the box (+-1, +-X, +-X*Y), so the gathers of flags, row numbers, open ratios
+// and velocities are coalesced across consecutive rows.  Expressions keep the
+// reference's order; with -ffp-contract=off the output equals the reference
+// bit for bit.
+//
+// The FUSED form continues in registers with PCGPME's row scaling, ghost
+// indices and exchange sort (pcgpme_para.F90:77-112) and writes the resident
+// operator directly: the unscaled system never round-trips through HBM.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace clowns {
+
+struct GridDev {
+    int nx, ny, nz, stride, N;
+    long ncell;
+    double dt, mp;
+    const int8_t *nf, *nb;
+    const int32_t *mno, *row2cell;
+    const double *sp_row, *F, *fb, *a, *un, *dx, *xf;
+};
+
+template <bool FUSED>
+__global__ void __launch_bounds__(256)
+k_assemble(GridDev g, int lda, double *__restrict__ AL, int *__restrict__ LL,
+           double *__restrict__ D, double *__restrict__ B, double *__restrict__ W,
+           double *__restrict__ AL0, int *__restrict__ LL0, double *__restrict__ D0,
+           double P1, double P2) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= g.N) return;
+    const int X = g.nx + 2, Y = g.ny + 2;
+    const long c = g.row2cell[r];
+    const int p[3] = {(int)(c % X), (int)((c / X) % Y), (int)(c / ((long)X * Y))};
+    const long step[3] = {1, X, (long)X * Y};
+    const int fc = g.nf[c];
+    const bool fluid = fc == 1;
+    const double fbc = g.fb[c];
+    const double mp = g.mp;
+
+    double alu[6];
+    int llu[6];
+    double de_c = 0.0;
+    // init_matrix: ALU_C, then cal_matrix_element per slot [x-,x+,y-,y+,z-,z+]
+    double de_adj[6];  // ALU_C added back for air neighbours (Neumann)
+    int adj = 0;
+#pragma unroll
+    for (int slot = 0; slot < 6; ++slot) {
+        const int l = slot >> 1, side = (slot & 1) ? 1 : -1;
+        const long nc = c + side * step[l];
+        const long pc = side > 0 ? nc : c;  // cell whose low face is shared
+        const int fn = g.nf[nc];
+        const double af = g.nf[pc] == -1 ? 0.0 : g.a[(size_t)l * g.ncell + pc];
+        const double fbn = fn == -1 ? 0.0 : g.fb[nc];
+        alu[slot] = 0.0;
+        llu[slot] = 0;
+        de_adj[slot] = 0.0;
+        if (fn == -1 || af == 0.0 || fbn == 0.0) continue;  // LL_C = 0
+        const double h = g.dx[(size_t)l * g.stride + p[l]];
+        const double hm = g.dx[(size_t)l * g.stride + p[l] + side];
+        const double alu_c = -mp * (2.0 / h / (h + hm) * af / fbc);
+        de_c = de_c - alu_c;
+        const int np = g.mno[nc];
+        if (fn == 0 || np == 0) {
+            if (fluid && fn != 2) {
+                de_adj[slot] = alu_c;
+                adj |= 1 << slot;
+            }
+            continue;
+        }
+        llu[slot] = np;
+        if (fluid) {
+            alu[slot] = alu_c;
+        } else if (abs((int)g.nb[c]) - 1 == l) {
+            // surface row, normal axis: linear extrapolation through sp
+            const int m2 = (slot + 1) & 1;  // mod(ll,2), ll = slot+1
+            const int ii = p[l];
+            const double sp = g.sp_row[r];
+            const double fss = (double)(1 - 2 * m2) * (g.xf[(size_t)l * g.stride + ii - m2 + 1] - sp) /
+                               g.dx[(size_t)l * g.stride + ii];
+            const double hh = g.dx[(size_t)l * g.stride + ii];
+            const double hm2 = g.dx[(size_t)l * g.stride + ii - 2 * m2 + 1];
+            alu[slot] = (fss - 0.5) * hh / (fss * hh + hm2 * 0.5) * (-1.0);
+        } else {
+            alu[slot] = -1.0e-20;
+        }
+    }
+    double de = fluid ? de_c : 1.0;
+#pragma unroll
+    for (int slot = 0; slot < 6; ++slot)
+        if (adj & (1 << slot)) de = de + de_adj[slot];
+    // right-hand side, fluid rows only
+    double b = 0.0;
+    if (fluid) {
+        double rn;
+        if (fbc != 0.0) {
+            const size_t n = (size_t)g.ncell;
+            auto A = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.a[l * n + cc]; };
+            auto U = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.un[l * n + cc]; };
+            const long cx = c + 1, cy = c + X, cz = c + (long)X * Y;
+            rn = ((U(0, cx) * A(0, cx) - g.un[c] * g.a[c]) / g.dx[p[0]] +
+                  (U(1, cy) * A(1, cy) - g.un[n + c] * g.a[n + c]) / g.dx[(size_t)g.stride + p[1]] +
+                  (U(2, cz) * A(2, cz) - g.un[2 * n + c] * g.a[2 * n + c]) /
+                      g.dx[2 * (size_t)g.stride + p[2]]) /
+                 fbc;
+        } else {
+            rn = -10000.0;
+        }
+        double ff = 1.0 - g.F[c];
+        ff = ff > 0.001 ? 0.001 : (ff < -0.001 ? -0.001 : ff);
+        const double dr1 = rn + ff / g.dt;
+        const double q = fabs(dr1 / g.dt);
+        if (!(q <= 1e-15 && q != 0.0)) b = g.mp * dr1 / g.dt;
+    }
+    B[r] = b;
+    if (!FUSED || AL0) {
+        double *oA = FUSED ? AL0 : AL;
+        int *oL = FUSED ? LL0 : LL;
+        double *oD = FUSED ? D0 : D;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            oA[(size_t)s * lda + r] = alu[s];
+            oL[(size_t)s * lda + r] = llu[s];
+        }
+        oD[r] = de;
+    }
+    if (!FUSED) return;
+    // PCGPME row scaling + ghost indices + exchange sort (pcgpme_para.F90:77-112)
+    const int I = r + 1;
+    double w = 0.0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        if (llu[s] == 0) llu[s] = g.N + 1 + I % 64;
+        w = w + fabs(alu[s]);
+    }
+    const double m = (w > de) ? w : de;
+    w = P2 / (m + de * P1);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) alu[s] = alu[s] * w;
+    D[r] = de * w;
+    W[r] = w;
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+#pragma unroll
+        for (int t = s + 1; t < 6; ++t) {
+            if (fabs(alu[s]) < fabs(alu[t])) {
+                const double tv = alu[s];
+                const int ti = llu[s];
+                alu[s] = alu[t];
+                llu[s] = llu[t];
+                alu[t] = tv;
+                llu[t] = ti;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        AL[(size_t)s * lda + r] = alu[s];
+        LL[(size_t)s * lda + r] = llu[s];
+    }
+}
+
+} // namespace clowns

@@ -20,6 +20,7 @@
 
 #include "clowns_pcg.h"
 #include "pcg_kernels.hpp"
+#include "assembly_kernels.hpp"
 
 using namespace clowns;
 
@@ -98,6 +99,7 @@ struct clowns_solver {
     // Caller-layout copy of the unscaled operator (gradients).
     DevBuf<double> AL0, D0;
     DevBuf<int> LL0;
+    bool raw_valid = false;
 
     VecSet<double> v64;
     VecSet<float> v32;
@@ -342,6 +344,7 @@ int load_matrix(clowns_solver *h, const double *AL, int NL, int lda, const int *
     h->parm = parm;
     h->have32 = false;
     h->have_t = false;
+    h->raw_valid = keep_raw;
     return CLOWNS_OK;
 }
 
@@ -758,7 +761,7 @@ namespace {
 // Build and scale/sort A^T from the caller-layout copy kept by set_matrix.
 int ensure_transpose(clowns_solver *h) {
     if (h->have_t) return CLOWNS_OK;
-    if (h->AL0.n < (size_t)h->lda * h->NL) return CLOWNS_EBADARG;
+    if (!h->raw_valid) return CLOWNS_EBADARG;
     int rc;
     const int N = h->N, NL = h->NL, lda = h->lda;
     const size_t nm = (size_t)lda * NL;
@@ -805,13 +808,88 @@ extern "C" int clowns_solver_solve_transpose(clowns_solver *h, const double *d_g
 extern "C" int clowns_grad_coefficients(clowns_solver *h, const double *d_lambda,
                                         const double *d_x, double *d_gAL, double *d_gD) {
     if (!h || !d_lambda || !d_x || h->N < 1) return CLOWNS_EBADARG;
-    if (h->LL0.n < (size_t)h->lda * h->NL) return CLOWNS_EBADARG;
+    if (!h->raw_valid) return CLOWNS_EBADARG;
     int rc = set_device(h);
     if (rc) return rc;
     k_grad_coef<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->NL, h->LL0.p, d_lambda,
                                                       d_x, d_gAL, d_gD);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
+    return CLOWNS_OK;
+}
+
+namespace {
+GridDev grid_of(const clowns_grid_dev *g) {
+    GridDev d;
+    d.nx = g->nx;
+    d.ny = g->ny;
+    d.nz = g->nz;
+    d.stride = g->stride;
+    d.N = g->nrows;
+    d.ncell = (long)(g->nx + 2) * (g->ny + 2) * (g->nz + 2);
+    d.dt = g->dt;
+    d.mp = g->mp;
+    d.nf = g->nf;
+    d.nb = g->nb;
+    d.mno = g->mno;
+    d.row2cell = g->row2cell;
+    d.sp_row = g->sp_row;
+    d.F = g->F;
+    d.fb = g->fb;
+    d.a = g->a;
+    d.un = g->un;
+    d.dx = g->dx;
+    d.xf = g->xf;
+    return d;
+}
+bool grid_ok(const clowns_grid_dev *g) {
+    return g && g->nrows >= 1 && g->nx >= 1 && g->ny >= 1 && g->nz >= 1 && g->nf && g->nb && g->mno &&
+           g->row2cell && g->sp_row && g->F && g->fb && g->a && g->un && g->dx && g->xf && g->dt != 0.0 &&
+           g->stride >= std::max(g->nx, std::max(g->ny, g->nz)) + 2;
+}
+} // namespace
+
+extern "C" int clowns_assemble(const clowns_grid_dev *g, int32_t lda, double *d_AL, int32_t *d_LL,
+                               double *d_D, double *d_B, void *stream) {
+    if (!grid_ok(g) || lda < g->nrows || !d_AL || !d_LL || !d_D || !d_B) return CLOWNS_EBADARG;
+    hipStream_t s = (hipStream_t)stream;
+    k_assemble<false><<<cdiv(g->nrows, 256), 256, 0, s>>>(grid_of(g), lda, d_AL, d_LL, d_D, d_B, nullptr,
+                                                          nullptr, nullptr, nullptr, 0.0, 0.0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return CLOWNS_OK;
+}
+
+extern "C" int clowns_solver_assemble(clowns_solver *h, const clowns_grid_dev *g, double parm,
+                                      double *d_B, int32_t keep_raw) {
+    if (!h || !grid_ok(g) || !d_B) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    const int N = g->nrows;
+    const int lda = (N + 63) & ~63;
+    const size_t nm = (size_t)lda * 6;
+    if ((rc = h->AL.ensure(nm)) || (rc = h->LL.ensure(nm)) || (rc = h->D.ensure(N)) ||
+        (rc = h->W.ensure(N)))
+        return rc;
+    if (keep_raw && ((rc = h->AL0.ensure(nm)) || (rc = h->LL0.ensure(nm)) || (rc = h->D0.ensure(N))))
+        return rc;
+    if ((rc = ensure_common(h, N))) return rc;
+    double P1 = parm, P2;
+    if (parm < 1.0) P1 = 2.0 * 2.0;
+    P2 = P1 + 1.0;
+    hipStream_t s = h->stream;
+    k_assemble<true><<<cdiv(N, 256), 256, 0, s>>>(grid_of(g), lda, h->AL.p, h->LL.p, h->D.p, d_B, h->W.p,
+                                                 keep_raw ? h->AL0.p : nullptr, keep_raw ? h->LL0.p : nullptr,
+                                                 keep_raw ? h->D0.p : nullptr, P1, P2);
+    HIPCHK(hipGetLastError());
+    h->N = N;
+    h->NL = 6;
+    h->lda = lda;
+    h->parm = parm;
+    h->have32 = false;
+    h->have_t = false;
+    h->raw_valid = keep_raw != 0;
+    HIPCHK(hipStreamSynchronize(s));
     return CLOWNS_OK;
 }
 

@@ -119,6 +119,62 @@ int clowns_grad_coefficients(clowns_solver *h, const double *d_lambda,
 int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32_t reps,
                               double *avg_ms);
 
+/* ---- grid geometry and assembly --------------------------------------
+ * Box convention: cells i=0..nx+1, j=0..ny+1, k=0..nz+1 whose outer ring is
+ * solid (closed walls); box index c = i + X*(j + Y*k), X = nx+2, Y = ny+2,
+ * which is the reference's k-major row order (src2LOWNS/nfd2015.F90:69-78).
+ * Per-cell fields: solid (int8, 1 = wall/object cell, nf = -1), F (VOF
+ * fill), fb (cell open ratio), a[3][ncell] (open ratio of the cell's LOW face
+ * per axis), un[3][ncell] (velocity on that face); per-axis dx[3][stride]
+ * (cell sizes) and xf[3][stride] (low-face coordinates), index = cell index
+ * along the axis. */
+typedef struct clowns_grid_desc {
+    int32_t nx, ny, nz;
+    int32_t stride;     /* >= max(nx,ny,nz) + 2 */
+    double dt;          /* time step of the RHS (1sub-momentum.F90:334-345) */
+} clowns_grid_desc;
+
+/* Replaces set_nflag (src2LOWNS/nfd2015.F90:7-306, serial semantics), the
+ * ik==0 row numbering of cal_err_of_continuity (1sub-momentum.F90:273-296)
+ * and the surface position of set_initial_surface_position
+ * (1sub-normal-ini.F90:23-40).  Host arrays.  Outputs per cell nf (-1 solid,
+ * 0 air, 1 fluid, 2 surface), nb (surface orientation +-1..3), mno (row,
+ * 1-based, or 0); per row row2cell (box index) and sp_row (surface position
+ * along the normal for surface rows, else 0); nrows = mnoe; mp = (min dx)^2
+ * (1sub-matrix.F90:17).  F is modified as the reference modifies it (air
+ * cells clipped to >= 0).  cap = capacity of row2cell/sp_row. */
+int clowns_classify(const clowns_grid_desc *g, const int8_t *solid, double *F,
+                    const double *fb, const double *a, const double *dx,
+                    const double *xf, int8_t *nf, int8_t *nb, int32_t *mno,
+                    int32_t *row2cell, double *sp_row, int32_t cap,
+                    int32_t *nrows, double *mp);
+
+/* Device-resident grid for the assembly kernels (all pointers on the GPU). */
+typedef struct clowns_grid_dev {
+    int32_t nx, ny, nz, stride, nrows;
+    int32_t reserved;
+    double dt, mp;
+    const int8_t *nf, *nb;
+    const int32_t *mno, *row2cell;
+    const double *sp_row, *F, *fb, *a, *un, *dx, *xf;
+} clowns_grid_dev;
+
+/* Replaces init_matrix + cal_matrix_element + the RHS of
+ * cal_err_of_continuity (src2LOWNS/1sub-matrix.F90:3-87,
+ * 1sub-momentum.F90:313-346, 465-551): writes the PCGPME inputs AL, LL
+ * (column stride lda >= nrows, slots [x-,x+,y-,y+,z-,z+]), D and B (length
+ * nrows) exactly as the reference's alu, llu, de and b.  `stream` is a
+ * hipStream_t or NULL (default stream); synchronous on return. */
+int clowns_assemble(const clowns_grid_dev *g, int32_t lda, double *d_AL,
+                    int32_t *d_LL, double *d_D, double *d_B, void *stream);
+
+/* The same assembly fused with PCGPME's row scaling and sort: the resident
+ * operator of `h` is written in one pass (no unscaled copy in HBM unless
+ * keep_raw, which the adjoint/gradient calls need); d_B gets the unscaled
+ * right-hand side for clowns_solver_solve. */
+int clowns_solver_assemble(clowns_solver *h, const clowns_grid_dev *g,
+                           double parm, double *d_B, int32_t keep_raw);
+
 /* Library build string, e.g. "clowns_pcg gfx950 ...". */
 const char *clowns_version(void);
 

@@ -312,7 +312,9 @@ int oracle_build_system(int nx, int ny, int nz, const int8_t *solid, const doubl
         long c = g.fln[s];
         int i = (int)(c % g.X), j = (int)((c / g.X) % g.Y), k = (int)(c / ((long)g.X * g.Y));
         if (g.nf[c] == 0 || g.nf[c] == -1 || g.fb[c] == 0.0) continue;
-        if (g.nf[c] == 2) {
+        if (g.nf[c] == 2 && g.nb[c] != 0) {
+            /* b = 0 (no orientation; fxx1 ties at zero) would index lf(-1,:)
+             * in the reference; such a cell is numbered without the check */
             int l = abs(g.nb[c]) - 1, sg = g.nb[c] > 0 ? 1 : -1;
             if (fflag(&g, i + sg * E[l][0], j + sg * E[l][1], k + sg * E[l][2]) == -1) continue;
         }
