@@ -126,15 +126,18 @@ class Solver:
                 "clowns_grad_coefficients")
         return gAL, gD
 
-    def get_scaled(self):
+    def get_scaled(self, residual=False):
+        """The resident scaled/sorted operator (AL, LL, D), the row scales W
+        and, with residual=True (after a solve), the final scaled residual."""
         torch = self._torch
         dev = f"cuda:{self.device}"
         AL = torch.empty((self.NL, self.lda), dtype=torch.float64, device=dev)
         LL = torch.empty((self.NL, self.lda), dtype=torch.int32, device=dev)
         D = torch.empty(self.N, dtype=torch.float64, device=dev)
         W = torch.empty(self.N, dtype=torch.float64, device=dev)
-        B = torch.empty(self.N, dtype=torch.float64, device=dev)
-        L.check(L.lib().clowns_solver_get_scaled(self._h, _dptr(AL), _dptr(LL), _dptr(D), _dptr(W), _dptr(B)),
+        B = torch.empty(self.N, dtype=torch.float64, device=dev) if residual else None
+        L.check(L.lib().clowns_solver_get_scaled(self._h, _dptr(AL), _dptr(LL), _dptr(D), _dptr(W),
+                                                 _dptr(B) if residual else None),
                 "clowns_solver_get_scaled")
         return AL, LL, D, W, B
 

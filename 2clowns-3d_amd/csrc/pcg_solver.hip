@@ -730,7 +730,7 @@ extern "C" int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32
     hipStream_t s = h->stream;
     const int N = h->N;
     if ((rc = ensure_vecs(h, h->v64, N))) return rc;
-    if (kernel >= 2) {
+    if (kernel == 2 || kernel == 3 || kernel == 6) {
         if ((rc = ensure_f32(h)) || (rc = ensure_vecs(h, h->v32, N))) return rc;
     }
     Scal *sc = h->sc.p;
@@ -740,10 +740,13 @@ extern "C" int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32
     Op<float> A32{N, h->NL, h->lda, h->AL32.p, h->LL.p, h->D32.p};
     auto one = [&]() {
         switch (kernel) {
-        case 0: spmv<double>(s, g, sc, A, 1, h->v64.p0.p, h->v64.r.p, h->v64.q.p, h->v64.p1.p, h->v64.y.p); break;
+        case 0: spmv<double>(s, g, sc, A, 0, h->v64.p0.p, nullptr, nullptr, nullptr, h->v64.y.p); break;
         case 1: launch_pm_stage<double, 1>(s, g, sc, A, 0, h->v64.y.p, h->v64.t.p, 0, nullptr, h->part.p); break;
-        case 2: spmv<float>(s, g, sc, A32, 1, h->v32.p0.p, h->v32.r.p, h->v32.q.p, h->v32.p1.p, h->v32.y.p); break;
-        default: launch_pm_stage<float, 1>(s, g, sc, A32, 0, h->v32.y.p, h->v32.t.p, 0, nullptr, h->part.p); break;
+        case 2: spmv<float>(s, g, sc, A32, 0, h->v32.p0.p, nullptr, nullptr, nullptr, h->v32.y.p); break;
+        case 3: launch_pm_stage<float, 1>(s, g, sc, A32, 0, h->v32.y.p, h->v32.t.p, 0, nullptr, h->part.p); break;
+        case 4: k_copy<double><<<cdiv(N, BS), BS, 0, s>>>(N, h->v64.r.p, h->v64.e.p); break;
+        case 5: launch_pm2<double>(s, g, sc, A, 0, h->v64.y.p, h->v64.t.p); break;
+        default: launch_pm2<float>(s, g, sc, A32, 0, h->v32.y.p, h->v32.t.p); break;
         }
     };
     one();

@@ -1,0 +1,238 @@
+"""Benchmark: one 2CLOWNS-3D pressure solve per step on one MI355X.
+
+A step is one pass of the hot path over the configuration-2 workload
+(256x256x64 cells, wavy free surface at 0.6 H, 64x64x24 solid obstacle,
+seeded U(-0.1,0.1) face velocities): device assembly of the pressure system
+fused with PCGPME's row scaling and sort (init_matrix + cal_matrix_element +
+RHS + pcgpme_para.F90:77-112), then the PM-preconditioned Bi-CGSTAB solve to
+the reference's stopping rule sqrt(RN/BN) <= 1e-10, FP64 end to end.  The
+geometry (classified once on the host) is resident in HBM before timing; no
+state carries from one step to the next (x0 = 0, operator rebuilt).
+
+The same steps are then run in mixed precision (FP32 inner solve, FP64
+refinement), reported under "mixed" with the error against the FP64 solution.
+
+Prints one JSON line.  `value` is seconds per step (lower is better).
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "2clowns-3d_amd"))
+
+PEAK_HBM_GBS = 8000.0
+# Algorithmic bytes per matrix row (values and vectors each kernel must stream
+# once; index arrays excluded).  Whole iteration: 512 B/row at FP64.
+ALG_BYTES = {
+    "spmv": 72,        # AL 48 + D 8 + x 8 + y 8
+    "pm_stage": 24,    # AL_j 8 + in 8 + out 8
+    "iteration": 512,
+    "setup_cold": 288,
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3])
+    ap.add_argument("--eps", type=float, default=1e-10)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only assemble and run the timed kernel loops (for rocprofv3 --pmc)")
+    return ap.parse_args()
+
+
+def cpu_baseline(g, iters_full, n_iter_hi, threads):
+    """The reference's own PCGPME (flang build in oracle/_ref) -- or, if that
+    was not built, the C oracle -- on the same configuration's system built
+    on the host, run for 5 and n_iter_hi iterations; solve time =
+    setup + iters_full * per-iteration time."""
+    import numpy as np
+    orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    A = g.arrays()
+    nc = g.ncell
+    nf = np.zeros(nc, np.int8)
+    nb = np.zeros(nc, np.int8)
+    r2c = np.zeros(nc, np.int32)
+    mno = np.zeros(nc, np.int32)
+    lda = nc
+    AL = np.zeros((6, lda))
+    LL = np.zeros((6, lda), np.int32)
+    D = np.zeros(lda)
+    B = np.zeros(lda)
+    orc.oracle_build_system.restype = ctypes.c_int
+    N = orc.oracle_build_system(g.nx, g.ny, g.nz, P(A["solid"]), P(A["F"]), P(A["fb"]), P(A["a"]), P(A["un"]),
+                                P(A["dx"]), P(A["xf"]), g.stride, ctypes.c_double(g.dt), P(nf), P(nb), P(r2c),
+                                P(mno), lda, P(AL), P(LL), P(D), P(B))
+    NA = N + 60
+    ALf = np.asfortranarray(AL[:, :NA].T)
+    LLf = np.asfortranarray(LL[:, :NA].T)
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libref_pcgpme.so")
+    kind = "reference" if os.path.exists(ref_path) else "port"
+
+    def run(iters):
+        a, l, d, b = ALf.copy(order="F"), LLf.copy(order="F"), D[:N].copy(), B[:N].copy()
+        x = np.zeros(N + 64)
+        eps = np.array([1e-10, 0.0])
+        it = ctypes.c_int(iters)
+        ier = ctypes.c_int(0)
+        if kind == "reference":
+            lib = ctypes.CDLL(ref_path)
+            lib.ref_set_threads(threads)
+            i32 = lambda v: ctypes.byref(ctypes.c_int(v))
+            call = lambda: lib.pcgpme_(P(a), i32(6), i32(NA), P(l), P(d), i32(N), i32(64), P(b), P(x), P(eps),
+                                       ctypes.byref(ctypes.c_double(5.0)), ctypes.byref(it), ctypes.byref(ier))
+        else:
+            orc.oracle_set_threads(threads)
+            call = lambda: orc.oracle_pcgpme(P(a), 6, NA, P(l), P(d), N, 64, P(b), P(x), P(eps),
+                                             ctypes.c_double(5.0), ctypes.byref(it), ctypes.byref(ier))
+        out = {}
+
+        def body():
+            t0 = time.perf_counter()
+            call()
+            out["t"] = time.perf_counter() - t0
+
+        old = threading.stack_size()
+        # the reference's work arrays are automatic (stack) arrays
+        threading.stack_size(max(32 << 20, 128 * (N + 64) + 32 * NA + (64 << 20)))
+        th = threading.Thread(target=body)
+        th.start()
+        th.join()
+        threading.stack_size(old)
+        return out["t"], it.value
+
+    lo = 5
+    t_lo, _ = run(lo)
+    t_hi, _ = run(n_iter_hi)
+    t_it = (t_hi - t_lo) / (n_iter_hi - lo)
+    t_setup = max(0.0, t_lo - lo * t_it)
+    value = t_setup + iters_full * t_it
+    return {"value": value, "unit": "s/step", "cores": threads, "kind": kind,
+            "sample": (f"configuration {g.name} system (N={N}) built on the host; "
+                       f"{'flang build of pcgpme_para.F90' if kind == 'reference' else 'C oracle of PCGPME'} "
+                       f"timed for {lo} and {n_iter_hi} iterations at {threads} OpenMP threads "
+                       f"({t_it * 1e3:.1f} ms/iteration, setup {t_setup * 1e3:.0f} ms); "
+                       f"value = setup + {iters_full} iterations (the GPU FP64 count)"),
+            "ms_per_iteration": t_it * 1e3}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU z-slab design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import clowns3d
+    from clowns3d import DeviceGrid, Solver, configs
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    g = configs.CONFIGS[args.config]()
+    dg = DeviceGrid(g)
+    N = dg.N
+    sol = Solver(0)
+    b = torch.empty(N, dtype=torch.float64, device=dev)
+    x = torch.zeros(N, dtype=torch.float64, device=dev)
+
+    if args.profile_kernels:
+        dg.assemble_into(sol, b=b)
+        res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
+               (("k_spmv<double,6,0>", 0), ("k_pm<double,1,0>", 1), ("k_copy<double>", 4))}
+        print(json.dumps({"rows": N, "avg_us": res}))
+        return
+
+    def step(mixed):
+        dg.assemble_into(sol, b=b)
+        x.zero_()
+        _, info = sol.solve(b, x, eps=args.eps, maxiter=10000, mixed=mixed)
+        return info
+
+    def run(mixed):
+        for _ in range(args.warmup):
+            step(mixed)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos = [step(mixed) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        return (t1 - t0) / args.steps, infos
+
+    t64, infos64 = run(False)
+    x64 = x.clone()
+    it64 = infos64[-1]["iters"]
+    assert all(i["ier"] == 0 for i in infos64), infos64
+
+    mixed = None
+    if not args.no_mixed:
+        t32, infos32 = run(True)
+        err = ((x - x64).norm() / x64.norm()).item()
+        i32 = infos32[-1]
+        mixed = {"value": t32, "ms_per_step": t32 * 1e3, "inner_iters": i32["iters"],
+                 "refinements": i32["refinements"], "err": i32["err"], "ier": i32["ier"],
+                 "rel_l2_vs_fp64": err, "speedup_vs_fp64": t64 / t32}
+
+    # dominant kernel of the FP64 iteration, timed live with HIP events on
+    # the solver's stream (2 SpMV and 8 single-term PM stages per iteration)
+    dg.assemble_into(sol, b=b)
+    sol.solve(b, x, eps=args.eps)
+    t_spmv = sol.time_kernel(0, 50)
+    t_pm = sol.time_kernel(1, 50)
+    t_copy = sol.time_kernel(4, 50)
+    if 2 * t_spmv >= 8 * t_pm:
+        kname, kms, kbytes = "k_spmv<double,6,0>", t_spmv, ALG_BYTES["spmv"] * N
+    else:
+        kname, kms, kbytes = "k_pm<double,1,0>", t_pm, ALG_BYTES["pm_stage"] * N
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "traffic_config2.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get(kname)
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "kernel": kname,
+                "alg_bytes_per_launch": kbytes, "avg_us": kms * 1e3,
+                "copy_GBps": 16 * N / (t_copy * 1e-3) / 1e9}
+    solve_ms = infos64[-1]["ms"]
+    solve_gbs = N * (ALG_BYTES["setup_cold"] + it64 * ALG_BYTES["iteration"]) / (solve_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(g, it64, args.cpu_iters, threads)
+
+    out = {
+        "metric": "pressure-solve s/step to 1e-10 rel. residual, FP64 vs mixed-prec; HBM GB/s",
+        "value": t64, "unit": "s/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t64 * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"configuration {args.config}: {g.name}, wavy surface + obstacle, "
+                               f"assemble + PM-Bi-CGSTAB to sqrt(RN/BN) <= {args.eps:g}",
+                   "rows": N, "cells": g.nx * g.ny * g.nz, "iterations": it64,
+                   "err": infos64[-1]["err"], "solve_ms_hip_events": solve_ms,
+                   "solve_alg_GBps": solve_gbs, "solve_alg_frac": solve_gbs / PEAK_HBM_GBS},
+        "mixed": mixed,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = cpu["value"] / t64
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

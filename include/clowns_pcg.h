@@ -115,7 +115,9 @@ int clowns_grad_coefficients(clowns_solver *h, const double *d_lambda,
 
 /* Time `reps` back-to-back launches of one solver kernel over the resident
  * operator with HIP events on the solver's stream: kernel 0 = FP64 SpMV,
- * 1 = FP64 PM stage, 2 = FP32 SpMV, 3 = FP32 PM stage.  *avg_ms out. */
+ * 1 = FP64 PM stage, 2 = FP32 SpMV, 3 = FP32 PM stage, 4 = plain FP64 copy
+ * of N entries (reference for counter scaling), 5 = FP64 fused PM stage
+ * pair, 6 = FP32 fused PM stage pair.  *avg_ms out. */
 int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32_t reps,
                               double *avg_ms);
 
