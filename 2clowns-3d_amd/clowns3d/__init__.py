@@ -8,3 +8,4 @@ from ._lib import ClownsError, Info, LIB_PATH, lib  # noqa: F401
 from .pcgpme import Solver, pcgpme  # noqa: F401
 from . import configs  # noqa: F401,E402
 from .assembly import DeviceGrid, classify  # noqa: F401,E402
+from .autograd import PCGSolve, pcg_solve  # noqa: F401,E402

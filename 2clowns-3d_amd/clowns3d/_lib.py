@@ -8,7 +8,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)
-LIB_PATH = os.path.join(PKG_ROOT, "lib", "libclowns_pcg.so")
+LIB_PATH = os.environ.get("CLOWNS_LIB") or os.path.join(PKG_ROOT, "lib", "libclowns_pcg.so")
 
 CLOWNS_OK = 0
 CLOWNS_NOCONV = 3000

@@ -117,4 +117,5 @@ class DeviceGrid:
                 "clowns_solver_assemble")
         solver.N, solver.NL = self.N, 6
         solver.lda = (self.N + 63) // 64 * 64
+        solver.version += 1
         return b

@@ -71,6 +71,7 @@ class Solver:
         self.N = 0
         self.lda = 0
         self.NL = 0
+        self.version = 0  # bumped whenever the resident operator changes
 
     def close(self):
         if self._h:
@@ -97,6 +98,7 @@ class Solver:
         L.check(L.lib().clowns_solver_set_matrix(self._h, _dptr(AL), NL, lda, _dptr(LL), _dptr(D), N,
                                                  float(parm)), "clowns_solver_set_matrix")
         self.N, self.lda, self.NL = N, lda, NL
+        self.version += 1
 
     def solve(self, b, x=None, eps=1e-10, maxiter=10000, mixed=False, warm=False):
         torch = self._torch

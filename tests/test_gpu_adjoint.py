@@ -1,0 +1,106 @@
+"""GPU: the hand-written adjoint.
+
+1. Small operator: gradients from PCGSolve.backward (transpose solve +
+   gradient kernel) against the FP64 dense reference gradient
+   (numpy: lambda = A^-T dL/dx, dL/dA = -lambda x^T) to 1e-8 relative.
+2. Configuration 4: the configuration-2 operator at 256x256x64,
+   L = sum_i w_i x_i^2 over 1000 seeded probe rows, w ~ U(0.5, 1.5);
+   directional derivatives along 20 seeded random directions in (AL, D, b)
+   (off-diagonals perturbed only where LL != 0) against FP64 central
+   differences with eps = 1e-6, to 1e-6 relative.  Each perturbed solve is
+   done as a correction x0 + A'^{-1}(b' - A' x0) with the residual formed in
+   FP64, so solver tolerance does not swamp the O(eps) difference.
+"""
+import numpy as np
+import pytest
+import torch
+
+from clowns3d import DeviceGrid, Solver, configs, pcg_solve
+from conftest import dense_matrix
+from _systems import box_system
+
+pytestmark = pytest.mark.gpu
+dev = torch.device("cuda:0")
+
+
+def test_adjoint_matches_dense_gradient():
+    s = box_system(10, 8, 6, seed=41)
+    N = s["N"]
+    AL = torch.tensor(np.ascontiguousarray(s["AL"].T), device=dev, requires_grad=True)
+    LL = torch.tensor(np.ascontiguousarray(s["LL"].T), device=dev)
+    D = torch.tensor(s["D"], device=dev, requires_grad=True)
+    b = torch.tensor(s["B"], device=dev, requires_grad=True)
+    w = torch.linspace(0.5, 1.5, N, dtype=torch.float64, device=dev)
+    x = pcg_solve(AL, D, b, LL, eps=1e-13)
+    L = (w * x * x).sum()
+    L.backward()
+    A = dense_matrix(s)
+    xd = np.linalg.solve(A, s["B"])
+    lam = np.linalg.solve(A.T, 2 * w.cpu().numpy() * xd)
+    gb = lam
+    gD = -lam * xd
+    gAL = np.zeros_like(s["AL"].T)
+    for k in range(6):
+        m = s["LL"][:N, k] > 0
+        gAL[k, :N][m] = -lam[m] * xd[s["LL"][:N, k][m] - 1]
+    rel = lambda a, r: np.linalg.norm(a - r) / np.linalg.norm(r)
+    assert rel(x.detach().cpu().numpy(), xd) < 1e-10
+    assert rel(b.grad.cpu().numpy(), gb) < 1e-8
+    assert rel(D.grad.cpu().numpy(), gD) < 1e-8
+    assert rel(AL.grad.cpu().numpy(), gAL) < 1e-8
+
+
+def _apply(AL, LL, D, x):
+    """y = A x on the device, reference layout (AL, LL: (6, lda))."""
+    N = D.shape[0]
+    y = D * x
+    for k in range(AL.shape[0]):
+        idx = LL[k, :N].long()
+        m = idx > 0
+        y = y + torch.where(m, AL[k, :N] * x[(idx - 1).clamp(min=0)], torch.zeros_like(y))
+    return y
+
+
+def test_config4_adjoint_vs_central_differences():
+    g = configs.config2()
+    dg = DeviceGrid(g)
+    AL, LL, D, b = dg.assemble()
+    N = dg.N
+    rng = np.random.default_rng(4)
+    probes = torch.tensor(rng.choice(N, 1000, replace=False), device=dev)
+    w = torch.tensor(rng.uniform(0.5, 1.5, 1000), device=dev)
+    sol = Solver(0)
+
+    ALr = AL.clone().requires_grad_(True)
+    Dr = D.clone().requires_grad_(True)
+    br = b.clone().requires_grad_(True)
+    x0 = pcg_solve(ALr, Dr, br, LL, solver=sol, eps=1e-12)
+    L0 = (w * x0[probes] ** 2).sum()
+    L0.backward()
+    gAL, gD, gb = ALr.grad, Dr.grad, br.grad
+    x0 = x0.detach()
+
+    def loss_at(ALp, Dp, bp):
+        r = bp - _apply(ALp, LL, Dp, x0)
+        sol.set_matrix(ALp.contiguous(), LL, Dp.contiguous())
+        d, info = sol.solve(r.contiguous(), eps=1e-12)
+        assert info["ier"] == 0, info
+        x = x0 + d
+        return (w * x[probes] ** 2).sum().item()
+
+    mask = (LL[:, :N] > 0).to(torch.float64)
+    eps = 1e-6
+    errs = []
+    gen = torch.Generator(device="cpu").manual_seed(20)
+    for t in range(20):
+        vAL = torch.zeros_like(AL)
+        vAL[:, :N] = torch.randn(AL[:, :N].shape, generator=gen, dtype=torch.float64).to(dev) * mask * AL[:, :N].abs()
+        vD = torch.randn(N, generator=gen, dtype=torch.float64).to(dev) * D.abs()
+        vb = torch.randn(N, generator=gen, dtype=torch.float64).to(dev) * b.abs().mean()
+        pred = ((gAL * vAL).sum() + (gD * vD).sum() + (gb * vb).sum()).item()
+        Lp = loss_at(AL + eps * vAL, D + eps * vD, b + eps * vb)
+        Lm = loss_at(AL - eps * vAL, D - eps * vD, b - eps * vb)
+        fd = (Lp - Lm) / (2 * eps)
+        errs.append(abs(fd - pred) / max(abs(fd), 1e-300))
+    print("config 4: N", N, "max rel err", max(errs), "median", float(np.median(errs)))
+    assert max(errs) <= 1e-6
