@@ -85,6 +85,24 @@ class DeviceGrid:
             "nf", "nb", "mno", "row2cell", "sp_row", "F", "fb", "a", "un", "dx", "xf")])
         self.device = dev
 
+    def update(self, g, classified):
+        """New fill F (same static geometry: solid, fb, a, un, dx): upload F and
+        the new classification (flags, numbering, surface positions)."""
+        torch = self._torch
+        c = classified
+        self.g = g
+        self.N = c["N"]
+        self.host = c
+        self.t["F"].copy_(torch.from_numpy(c["F"]))
+        self.t["nf"].copy_(torch.from_numpy(c["nf"]))
+        self.t["nb"].copy_(torch.from_numpy(c["nb"]))
+        self.t["mno"].copy_(torch.from_numpy(c["mno"]))
+        self.t["row2cell"] = torch.from_numpy(c["row2cell"]).to(self.device)
+        self.t["sp_row"] = torch.from_numpy(c["sp_row"]).to(self.device)
+        self.dev.nrows = self.N
+        self.dev.row2cell = ctypes.c_void_p(self.t["row2cell"].data_ptr())
+        self.dev.sp_row = ctypes.c_void_p(self.t["sp_row"].data_ptr())
+
     def set_velocity(self, un):
         """Replace the face velocities (e.g. between pressure-correction passes)."""
         self.t["un"].copy_(self._torch.as_tensor(un).reshape(self.t["un"].shape))

@@ -168,19 +168,29 @@ def config_solitary(nx, ny, nz, t, seed=2026, amp_frac=0.1, x0_frac=0.2, g_acc=9
     surface lifted by a solitary wave eta = A sech^2(kappa (x - x0 - c t)),
     A = 0.1 H, c = sqrt(g H), kappa = sqrt(3 A / (4 H^3))."""
     g = Grid(nx, ny, nz)
-    H = 0.6 * nz * g.h[2]
+    g.fb[:] = 1.0
+    g.set_faces()
+    solitary_water(g, t, amp_frac, x0_frac, g_acc)
+    g.set_velocity(seed)
+    g.name = f"solitary{nx}x{ny}x{nz}@t={t:g}"
+    return g
+
+
+def solitary_water(g, t, amp_frac=0.1, x0_frac=0.2, g_acc=9.80665):
+    """Set F of grid g for the solitary wave at time t (static geometry kept)."""
+    H = 0.6 * g.nz * g.h[2]
     A = amp_frac * H
     c = math.sqrt(g_acc * H)
     kap = math.sqrt(3 * A / (4 * H ** 3))
     xc, _ = g.centres()
-    x0 = x0_frac * nx * g.h[0]
+    x0 = x0_frac * g.nx * g.h[0]
     eta = A / np.cosh(kap * (xc - x0 - c * t)) ** 2
-    g.fb[:] = 1.0
-    g.set_faces()
     g.set_water(H + eta)
-    g.set_velocity(seed)
-    g.name = f"solitary{nx}x{ny}x{nz}@t={t:g}"
-    return g
+    g.name = f"solitary{g.nx}x{g.ny}x{g.nz}@t={t:g}"
+
+
+def config5(t=0.0, seed=2026):
+    return config_solitary(1024, 512, 80, t=t, seed=seed)
 
 
 CONFIGS = {1: config1, 2: config2, 3: config3}

@@ -27,6 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "2clowns-3d_amd"))
 
 PEAK_HBM_GBS = 8000.0
+DESCR = {1: "wavy surface at 0.6H + 16x16x12 obstacle", 2: "wavy surface at 0.6H + 64x64x24 obstacle",
+         3: "40 Gaussian bed bumps, partial cells fb in [0.05,1], wavy surface at 0.6H"}
 # Algorithmic bytes per matrix row (values and vectors each kernel must stream
 # once; index arrays excluded).  Whole iteration: 512 B/row at FP64.
 ALG_BYTES = {
@@ -220,7 +222,7 @@ def main():
         "value": t64, "unit": "s/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": t64 * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"configuration {args.config}: {g.name}, wavy surface + obstacle, "
+        "config": {"workload": f"configuration {args.config}: {g.name} ({DESCR[args.config]}), "
                                f"assemble + PM-Bi-CGSTAB to sqrt(RN/BN) <= {args.eps:g}",
                    "rows": N, "cells": g.nx * g.ny * g.nz, "iterations": it64,
                    "err": infos64[-1]["err"], "solve_ms_hip_events": solve_ms,
