@@ -554,15 +554,10 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
         delete h;
         return CLOWNS_EHIP;
     }
-    if (stream) {
-        h->stream = (hipStream_t)stream;
-    } else {
-        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-            delete h;
-            return CLOWNS_EHIP;
-        }
-        h->own_stream = true;
-    }
+    // NULL is the device's null stream (torch's default stream), so work is
+    // ordered with the caller's default-stream work.
+    h->stream = (hipStream_t)stream;
+    h->own_stream = false;
     if (const char *e = getenv("CLOWNS_ROWS_PER_THREAD")) h->R = std::max(1, atoi(e));
     if (const char *e = getenv("CLOWNS_POLL")) h->poll = std::max(1, atoi(e));
     if (const char *e = getenv("CLOWNS_FUSE_PUPD")) h->fuse_pupd = atoi(e) != 0;
@@ -592,10 +587,6 @@ extern "C" int clowns_solver_set_stream(clowns_solver *h, void *stream) {
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     h->own_stream = false;
     h->stream = (hipStream_t)stream;
-    if (!stream) {
-        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return CLOWNS_EHIP;
-        h->own_stream = true;
-    }
     return CLOWNS_OK;
 }
 

@@ -68,8 +68,9 @@ typedef struct clowns_info {
 #define CLOWNS_MIXED 1       /* FP32 inner PM-Bi-CGSTAB + FP64 refinement   */
 #define CLOWNS_WARM_START 2  /* d_x holds the initial guess (else zero)      */
 
-/* Create a solver bound to `device` (-1 = current).  `stream` is a
- * hipStream_t, or NULL for a library-owned stream. */
+/* Create a solver bound to `device` (-1 = current).  `stream` is the
+ * hipStream_t all its work goes to; NULL is the device's null stream (so the
+ * solver orders with the caller's default-stream work, e.g. torch's). */
 int clowns_solver_create(clowns_solver **out, int device, void *stream);
 int clowns_solver_destroy(clowns_solver *h);
 int clowns_solver_set_stream(clowns_solver *h, void *stream);

@@ -104,3 +104,19 @@ def test_config4_adjoint_vs_central_differences():
         errs.append(abs(fd - pred) / max(abs(fd), 1e-300))
     print("config 4: N", N, "max rel err", max(errs), "median", float(np.median(errs)))
     assert max(errs) <= 1e-6
+
+
+def test_solver_orders_with_torch_default_stream():
+    """Regression: the solver must run on torch's stream (the null stream by
+    default); a zero-fill issued by torch just before a library kernel must
+    not land after it."""
+    s = box_system(40, 30, 12, seed=43)
+    N = s["N"]
+    sol = Solver(0)
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
+    x, _ = sol.solve(t(s["B"]), eps=1e-12)
+    lam, _ = sol.solve_transpose(t(np.ones(N)), eps=1e-12)
+    for _ in range(20):
+        gAL, gD = sol.grad_coefficients(lam, x)  # allocates with torch.zeros, then the kernel writes
+        assert torch.equal(gD, -lam * x)
