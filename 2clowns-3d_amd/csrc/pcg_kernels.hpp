@@ -310,6 +310,110 @@ k_pupd(const Scal *__restrict__ sc, int N, T *__restrict__ p,
     p[i] = r[i] + beta * (p[i] - amu * q[i]);
 }
 
+// ------------------------------------------------ two rows per thread --
+// The same PM stages with each thread owning two consecutive rows and
+// loading both with one 8-byte (FP32) or 16-byte (FP64) access: twice the
+// bytes in flight per memory instruction, which the FP32 stages need to get
+// near the copy rate.  Per-row arithmetic is unchanged (bit-identical rows);
+// only the dot products' summation order differs from the one-row kernels.
+// Column starts and vectors must be 2-element aligned (lda even).
+template <typename T> struct V2;
+template <> struct V2<float> { using t = float2; };
+template <> struct V2<double> { using t = double2; };
+template <> struct V2<int> { using t = int2; };
+
+template <typename T>
+__device__ __forceinline__ typename V2<T>::t ld2(const T *p) {
+    return *reinterpret_cast<const typename V2<T>::t *>(p);
+}
+
+template <typename T, int NT, int DOT>
+__global__ void __launch_bounds__(BS)
+k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0,
+        const int *__restrict__ l0, const T *__restrict__ a1,
+        const int *__restrict__ l1, const T *__restrict__ in,
+        T *__restrict__ out, const T *__restrict__ w, double *__restrict__ part) {
+    if (sc->done) return;
+    const int lb = logical_block();
+    const int i = (lb * BS + threadIdx.x) * 2;
+    double s0 = 0.0, s1 = 0.0;
+    if (i + 1 < N) {
+        const auto c0 = ld2(a0 + i);
+        const auto j0 = ld2(l0 + i);
+        const auto xi = ld2(in + i);
+        T o0 = xi.x - c0.x * in[j0.x - 1];
+        T o1 = xi.y - c0.y * in[j0.y - 1];
+        if (NT == 2) {
+            const auto c1 = ld2(a1 + i);
+            const auto j1 = ld2(l1 + i);
+            o0 = o0 - c1.x * in[j1.x - 1];
+            o1 = o1 - c1.y * in[j1.y - 1];
+        }
+        typename V2<T>::t o;
+        o.x = o0;
+        o.y = o1;
+        *reinterpret_cast<typename V2<T>::t *>(out + i) = o;
+        if (DOT == 1 || DOT == 2) {
+            const auto wi = ld2(w + i);
+            s0 += (double)wi.x * (double)o0;
+            s0 += (double)wi.y * (double)o1;
+        }
+        if (DOT == 2) {
+            s1 += (double)o0 * (double)o0;
+            s1 += (double)o1 * (double)o1;
+        }
+        if (DOT == 3) {
+            s0 += (double)o0 * (double)o0;
+            s0 += (double)o1 * (double)o1;
+        }
+    } else if (i < N) {
+        T o = in[i] - a0[i] * in[l0[i] - 1];
+        if (NT == 2) o = o - a1[i] * in[l1[i] - 1];
+        out[i] = o;
+        if (DOT == 1 || DOT == 2) s0 += (double)w[i] * (double)o;
+        if (DOT == 2) s1 += (double)o * (double)o;
+        if (DOT == 3) s0 += (double)o * (double)o;
+    }
+    if (DOT != 0) {
+        __shared__ double lds[BS / 64];
+        const double t0 = block_sum<BS>(s0, lds);
+        if (DOT == 2) {
+            const double t1 = block_sum<BS>(s1, lds);
+            if (threadIdx.x == 0) part[gridDim.x + lb] = t1;
+        }
+        if (threadIdx.x == 0) part[lb] = t0;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_pm2_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0,
+         const int *__restrict__ l0, const T *__restrict__ a1,
+         const int *__restrict__ l1, const T *__restrict__ in,
+         T *__restrict__ out) {
+    if (sc->done) return;
+    const int i = (logical_block() * BS + threadIdx.x) * 2;
+    auto one = [&](int r, T c0, int jj0, T c1, int m, T xr) -> T {
+        const T o1i = xr - c0 * in[jj0 - 1];
+        T o1m = T(0);
+        if (m - 1 < N) o1m = in[m - 1] - a0[m - 1] * in[l0[m - 1] - 1];
+        return o1i - c1 * o1m;
+    };
+    if (i + 1 < N) {
+        const auto c0 = ld2(a0 + i);
+        const auto j0 = ld2(l0 + i);
+        const auto c1 = ld2(a1 + i);
+        const auto j1 = ld2(l1 + i);
+        const auto xi = ld2(in + i);
+        typename V2<T>::t o;
+        o.x = one(i, c0.x, j0.x, c1.x, j1.x, xi.x);
+        o.y = one(i + 1, c0.y, j0.y, c1.y, j1.y, xi.y);
+        *reinterpret_cast<typename V2<T>::t *>(out + i) = o;
+    } else if (i < N) {
+        out[i] = one(i, a0[i], l0[i], a1[i], l1[i], in[i]);
+    }
+}
+
 // ---------------------------------------------------------- update (a11) --
 // X = X + ALP*P + AMU*E;  R = E - AMU*V;  RN += R*R;  C1 += R*R^.
 template <typename T>
@@ -374,18 +478,18 @@ k_init(const Scal *__restrict__ sc, int N, int R, T *__restrict__ wk1,
 }
 
 // ------------------------------------------------------------ finalize --
-// Loads are issued four at a time (independent) before the fixed-order adds:
+// Loads are issued eight at a time (independent) before the fixed-order adds:
 // the partials are few and L2-resident, so latency, not bandwidth, bounds this.
 __device__ __forceinline__ double fin_sum(const double *__restrict__ p, int G) {
     __shared__ double lds[FIN_BS / 64];
     double s = 0.0;
     int k = threadIdx.x;
-    for (; k + 3 * FIN_BS < G; k += 4 * FIN_BS) {
-        const double a = p[k], b = p[k + FIN_BS], c = p[k + 2 * FIN_BS], d = p[k + 3 * FIN_BS];
-        s += a;
-        s += b;
-        s += c;
-        s += d;
+    for (; k + 7 * FIN_BS < G; k += 8 * FIN_BS) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[k + u * FIN_BS];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
     }
     for (; k < G; k += FIN_BS) s += p[k];
     return block_sum<FIN_BS>(s, lds);
@@ -555,7 +659,7 @@ k_transpose(int N, int lda, int NL, const double *__restrict__ AL,
 // dL/dAL(j,k) = -lambda_j * x(LL(j,k)),  dL/dD_j = -lambda_j * x_j, in the
 // caller's unscaled layout (LL = 0 slots get 0).
 __global__ void __launch_bounds__(BS)
-k_grad_coef(int N, int lda, int NL, const int *__restrict__ LL,
+k_grad_coef(int N, int lda, int lda_out, int NL, const int *__restrict__ LL,
             const double *__restrict__ lam, const double *__restrict__ x,
             double *__restrict__ gAL, double *__restrict__ gD) {
     const int j = blockIdx.x * BS + threadIdx.x;
@@ -563,7 +667,7 @@ k_grad_coef(int N, int lda, int NL, const int *__restrict__ LL,
     const double l = lam[j];
     for (int k = 0; k < NL; ++k) {
         const int i = LL[(size_t)k * lda + j];
-        if (gAL) gAL[(size_t)k * lda + j] = (i >= 1 && i <= N) ? -l * x[i - 1] : 0.0;
+        if (gAL) gAL[(size_t)k * lda_out + j] = (i >= 1 && i <= N) ? -l * x[i - 1] : 0.0;
     }
     if (gD) gD[j] = -l * x[j];
 }

@@ -86,7 +86,8 @@ struct clowns_solver {
     bool own_stream = false;
 
     // Resident operator (reference layout after PCGPME's scale + sort).
-    int N = 0, NL = 0, lda = 0;
+    int N = 0, NL = 0, lda = 0;  // lda: device column stride (multiple of 64)
+    int lda_user = 0;             // column stride of the caller's arrays
     double parm = 5.0;
     DevBuf<double> AL, D, W;
     DevBuf<int> LL;
@@ -113,6 +114,7 @@ struct clowns_solver {
 
     int R = 1;             // rows per thread in the streaming kernels
     bool fuse_pupd = false; // p update inside the SpMV gathers (else own pass)
+    double tau_min = 1e-3;  // mixed: smallest relative tolerance of an inner solve
     int poll = 8;          // iterations between host polls of `done`
 };
 
@@ -159,6 +161,29 @@ template <typename T> struct Op {
     const T *D;
 };
 
+// Pairwise fusion of single-term PM stages (k_pm2).  Measured on MI355X at
+// 4.2M rows: FP32 inner solve 8% faster per iteration, FP64 1.5% slower (the
+// three dependent loads of the recomputed gather cost more than the 8-byte
+// stream they save), so it is on for FP32 only.  CLOWNS_FUSE_PM=0/1 forces
+// both precisions off/on.
+int h_fuse_pm = -1;
+// Two rows per thread in the PM stages (k_pm_v2, k_pm2_v2): 8/16-byte
+// accesses, twice the bytes in flight per memory instruction.  Measured on
+// MI355X: single FP64 PM stage 4-5% faster (configurations 2 and 3); on for
+// both precisions (CLOWNS_V2_32 / CLOWNS_V2_64 = 0 to switch off).
+bool h_v2_32 = true, h_v2_64 = true;
+template <typename T> bool use_v2() { return sizeof(T) == 4 ? h_v2_32 : h_v2_64; }
+
+template <typename T>
+void launch_pm2(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j, const T *in, T *out) {
+    const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
+    if (use_v2<T>())
+        k_pm2_v2<T><<<geo_of(A.N, 2).G, BS, 0, s>>>(sc, A.N, A.AL + o0, A.LL + o0, A.AL + o1, A.LL + o1,
+                                                    in, out);
+    else
+        k_pm2<T><<<g.G, BS, 0, s>>>(sc, A.N, g.R, A.AL + o0, A.LL + o0, A.AL + o1, A.LL + o1, in, out);
+}
+
 template <typename T, int NL>
 void launch_spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
                  const T *v0, const T *v1, const T *v2, T *vout, T *y) {
@@ -179,62 +204,57 @@ void launch_spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
 }
 
 template <typename T, int NT>
-void launch_pm_stage(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j,
-                     const T *in, T *out, int dot, const T *w, double *part) {
+int launch_pm_stage(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j,
+                    const T *in, T *out, int dot, const T *w, double *part) {
     const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
     const T *a0 = A.AL + o0, *a1 = NT == 2 ? A.AL + o1 : nullptr;
     const int *l0 = A.LL + o0, *l1 = NT == 2 ? A.LL + o1 : nullptr;
+    if (use_v2<T>()) {
+        const int G2 = geo_of(A.N, 2).G;
+        switch (dot) {
+        case 0: k_pm_v2<T, NT, 0><<<G2, BS, 0, s>>>(sc, A.N, a0, l0, a1, l1, in, out, w, part); break;
+        case 1: k_pm_v2<T, NT, 1><<<G2, BS, 0, s>>>(sc, A.N, a0, l0, a1, l1, in, out, w, part); break;
+        case 2: k_pm_v2<T, NT, 2><<<G2, BS, 0, s>>>(sc, A.N, a0, l0, a1, l1, in, out, w, part); break;
+        default: k_pm_v2<T, NT, 3><<<G2, BS, 0, s>>>(sc, A.N, a0, l0, a1, l1, in, out, w, part); break;
+        }
+        return G2;
+    }
     switch (dot) {
     case 0: k_pm<T, NT, 0><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
     case 1: k_pm<T, NT, 1><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
     case 2: k_pm<T, NT, 2><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
     default: k_pm<T, NT, 3><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
     }
-}
-
-// Pairwise fusion of single-term PM stages (k_pm2).  Measured on MI355X at
-// 4.2M rows: FP32 inner solve 8% faster per iteration, FP64 1.5% slower (the
-// three dependent loads of the recomputed gather cost more than the 8-byte
-// stream they save), so it is on for FP32 only.  CLOWNS_FUSE_PM=0/1 forces
-// both precisions off/on.
-int h_fuse_pm = -1;
-
-template <typename T>
-void launch_pm2(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j, const T *in, T *out) {
-    const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
-    k_pm2<T><<<g.G, BS, 0, s>>>(sc, A.N, g.R, A.AL + o0, A.LL + o0, A.AL + o1, A.LL + o1, in, out);
+    return g.G;
 }
 
 // LDUSUB (pcgpme_para.F90:263-320): out = PM*in through scratch y, t.
 // `in` may be `out`; neither may be y or t except in == y.
 template <typename T>
-void apply_pm(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, const T *in,
-              T *out, T *y, T *t, int dot, const T *w, double *part) {
+int apply_pm(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, const T *in,
+             T *out, T *y, T *t, int dot, const T *w, double *part) {
     if (h_fuse_pm == 1 || (h_fuse_pm < 0 && sizeof(T) == 4)) {
         // stages (1,2) and (3,4) fused pairwise; 5/6 (two-term) carries the dots
         launch_pm2<T>(s, g, sc, A, 0, in, t);
         if (A.NL == 6) {
             launch_pm2<T>(s, g, sc, A, 2, t, y);
-            launch_pm_stage<T, 2>(s, g, sc, A, 4, y, out, dot, w, part);
-        } else {
-            // NL = 4: stage 3 alone, then stage 4 with the dots
-            launch_pm_stage<T, 1>(s, g, sc, A, 2, t, y, 0, w, part);
-            launch_pm_stage<T, 1>(s, g, sc, A, 3, y, out, dot, w, part);
+            return launch_pm_stage<T, 2>(s, g, sc, A, 4, y, out, dot, w, part);
         }
-        return;
+        // NL = 4: stage 3 alone, then stage 4 with the dots
+        launch_pm_stage<T, 1>(s, g, sc, A, 2, t, y, 0, w, part);
+        return launch_pm_stage<T, 1>(s, g, sc, A, 3, y, out, dot, w, part);
     }
     if (A.NL == 6) {
         launch_pm_stage<T, 1>(s, g, sc, A, 0, in, t, 0, w, part);
         launch_pm_stage<T, 1>(s, g, sc, A, 1, t, y, 0, w, part);
         launch_pm_stage<T, 1>(s, g, sc, A, 2, y, t, 0, w, part);
         launch_pm_stage<T, 1>(s, g, sc, A, 3, t, y, 0, w, part);
-        launch_pm_stage<T, 2>(s, g, sc, A, 4, y, out, dot, w, part);
-    } else {
-        launch_pm_stage<T, 1>(s, g, sc, A, 0, in, t, 0, w, part);
-        launch_pm_stage<T, 1>(s, g, sc, A, 1, t, y, 0, w, part);
-        launch_pm_stage<T, 1>(s, g, sc, A, 2, y, t, 0, w, part);
-        launch_pm_stage<T, 1>(s, g, sc, A, 3, t, out, dot, w, part);
+        return launch_pm_stage<T, 2>(s, g, sc, A, 4, y, out, dot, w, part);
     }
+    launch_pm_stage<T, 1>(s, g, sc, A, 0, in, t, 0, w, part);
+    launch_pm_stage<T, 1>(s, g, sc, A, 1, t, y, 0, w, part);
+    launch_pm_stage<T, 1>(s, g, sc, A, 2, y, t, 0, w, part);
+    return launch_pm_stage<T, 1>(s, g, sc, A, 3, t, out, dot, w, part);
 }
 
 template <typename T>
@@ -270,12 +290,12 @@ int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc,
                 spmv<T>(s, g, sc, A, 1, pc, V.r.p, V.q.p, pn, V.y.p);
                 std::swap(pc, pn);
             }
-            apply_pm<T>(s, g, sc, A, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
-            k_fin_alpha<<<1, FIN_BS, 0, s>>>(sc, part, g.G);
+            const int Gq = apply_pm<T>(s, g, sc, A, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
+            k_fin_alpha<<<1, FIN_BS, 0, s>>>(sc, part, Gq);
             // V = PM*A*E with E = R - ALP*Q fused.
             spmv<T>(s, g, sc, A, 2, nullptr, V.r.p, V.q.p, V.e.p, V.y.p);
-            apply_pm<T>(s, g, sc, A, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
-            k_fin_omega<<<1, FIN_BS, 0, s>>>(sc, part, g.G);
+            const int Gv = apply_pm<T>(s, g, sc, A, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
+            k_fin_omega<<<1, FIN_BS, 0, s>>>(sc, part, Gv);
             k_update<T><<<g.G, BS, 0, s>>>(sc, A.N, g.R, V.x.p, pc, V.e.p, V.v.p,
                                           V.r.p, V.rh.p, part);
             k_fin_beta<<<1, FIN_BS, 0, s>>>(sc, part, g.G, K);
@@ -303,14 +323,19 @@ int load_matrix(clowns_solver *h, const double *AL, int NL, int lda, const int *
     int rc;
     if ((rc = check_nl(NL))) return rc;
     if (N < 1 || lda < N || !AL || !LL || !D) return CLOWNS_EBADARG;
+    // The device copy gets its own 64-aligned stride (aligned vector loads).
+    const int ldu = lda;
+    lda = (N + 63) & ~63;
     const size_t nm = (size_t)lda * NL;
     if ((rc = h->AL.ensure(nm)) || (rc = h->LL.ensure(nm)) || (rc = h->D.ensure(N)) ||
         (rc = h->W.ensure(N)))
         return rc;
     if ((rc = ensure_common(h, N))) return rc;
     hipStream_t s = h->stream;
-    HIPCHK(hipMemcpyAsync(h->AL.p, AL, nm * sizeof(double), kind, s));
-    HIPCHK(hipMemcpyAsync(h->LL.p, LL, nm * sizeof(int), kind, s));
+    HIPCHK(hipMemcpy2DAsync(h->AL.p, (size_t)lda * sizeof(double), AL, (size_t)ldu * sizeof(double),
+                            (size_t)N * sizeof(double), NL, kind, s));
+    HIPCHK(hipMemcpy2DAsync(h->LL.p, (size_t)lda * sizeof(int), LL, (size_t)ldu * sizeof(int),
+                            (size_t)N * sizeof(int), NL, kind, s));
     HIPCHK(hipMemcpyAsync(h->D.p, D, (size_t)N * sizeof(double), kind, s));
     if (keep_raw) {
         if ((rc = h->AL0.ensure(nm)) || (rc = h->LL0.ensure(nm)) || (rc = h->D0.ensure(N)))
@@ -341,6 +366,7 @@ int load_matrix(clowns_solver *h, const double *AL, int NL, int lda, const int *
     h->N = N;
     h->NL = NL;
     h->lda = lda;
+    h->lda_user = ldu;
     h->parm = parm;
     h->have32 = false;
     h->have_t = false;
@@ -445,8 +471,8 @@ int solve_mixed(clowns_solver *h, const Op<double> &A, const Op<float> &A32, con
     k_reset<<<1, 64, 0, s>>>(sc, eps, itermax);
     // bs = W*b ; BN = ||PM*bs||^2.
     k_scale_b<<<cdiv(N, BS), BS, 0, s>>>(N, d_b, Wsc, h->bs.p, warm ? d_x : nullptr, V.x.p);
-    apply_pm<double>(s, g, sc, A, h->bs.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
-    k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, g.G);
+    const int Gb = apply_pm<double>(s, g, sc, A, h->bs.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
+    k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, Gb);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h->h_sc + 1, outer, sizeof(Scal), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -466,10 +492,10 @@ int solve_mixed(clowns_solver *h, const Op<double> &A, const Op<float> &A32, con
         if (warm) {
             // r = PM*(bs - A x).
             spmv<double>(s, g, sc, A, 3, V.x.p, h->bs.p, nullptr, nullptr, V.y.p);
-            apply_pm<double>(s, g, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
-            k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, g.G);
+            const int Gr = apply_pm<double>(s, g, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
+            k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, Gr);
         }
-        const float tau_min = 1e-4f;
+        const double tau_min = h->tau_min;
         for (int it = 0; it < 60; ++it) {
             HIPCHK(hipMemcpyAsync(h->h_sc + 1, outer, sizeof(Scal), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -484,7 +510,7 @@ int solve_mixed(clowns_solver *h, const Op<double> &A, const Op<float> &A32, con
             }
             if (total_inner >= itermax) break;
             // Inner solve of (M A) d = r/||r|| to relative tolerance tau.
-            double tau = std::max((double)tau_min, 0.3 * eps / err);
+            double tau = std::max(tau_min, 0.3 * eps / err);
             k_reset<<<1, 64, 0, s>>>(sc, tau, std::max(1, itermax - total_inner));
             k_rhs32<<<cdiv(N, BS), BS, 0, s>>>(N, outer, V.r.p, F.p0.p);
             fill<float>(s, F.x.p, (size_t)N, 0.f);
@@ -498,8 +524,8 @@ int solve_mixed(clowns_solver *h, const Op<double> &A, const Op<float> &A32, con
             k_correct<<<cdiv(N, BS), BS, 0, s>>>(N, outer, F.x.p, V.x.p);
             k_reset<<<1, 64, 0, s>>>(sc, eps, itermax);
             spmv<double>(s, g, sc, A, 3, V.x.p, h->bs.p, nullptr, nullptr, V.y.p);
-            apply_pm<double>(s, g, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
-            k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, g.G);
+            const int Gr = apply_pm<double>(s, g, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
+            k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, Gr);
             HIPCHK(hipGetLastError());
         }
     }
@@ -561,7 +587,10 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_ROWS_PER_THREAD")) h->R = std::max(1, atoi(e));
     if (const char *e = getenv("CLOWNS_POLL")) h->poll = std::max(1, atoi(e));
     if (const char *e = getenv("CLOWNS_FUSE_PUPD")) h->fuse_pupd = atoi(e) != 0;
+    if (const char *e = getenv("CLOWNS_TAU_MIN")) h->tau_min = std::max(1e-8, atof(e));
     if (const char *e = getenv("CLOWNS_FUSE_PM")) h_fuse_pm = atoi(e) != 0 ? 1 : 0;
+    if (const char *e = getenv("CLOWNS_V2_32")) h_v2_32 = atoi(e) != 0;
+    if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
     *out = h;
     return CLOWNS_OK;
 }
@@ -626,9 +655,13 @@ extern "C" int clowns_solver_get_scaled(clowns_solver *h, double *d_AL, int32_t 
     int rc = set_device(h);
     if (rc) return rc;
     hipStream_t s = h->stream;
-    const size_t nm = (size_t)h->lda * h->NL;
-    if (d_AL) HIPCHK(hipMemcpyAsync(d_AL, h->AL.p, nm * sizeof(double), hipMemcpyDeviceToDevice, s));
-    if (d_LL) HIPCHK(hipMemcpyAsync(d_LL, h->LL.p, nm * sizeof(int), hipMemcpyDeviceToDevice, s));
+    const size_t N = h->N;
+    if (d_AL)
+        HIPCHK(hipMemcpy2DAsync(d_AL, (size_t)h->lda_user * sizeof(double), h->AL.p, (size_t)h->lda * sizeof(double),
+                                N * sizeof(double), h->NL, hipMemcpyDeviceToDevice, s));
+    if (d_LL)
+        HIPCHK(hipMemcpy2DAsync(d_LL, (size_t)h->lda_user * sizeof(int), h->LL.p, (size_t)h->lda * sizeof(int),
+                                N * sizeof(int), h->NL, hipMemcpyDeviceToDevice, s));
     if (d_D) HIPCHK(hipMemcpyAsync(d_D, h->D.p, (size_t)h->N * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (d_W) HIPCHK(hipMemcpyAsync(d_W, h->W.p, (size_t)h->N * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (d_B) {
@@ -691,13 +724,15 @@ extern "C" void clowns_pcgpme(double *AL, const int32_t *NL_, const int32_t *NA_
         rc = CLOWNS_EHIP;
     clowns_info info{};
     if (!rc) {
-        Op<double> A{N, NL, NA, h->AL.p, h->LL.p, h->D.p};
+        Op<double> A{N, NL, h->lda, h->AL.p, h->LL.p, h->D.p};
         rc = solve_fp64(h, A, h->W.p, h->stage_b.p, h->stage_x.p, eps, itermax, warm, &info);
     }
     if (!rc) {
-        const size_t nm = (size_t)NA * NL;
-        if (hipMemcpyAsync(AL, h->AL.p, nm * sizeof(double), hipMemcpyDeviceToHost, s) ||
-            hipMemcpyAsync(LL, h->LL.p, nm * sizeof(int), hipMemcpyDeviceToHost, s) ||
+        const size_t ld = (size_t)h->lda;
+        if (hipMemcpy2DAsync(AL, (size_t)NA * sizeof(double), h->AL.p, ld * sizeof(double),
+                             (size_t)N * sizeof(double), NL, hipMemcpyDeviceToHost, s) ||
+            hipMemcpy2DAsync(LL, (size_t)NA * sizeof(int), h->LL.p, ld * sizeof(int), (size_t)N * sizeof(int),
+                             NL, hipMemcpyDeviceToHost, s) ||
             hipMemcpyAsync(D, h->D.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s) ||
             hipMemcpyAsync(B, h->v64.r.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s) ||
             hipMemcpyAsync(X, h->stage_x.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s) ||
@@ -805,8 +840,8 @@ extern "C" int clowns_grad_coefficients(clowns_solver *h, const double *d_lambda
     if (!h->raw_valid) return CLOWNS_EBADARG;
     int rc = set_device(h);
     if (rc) return rc;
-    k_grad_coef<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->NL, h->LL0.p, d_lambda,
-                                                      d_x, d_gAL, d_gD);
+    k_grad_coef<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->lda_user, h->NL, h->LL0.p,
+                                                      d_lambda, d_x, d_gAL, d_gD);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     return CLOWNS_OK;
@@ -879,6 +914,7 @@ extern "C" int clowns_solver_assemble(clowns_solver *h, const clowns_grid_dev *g
     h->N = N;
     h->NL = 6;
     h->lda = lda;
+    h->lda_user = lda;
     h->parm = parm;
     h->have32 = false;
     h->have_t = false;
