@@ -35,6 +35,8 @@ L.SIGNATURES.update({
                                                                                  ctypes.POINTER(ctypes.c_double)]),
     "clowns_assemble": (ctypes.c_int, [ctypes.POINTER(GridDev), _I, _P, _P, _P, _P, _P]),
     "clowns_solver_assemble": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), ctypes.c_double, _P, _I]),
+    "clowns_solver_classify": (ctypes.c_int, [_P, ctypes.POINTER(GridDesc)] + [_P] * 11 +
+                               [_I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
 })
 
 
@@ -84,6 +86,61 @@ class DeviceGrid:
         self.dev = GridDev(g.nx, g.ny, g.nz, g.stride, self.N, 0, g.dt, self.mp, *[ptrs[k] for k in (
             "nf", "nb", "mno", "row2cell", "sp_row", "F", "fb", "a", "un", "dx", "xf")])
         self.device = dev
+
+    @classmethod
+    def on_device(cls, g, solver, device=0):
+        """Upload the raw fields once and classify on the GPU
+        (clowns_solver_classify); no host classification."""
+        import torch
+        self = cls.__new__(cls)
+        self._torch = torch
+        self.g = g
+        dev = torch.device(f"cuda:{device}")
+        self.device = dev
+        A = g.arrays()
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        nc = g.ncell
+        self.t = dict(solid=t(A["solid"]), F=t(A["F"]), fb=t(A["fb"]), a=t(A["a"]), un=t(A["un"]),
+                      dx=t(A["dx"]), xf=t(A["xf"]),
+                      nf=torch.empty(nc, dtype=torch.int8, device=dev),
+                      nb=torch.empty(nc, dtype=torch.int8, device=dev),
+                      mno=torch.empty(nc, dtype=torch.int32, device=dev),
+                      row2cell=torch.empty(nc, dtype=torch.int32, device=dev),
+                      sp_row=torch.empty(nc, dtype=torch.float64, device=dev))
+        self.host = None
+        self.N = 0
+        self.mp = 0.0
+        ptrs = {k: ctypes.c_void_p(v.data_ptr()) for k, v in self.t.items()}
+        self.dev = GridDev(g.nx, g.ny, g.nz, g.stride, 0, 0, g.dt, 0.0, *[ptrs[k] for k in (
+            "nf", "nb", "mno", "row2cell", "sp_row", "F", "fb", "a", "un", "dx", "xf")])
+        self.classify_device(solver)
+        return self
+
+    def classify_device(self, solver):
+        """(Re)classify on the GPU from the current device F.  Returns True
+        if the exact host fallback was needed."""
+        g = self.g
+        T = self.t
+        desc = GridDesc(g.nx, g.ny, g.nz, g.stride, g.dt)
+        n = _I(0)
+        mp = ctypes.c_double(0)
+        fb = _I(0)
+        p = lambda k: ctypes.c_void_p(T[k].data_ptr())
+        L.check(L.lib().clowns_solver_classify(solver._h, ctypes.byref(desc), p("solid"), p("F"), p("fb"), p("a"),
+                                               p("dx"), p("xf"), p("nf"), p("nb"), p("mno"), p("row2cell"),
+                                               p("sp_row"), g.ncell, ctypes.byref(n), ctypes.byref(mp),
+                                               ctypes.byref(fb)), "clowns_solver_classify")
+        self.N = n.value
+        self.mp = mp.value
+        self.dev.nrows = self.N
+        self.dev.mp = self.mp
+        self.last_fallback = bool(fb.value)
+        return self.last_fallback
+
+    @property
+    def row2cell(self):
+        """Box index of each row (the first N entries of the row buffer)."""
+        return self.t["row2cell"][: self.N]
 
     def update(self, g, classified):
         """New fill F (same static geometry: solid, fb, a, un, dx): upload F and

@@ -189,6 +189,32 @@ def solitary_water(g, t, amp_frac=0.1, x0_frac=0.2, g_acc=9.80665):
     g.name = f"solitary{g.nx}x{g.ny}x{g.nz}@t={t:g}"
 
 
+def solitary_fill_device(g, F_dev, t, amp_frac=0.1, x0_frac=0.2, g_acc=9.80665):
+    """F of the solitary wave at time t computed on the GPU into the flat box
+    tensor F_dev (same expressions as set_water; solid cells stay 0)."""
+    import torch
+    H = 0.6 * g.nz * g.h[2]
+    A = amp_frac * H
+    c = math.sqrt(g_acc * H)
+    kap = math.sqrt(3 * A / (4 * H ** 3))
+    dev = F_dev.device
+    hx, _, dz = g.h
+    xc = (torch.arange(1, g.nx + 1, device=dev, dtype=torch.float64) - 0.5) * hx
+    x0 = x0_frac * g.nx * hx
+    zs = H + A / torch.cosh(kap * (xc - x0 - c * t)) ** 2          # (nx,)
+    k = torch.arange(1, g.nz + 1, device=dev, dtype=torch.float64)[:, None, None]
+    zlo, zhi = (k - 1) * dz, k * dz
+    zs3 = zs[None, None, :]
+    F3 = F_dev.view(g.Z, g.Y, g.X)
+    fb = torch.ones((), dtype=torch.float64, device=dev)
+    part = (zs3 - zhi + fb * dz) / (dz * fb)
+    F = torch.where(zlo >= zs3, 0.0, torch.where(zhi <= zs3, 1.0,
+                    torch.where(fb > (zhi - zs3) / dz, part.clamp(0.0, 1.0), 0.0)))
+    F = F.expand(g.nz, g.ny, g.nx)
+    F3[1:-1, 1:-1, 1:-1] = F
+    return F_dev
+
+
 def config5(t=0.0, seed=2026):
     return config_solitary(1024, 512, 80, t=t, seed=seed)
 

@@ -25,9 +25,10 @@ from .pcgpme import Solver
 
 
 class SolitaryRun:
-    def __init__(self, nx, ny, nz, dt_step=0.05, device=0, seed=2026):
+    def __init__(self, nx, ny, nz, dt_step=0.05, device=0, seed=2026, on_device=False):
         import torch
         self.torch = torch
+        self.on_device = on_device  # F and classification on the GPU
         self.nx, self.ny, self.nz = nx, ny, nz
         self.dt_step = dt_step
         self.seed = seed
@@ -46,7 +47,21 @@ class SolitaryRun:
         return self._g
 
     def setup_step(self, n):
-        """Host: F at t_n and classification; device: upload + fused assembly."""
+        """F at t_n and classification: on the host (upload what changed), or
+        with on_device=True entirely on the GPU."""
+        if self.on_device:
+            torch = self.torch
+            t0 = time.perf_counter()
+            if self.grid is None:
+                g = self.geometry(n)
+                self.grid = DeviceGrid.on_device(g, self.sol)
+            else:
+                configs.solitary_fill_device(self.grid.g, self.grid.t["F"], n * self.dt_step)
+                self.grid.classify_device(self.sol)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            return self.grid.g, {"gen_s": 0.0, "classify_s": t1 - t0, "upload_s": 0.0,
+                                 "host_fallback": self.grid.last_fallback}
         t0 = time.perf_counter()
         g = self.geometry(n)
         t1 = time.perf_counter()
@@ -66,7 +81,7 @@ class SolitaryRun:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         b = dg.assemble_into(self.sol, keep_raw=keep_raw)
-        r2c = dg.t["row2cell"].long()
+        r2c = dg.row2cell.long()
         use_warm = warm and self.x_cell is not None
         x = self.x_cell[r2c].clone() if use_warm else torch.zeros(dg.N, dtype=torch.float64, device=self.dev)
         x, info = self.sol.solve(b, x, eps=eps, mixed=mixed, warm=use_warm)
@@ -97,17 +112,17 @@ class SolitaryRun:
             if keep:
                 if probe_cells is None:
                     # probes among the rows of the first checkpointed step
-                    r2c = self.grid.t["row2cell"]
+                    r2c = self.grid.row2cell
                     pick = rng.choice(self.grid.N, size=min(probes, self.grid.N), replace=False)
                     probe_cells = r2c[torch.as_tensor(pick, device=self.dev)].long()
                     weights = torch.tensor(rng.uniform(0.5, 1.5, len(pick)), device=self.dev)
-                ckpt.append({"n": n, "x": x.detach().clone(), "row2cell": self.grid.t["row2cell"].clone()})
+                ckpt.append({"n": n, "x": x.detach().clone(), "row2cell": self.grid.row2cell.clone()})
         adj = []
         for c in reversed(ckpt):
             n = c["n"]
             g, tm = self.setup_step(n)
             dg = self.grid
-            assert torch.equal(dg.t["row2cell"], c["row2cell"]), "re-assembled numbering differs"
+            assert torch.equal(dg.row2cell, c["row2cell"]), "re-assembled numbering differs"
             dg.assemble_into(self.sol, keep_raw=True)
             # dL/dx_t on rows: probes -> rows through the cell map
             cell_to_row = torch.full((g.ncell,), -1, dtype=torch.long, device=self.dev)
@@ -141,8 +156,10 @@ def main():
     ap.add_argument("--dt", type=float, default=0.05)
     ap.add_argument("--fp64", action="store_true")
     ap.add_argument("--cold", action="store_true")
+    ap.add_argument("--host-classify", action="store_true",
+                    help="F and classification on the host (default: on the GPU)")
     a = ap.parse_args()
-    run = SolitaryRun(a.nx, a.ny, a.nz, dt_step=a.dt)
+    run = SolitaryRun(a.nx, a.ny, a.nz, dt_step=a.dt, on_device=not a.host_classify)
     log = lambda r: print(json.dumps(r), flush=True)
     recs, adj = run.run(a.steps, mixed=not a.fp64, warm=not a.cold, adjoint_steps=a.adjoint, log=log)
     for r in adj:

@@ -21,6 +21,7 @@
 #include "clowns_pcg.h"
 #include "pcg_kernels.hpp"
 #include "assembly_kernels.hpp"
+#include "classify_kernels.hpp"
 
 using namespace clowns;
 
@@ -108,6 +109,7 @@ struct clowns_solver {
     DevBuf<double> stage_b, stage_x;  // clowns_pcgpme host staging
     DevBuf<double> part;   // per-block partial sums
     DevBuf<int> flag;      // device error flag
+    DevBuf<int> cls_scratch;  // classification: extents, change flag, block counts
     DevBuf<Scal> sc;       // [0] inner/FP64 run, [1] outer (mixed)
     Scal *h_sc = nullptr;  // pinned mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
@@ -115,6 +117,7 @@ struct clowns_solver {
     int R = 1;             // rows per thread in the streaming kernels
     bool fuse_pupd = false; // p update inside the SpMV gathers (else own pass)
     double tau_min = 1e-3;  // mixed: smallest relative tolerance of an inner solve
+    bool cls_force_host = false;  // test hook: always take the exact host path
     int poll = 8;          // iterations between host polls of `done`
 };
 
@@ -588,6 +591,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_POLL")) h->poll = std::max(1, atoi(e));
     if (const char *e = getenv("CLOWNS_FUSE_PUPD")) h->fuse_pupd = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_TAU_MIN")) h->tau_min = std::max(1e-8, atof(e));
+    if (const char *e = getenv("CLOWNS_CLASSIFY_HOST")) h->cls_force_host = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_FUSE_PM")) h_fuse_pm = atoi(e) != 0 ? 1 : 0;
     if (const char *e = getenv("CLOWNS_V2_32")) h_v2_32 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
@@ -603,7 +607,7 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     h->AL32.release(); h->D32.release();
     h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
     h->AL0.release(); h->D0.release(); h->LL0.release();
-    h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->sc.release();
+    h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->cls_scratch.release(); h->sc.release();
     if (h->h_sc) (void)hipHostFree(h->h_sc);
     if (h->ev0) { (void)hipEventDestroy(h->ev0); (void)hipEventDestroy(h->ev1); (void)hipEventDestroy(h->ev2); }
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -920,6 +924,138 @@ extern "C" int clowns_solver_assemble(clowns_solver *h, const clowns_grid_dev *g
     h->have_t = false;
     h->raw_valid = keep_raw != 0;
     HIPCHK(hipStreamSynchronize(s));
+    return CLOWNS_OK;
+}
+
+namespace {
+__global__ void __launch_bounds__(256)
+k_cls_ring(int nx, int ny, int nz, const int8_t *__restrict__ solid, int *__restrict__ bad) {
+    const long X = nx + 2, Y = ny + 2, Z = nz + 2;
+    const long c = (long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= X * Y * Z) return;
+    const long i = c % X, j = (c / X) % Y, k = c / (X * Y);
+    if ((i == 0 || j == 0 || k == 0 || i == X - 1 || j == Y - 1 || k == Z - 1) && !solid[c]) atomicOr(bad, 1);
+}
+
+template <typename T> int to_host(std::vector<T> &v, const T *d, size_t n, hipStream_t s) {
+    v.resize(n);
+    HIPCHK(hipMemcpyAsync(v.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    return CLOWNS_OK;
+}
+template <typename T> int to_dev(T *d, const std::vector<T> &v, size_t n, hipStream_t s) {
+    HIPCHK(hipMemcpyAsync(d, v.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+    return CLOWNS_OK;
+}
+} // namespace
+
+extern "C" int clowns_solver_classify(clowns_solver *h, const clowns_grid_desc *g, const int8_t *d_solid,
+                                      double *d_F, const double *d_fb, const double *d_a, const double *d_dx,
+                                      const double *d_xf, int8_t *d_nf, int8_t *d_nb, int32_t *d_mno,
+                                      int32_t *d_row2cell, double *d_sp_row, int32_t cap, int32_t *nrows,
+                                      double *mp, int32_t *host_fallback) {
+    if (!h || !g || !d_solid || !d_F || !d_fb || !d_a || !d_dx || !d_xf || !d_nf || !d_nb || !d_mno ||
+        !d_row2cell || !d_sp_row || !nrows || !mp)
+        return CLOWNS_EBADARG;
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1 || g->stride < std::max(g->nx, std::max(g->ny, g->nz)) + 2)
+        return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    ClsGrid G;
+    G.nx = g->nx;
+    G.ny = g->ny;
+    G.nz = g->nz;
+    G.X = g->nx + 2;
+    G.Y = g->ny + 2;
+    G.stride = g->stride;
+    G.ncell = (long)G.X * G.Y * (g->nz + 2);
+    G.solid = d_solid;
+    G.F = d_F;
+    G.fb = d_fb;
+    G.a = d_a;
+    G.dx = d_dx;
+    G.xf = d_xf;
+    G.nf = d_nf;
+    G.nb = d_nb;
+    const int nblk = cdiv(G.ncell, 256);
+    const int nscan = cdiv(G.ncell, (long)SCAN_BS * SCAN_PER);
+    if ((rc = h->cls_scratch.ensure(8 + (size_t)nscan + 1))) return rc;
+    int *ext = h->cls_scratch.p, *changed = ext + 2, *bcount = ext + 8;
+    int hv[5];
+    fill<int>(s, ext, 8, 0);
+    k_cls_ring<<<nblk, 256, 0, s>>>(g->nx, g->ny, g->nz, d_solid, changed);
+    HIPCHK(hipMemcpyAsync(hv, changed, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hv[0]) return CLOWNS_EBADARG;  // the outer ring must be solid
+    k_cls_init<<<nblk, 256, 0, s>>>(G, ext);
+    HIPCHK(hipMemcpyAsync(hv, ext, 5 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int imax = std::min(g->nx, hv[0] + 1), kmax = std::min(g->nz, hv[1] + 1);
+    // A negative F is clipped during the serial first pass (nfd2015.F90:45),
+    // which later passes read: the exact host path handles that input.
+    bool fallback = hv[4] != 0;
+    if (!fallback) k_cls_clip<<<nblk, 256, 0, s>>>(G.ncell, d_nf, d_F);
+    for (int pass = 0; !fallback; ++pass) {
+        k_cls_surface<<<nblk, 256, 0, s>>>(G, imax, kmax);
+        fill<int>(s, changed, 1, 0);
+        k_cls_unsupported<<<nblk, 256, 0, s>>>(G, changed);
+        HIPCHK(hipMemcpyAsync(hv, changed, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (hv[0] && pass < 10000) continue;  // nfd2015.F90:128-131: restart
+        fill<int>(s, changed, 1, 0);
+        k_cls_onesided<<<nblk, 256, 0, s>>>(G, changed);
+        k_cls_orient<<<nblk, 256, 0, s>>>(G, changed + 1);
+        HIPCHK(hipMemcpyAsync(hv, changed, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        fallback = hv[0] || hv[1] || h->cls_force_host;
+        break;
+    }
+    // dx/xf are tiny: mp = (smallest cell size)^2 on the host
+    std::vector<double> hdx;
+    if ((rc = to_host(hdx, d_dx, (size_t)3 * g->stride, s))) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    double hmin = 1e300;
+    const int n3[3] = {g->nx, g->ny, g->nz};
+    for (int l = 0; l < 3; ++l)
+        for (int i = 1; i <= n3[l]; ++i) {
+            const double v = hdx[(size_t)l * g->stride + i];
+            if (v > 0.0 && v < hmin) hmin = v;
+        }
+    *mp = hmin * hmin;
+    if (host_fallback) *host_fallback = fallback ? 1 : 0;
+    if (fallback) {
+        // an order-dependent sweep changes flags: the exact serial pass on the host
+        const size_t nc = (size_t)G.ncell;
+        std::vector<int8_t> sol, nf(nc), nb(nc);
+        std::vector<double> F, fb, a, xf, sp(cap);
+        std::vector<int32_t> mno(nc), r2c(cap);
+        if ((rc = to_host(sol, d_solid, nc, s)) || (rc = to_host(F, (const double *)d_F, nc, s)) ||
+            (rc = to_host(fb, d_fb, nc, s)) || (rc = to_host(a, d_a, 3 * nc, s)) ||
+            (rc = to_host(xf, d_xf, (size_t)3 * g->stride, s)))
+            return rc;
+        HIPCHK(hipStreamSynchronize(s));
+        int32_t N = 0;
+        double mph = 0;
+        rc = clowns_classify(g, sol.data(), F.data(), fb.data(), a.data(), hdx.data(), xf.data(), nf.data(),
+                             nb.data(), mno.data(), r2c.data(), sp.data(), cap, &N, &mph);
+        if (rc) return rc;
+        if ((rc = to_dev(d_F, F, nc, s)) || (rc = to_dev(d_nf, nf, nc, s)) || (rc = to_dev(d_nb, nb, nc, s)) ||
+            (rc = to_dev(d_mno, mno, nc, s)) || (rc = to_dev(d_row2cell, r2c, (size_t)N, s)) ||
+            (rc = to_dev(d_sp_row, sp, (size_t)N, s)))
+            return rc;
+        HIPCHK(hipStreamSynchronize(s));
+        *nrows = N;
+        return CLOWNS_OK;
+    }
+    k_cls_count<<<nscan, SCAN_BS, 0, s>>>(G, imax, kmax, bcount);
+    k_cls_scan_blocks<<<1, 1024, 0, s>>>(nscan, bcount);
+    k_cls_number<<<nscan, SCAN_BS, 0, s>>>(G, imax, kmax, bcount, cap, d_mno, d_row2cell, d_sp_row);
+    HIPCHK(hipGetLastError());
+    int N = 0;
+    HIPCHK(hipMemcpyAsync(&N, bcount + nscan, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (N > cap) return CLOWNS_EBADARG;
+    *nrows = N;
     return CLOWNS_OK;
 }
 

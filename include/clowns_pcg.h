@@ -152,6 +152,20 @@ int clowns_classify(const clowns_grid_desc *g, const int8_t *solid, double *F,
                     int32_t *row2cell, double *sp_row, int32_t cap,
                     int32_t *nrows, double *mp);
 
+/* Device form of clowns_classify on the solver's stream (all arrays on the
+ * GPU, same meaning; F modified in place as the reference does).  The
+ * order-independent passes of set_nflag run one thread per cell and the row
+ * numbering is a prefix sum in box order; if one of set_nflag's
+ * order-dependent sweeps (nfd2015.F90:133-153, 167-243) would change a flag,
+ * the exact serial pass runs on the host instead (*host_fallback = 1). */
+int clowns_solver_classify(clowns_solver *h, const clowns_grid_desc *g,
+                           const int8_t *d_solid, double *d_F,
+                           const double *d_fb, const double *d_a,
+                           const double *d_dx, const double *d_xf,
+                           int8_t *d_nf, int8_t *d_nb, int32_t *d_mno,
+                           int32_t *d_row2cell, double *d_sp_row, int32_t cap,
+                           int32_t *nrows, double *mp, int32_t *host_fallback);
+
 /* Device-resident grid for the assembly kernels (all pointers on the GPU). */
 typedef struct clowns_grid_dev {
     int32_t nx, ny, nz, stride, nrows;

@@ -1,8 +1,9 @@
 """GPU: mixed-precision solve and the adjoint (transpose) solve.
 
 Mixed: FP32 inner PM-Bi-CGSTAB + FP64 refinement must reach the reference's
-stopping value (sqrt(RN/BN) <= 1e-10, here on the true residual) and agree
-with the FP64 solve to 1e-9 relative L2.  Adjoint: lambda = A^-T g must
+stopping value (sqrt(RN/BN) <= 1e-10, here on the true residual) and be as
+accurate as the FP64 solve: error against a tight FP64 solve within
+max(1e-9, 2x the FP64 solve's).  Adjoint: lambda = A^-T g must
 satisfy A^T lambda = g (checked with a host sparse product of the unscaled
 transpose) and the duality <g, A^-1 b> = <A^-T g, b>.
 """
@@ -41,11 +42,13 @@ def test_mixed_matches_fp64(shape):
     b = t(s["B"])
     x64, i64 = sol.solve(b)
     x32, i32 = sol.solve(b, mixed=True)
+    xt, it = sol.solve(b, eps=2e-15)  # tight FP64 solve (stagnation exit)
     assert i64["ier"] == 0 and i32["ier"] == 0, (i64, i32)
     assert i32["err"] <= 1e-10
-    e = (x32 - x64).norm().item() / x64.norm().item()
-    print("mixed vs fp64 rel L2", e, i32, i64)
-    assert e < 1e-9
+    err = lambda x: (x - xt).norm().item() / xt.norm().item()
+    print("error vs tight: mixed", err(x32), "fp64", err(x64), i32, i64)
+    # both stop at the same rule; mixed must be as accurate as FP64
+    assert err(x32) <= max(1e-9, 2.0 * err(x64))
     sol.close()
 
 

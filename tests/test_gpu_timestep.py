@@ -44,3 +44,18 @@ def test_solitary_assembly_matches_oracle():
         assert np.array_equal(LL.cpu().numpy(), o["LL"][:, :N])
         assert np.array_equal(D.cpu().numpy(), o["D"][:N])
         assert np.array_equal(B.cpu().numpy(), o["B"][:N])
+
+
+def test_solitary_steps_device_classification():
+    """The same run with F and the classification on the GPU gives the same
+    rows and, to the solve tolerance, the same solutions as the
+    host-classified run (F is computed by torch there and by numpy here, so
+    the inputs may differ in the last bit)."""
+    a = SolitaryRun(96, 16, 20, dt_step=0.4)
+    b = SolitaryRun(96, 16, 20, dt_step=0.4, on_device=True)
+    for n in range(4):
+        xa, _, ta = a.step(n, mixed=False)
+        xb, _, tb = b.step(n, mixed=False)
+        assert ta["N"] == tb["N"] and not tb["host_fallback"]
+        assert torch.equal(a.grid.row2cell, b.grid.row2cell)
+        assert (xa - xb).norm().item() <= 1e-8 * xa.norm().item()
