@@ -28,6 +28,12 @@ class GridDev(ctypes.Structure):
                                                   "dx", "xf")]
 
 
+class PCInfo(ctypes.Structure):
+    _fields_ = [("passes", ctypes.c_int32), ("iters", ctypes.c_int32), ("ier", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("dmax_first", ctypes.c_double), ("dmax_last", ctypes.c_double),
+                ("ms", ctypes.c_double)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 L.SIGNATURES.update({
@@ -35,6 +41,8 @@ L.SIGNATURES.update({
                                                                                  ctypes.POINTER(ctypes.c_double)]),
     "clowns_assemble": (ctypes.c_int, [ctypes.POINTER(GridDev), _I, _P, _P, _P, _P, _P]),
     "clowns_solver_assemble": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), ctypes.c_double, _P, _I]),
+    "clowns_solver_pressure_correction": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), _P, _P, ctypes.c_double, _I,
+                                                         ctypes.c_double, _I, ctypes.POINTER(PCInfo)]),
     "clowns_solver_classify": (ctypes.c_int, [_P, ctypes.POINTER(GridDesc)] + [_P] * 11 +
                                [_I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
 })
@@ -180,6 +188,24 @@ class DeviceGrid:
                                         ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(stream)),
                 "clowns_assemble")
         return AL, LL, D, B
+
+    def pressure_correction(self, solver, p=None, d0=1e-5, max_passes=1000, eps=1e-10, mixed=False):
+        """The SMAC pressure-correction loop of one time step on the device
+        (clowns_solver_pressure_correction): corrects the face velocities in
+        self.t["un"] in place and the cell pressure p (zeros if None).
+        Returns (p, info dict)."""
+        torch = self._torch
+        if p is None:
+            p = torch.zeros(self.g.ncell, dtype=torch.float64, device=self.device)
+        info = PCInfo()
+        L.check(L.lib().clowns_solver_pressure_correction(
+            solver._h, ctypes.byref(self.dev), ctypes.c_void_p(self.t["un"].data_ptr()),
+            ctypes.c_void_p(p.data_ptr()), float(d0), int(max_passes), float(eps), 1 if mixed else 0,
+            ctypes.byref(info)), "clowns_solver_pressure_correction")
+        solver.N, solver.NL = self.N, 6
+        solver.lda = (self.N + 63) // 64 * 64
+        solver.version += 1
+        return p, {k: getattr(info, k) for k, _ in PCInfo._fields_ if k != "reserved"}
 
     def assemble_into(self, solver, parm=5.0, keep_raw=False, b=None):
         """Fused assembly + scaling + sort into the solver's resident operator.

@@ -168,4 +168,103 @@ k_assemble(GridDev g, int lda, double *__restrict__ AL, int *__restrict__ LL,
     }
 }
 
+// ---------------------------------------------- pressure-correction pass --
+// p(mno2mn(r)) -= xe(r)  (src2LOWNS/1sub-momentum.F90:53-58)
+__global__ void __launch_bounds__(256)
+k_pressure_update(int N, const int32_t *__restrict__ row2cell, const double *__restrict__ xe,
+                  double *__restrict__ p) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < N) p[row2cell[r]] -= xe[r];
+}
+
+// estimate_velocity2 (src2LOWNS/1sub-momentum.F90:590-651) for grids without
+// pressure-boundary objects: un(l,c) = un + dt*(pm - pc)/dxc on every open low
+// face of a non-air cell whose low neighbour is fluid or surface, with
+// pc = -xe(row of c), pm = -xe(row of the neighbour) (0 for an unnumbered
+// cell: the reference's never-written xe(0)), dxc = (dx_i + dx_{i-1})/2
+// (momentum_d, lib/src/math_func.F90:251-255); |un| < 1e-15 -> 0.
+__global__ void __launch_bounds__(256)
+k_velocity_correction(GridDev g, const double *__restrict__ xe, double *__restrict__ un) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.ncell) return;
+    const int fc = g.nf[c];
+    if (fc == -1 || fc == 0) return;
+    const int X = g.nx + 2, Y = g.ny + 2;
+    const int p[3] = {(int)(c % X), (int)((c / X) % Y), (int)(c / ((long)X * Y))};
+    const long step[3] = {1, X, (long)X * Y};
+    const int nc = g.mno[c];
+    const double xec = -(nc ? xe[nc - 1] : 0.0);
+    for (int l = 0; l < 3; ++l) {
+        const size_t o = (size_t)l * g.ncell + c;
+        if (g.a[o] == 0.0) continue;
+        const long m = c - step[l];
+        if (g.nf[m] < 1) continue;
+        const double dxc = (g.dx[(size_t)l * g.stride + p[l]] + g.dx[(size_t)l * g.stride + p[l] - 1]) * 0.5;
+        const int nm = g.mno[m];
+        const double pm = -(nm ? xe[nm - 1] : 0.0);
+        double v = un[o] + g.dt * (pm - xec) / dxc;
+        if (fabs(v) < 1e-15) v = 0.0;
+        un[o] = v;
+    }
+}
+
+// Right-hand side and DMAX of a pressure-correction pass for the fixed row
+// numbering (cal_err_of_continuity, 1sub-momentum.F90:315-359); per-block
+// maxima of |DR1| go to part.
+__global__ void __launch_bounds__(256)
+k_rhs(GridDev g, double *__restrict__ B, double *__restrict__ part) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    double dm = 0.0;
+    if (r < g.N) {
+        const long c = g.row2cell[r];
+        double b = 0.0;
+        if (g.nf[c] == 1) {
+            const int X = g.nx + 2, Y = g.ny + 2;
+            const int p[3] = {(int)(c % X), (int)((c / X) % Y), (int)(c / ((long)X * Y))};
+            const double fbc = g.fb[c];
+            double rn;
+            if (fbc != 0.0) {
+                const size_t n = (size_t)g.ncell;
+                auto A = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.a[l * n + cc]; };
+                auto U = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.un[l * n + cc]; };
+                const long cx = c + 1, cy = c + X, cz = c + (long)X * Y;
+                rn = ((U(0, cx) * A(0, cx) - g.un[c] * g.a[c]) / g.dx[p[0]] +
+                      (U(1, cy) * A(1, cy) - g.un[n + c] * g.a[n + c]) / g.dx[(size_t)g.stride + p[1]] +
+                      (U(2, cz) * A(2, cz) - g.un[2 * n + c] * g.a[2 * n + c]) / g.dx[2 * (size_t)g.stride + p[2]]) /
+                     fbc;
+            } else {
+                rn = -10000.0;
+            }
+            double ff = 1.0 - g.F[c];
+            ff = ff > 0.001 ? 0.001 : (ff < -0.001 ? -0.001 : ff);
+            const double dr1 = rn + ff / g.dt;
+            const double q = fabs(dr1 / g.dt);
+            if (!(q <= 1e-15 && q != 0.0)) b = g.mp * dr1 / g.dt;
+            dm = fabs(dr1);
+        }
+        B[r] = b;
+    }
+    __shared__ double lds[256];
+    lds[threadIdx.x] = dm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = lds[0];
+}
+
+__global__ void __launch_bounds__(1024) k_max_fin(const double *__restrict__ part, int G, double *__restrict__ out) {
+    __shared__ double lds[1024];
+    double m = 0.0;
+    for (int k = threadIdx.x; k < G; k += 1024) m = fmax(m, part[k]);
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (threadIdx.x < s) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = lds[0];
+}
+
 } // namespace clowns

@@ -414,6 +414,142 @@ k_pm2_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0,
     }
 }
 
+// SpMV with two consecutive rows per thread (same per-row arithmetic).
+template <typename T, int NL, int MODE>
+__global__ void __launch_bounds__(BS)
+k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
+          const int *__restrict__ LL, const T *__restrict__ D,
+          const T *__restrict__ v0, const T *__restrict__ v1,
+          const T *__restrict__ v2, T *__restrict__ vout, T *__restrict__ y) {
+    if (sc->done) return;
+    T c0 = T(0), c1 = T(0);
+    if (MODE == 1) {
+        c0 = (T)sc->BETA;
+        c1 = (T)sc->AMU;
+    } else if (MODE == 2) {
+        c0 = (T)sc->ALP;
+    }
+    auto X = [&](int j) -> T {
+        if (MODE == 1) return v1[j] + c0 * (v0[j] - c1 * v2[j]);
+        if (MODE == 2) return v1[j] - c0 * v2[j];
+        return v0[j];
+    };
+    const int i = (logical_block() * BS + threadIdx.x) * 2;
+    if (i + 1 < N) {
+        typename V2<T>::t a[NL];
+        int2 l[NL];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            l[k] = ld2(LL + (size_t)k * lda + i);
+            a[k] = ld2(AL + (size_t)k * lda + i);
+        }
+        const auto d = ld2(D + i);
+        T x0, x1;
+        if (MODE == 2) {
+            const auto r = ld2(v1 + i), q = ld2(v2 + i);
+            x0 = r.x - c0 * q.x;
+            x1 = r.y - c0 * q.y;
+        } else if (MODE == 1) {
+            const auto r = ld2(v1 + i), p = ld2(v0 + i), q = ld2(v2 + i);
+            x0 = r.x + c0 * (p.x - c1 * q.x);
+            x1 = r.y + c0 * (p.y - c1 * q.y);
+        } else {
+            const auto xv = ld2(v0 + i);
+            x0 = xv.x;
+            x1 = xv.y;
+        }
+        T acc0 = x0 * d.x, acc1 = x1 * d.y;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            acc0 = acc0 + a[k].x * X(l[k].x - 1);
+            acc1 = acc1 + a[k].y * X(l[k].y - 1);
+        }
+        typename V2<T>::t o;
+        if (MODE == 1 || MODE == 2) {
+            o.x = x0;
+            o.y = x1;
+            *reinterpret_cast<typename V2<T>::t *>(vout + i) = o;
+        }
+        if (MODE == 3) {
+            const auto bv = ld2(v1 + i);
+            o.x = bv.x - acc0;
+            o.y = bv.y - acc1;
+        } else {
+            o.x = acc0;
+            o.y = acc1;
+        }
+        *reinterpret_cast<typename V2<T>::t *>(y + i) = o;
+    } else if (i < N) {
+        const T xi = X(i);
+        T acc = xi * D[i];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) acc = acc + AL[(size_t)k * lda + i] * X(LL[(size_t)k * lda + i] - 1);
+        if (MODE == 1 || MODE == 2) vout[i] = xi;
+        y[i] = MODE == 3 ? v1[i] - acc : acc;
+    }
+}
+
+// x/r update with two consecutive rows per thread.
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_update_v2(const Scal *__restrict__ sc, int N, T *__restrict__ x,
+            const T *__restrict__ p, const T *__restrict__ e,
+            const T *__restrict__ v, T *__restrict__ r,
+            const T *__restrict__ rh, double *__restrict__ part) {
+    if (sc->done) return;
+    const T alp = (T)sc->ALP, amu = (T)sc->AMU;
+    double rn = 0.0, c1 = 0.0;
+    const int lb = logical_block();
+    const int i = (lb * BS + threadIdx.x) * 2;
+    auto row = [&](T xi, T pi, T ei, T vi, T rhi, T &xo, T &ro) {
+        T t = ei;
+        xo = xi + alp * pi + amu * t;
+        t = t - amu * vi;
+        ro = t;
+        rn += (double)t * (double)t;
+        c1 += (double)t * (double)rhi;
+    };
+    if (i + 1 < N) {
+        const auto xv = ld2(x + i), pv = ld2(p + i), ev = ld2(e + i), vv = ld2(v + i), hv = ld2(rh + i);
+        typename V2<T>::t xo, ro;
+        row(xv.x, pv.x, ev.x, vv.x, hv.x, xo.x, ro.x);
+        row(xv.y, pv.y, ev.y, vv.y, hv.y, xo.y, ro.y);
+        *reinterpret_cast<typename V2<T>::t *>(x + i) = xo;
+        *reinterpret_cast<typename V2<T>::t *>(r + i) = ro;
+    } else if (i < N) {
+        T xo, ro;
+        row(x[i], p[i], e[i], v[i], rh[i], xo, ro);
+        x[i] = xo;
+        r[i] = ro;
+    }
+    __shared__ double lds[BS / 64];
+    const double t0 = block_sum<BS>(rn, lds);
+    const double t1 = block_sum<BS>(c1, lds);
+    if (threadIdx.x == 0) {
+        part[lb] = t0;
+        part[gridDim.x + lb] = t1;
+    }
+}
+
+// P = R + BETA*(P - AMU*Q), two rows per thread.
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_pupd_v2(const Scal *__restrict__ sc, int N, T *__restrict__ p,
+          const T *__restrict__ r, const T *__restrict__ q) {
+    if (sc->done) return;
+    const int i = (logical_block() * BS + threadIdx.x) * 2;
+    const T beta = (T)sc->BETA, amu = (T)sc->AMU;
+    if (i + 1 < N) {
+        const auto pv = ld2(p + i), rv = ld2(r + i), qv = ld2(q + i);
+        typename V2<T>::t o;
+        o.x = rv.x + beta * (pv.x - amu * qv.x);
+        o.y = rv.y + beta * (pv.y - amu * qv.y);
+        *reinterpret_cast<typename V2<T>::t *>(p + i) = o;
+    } else if (i < N) {
+        p[i] = r[i] + beta * (p[i] - amu * q[i]);
+    }
+}
+
 // ---------------------------------------------------------- update (a11) --
 // X = X + ALP*P + AMU*E;  R = E - AMU*V;  RN += R*R;  C1 += R*R^.
 template <typename T>

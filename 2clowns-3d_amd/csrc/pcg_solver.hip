@@ -110,6 +110,7 @@ struct clowns_solver {
     DevBuf<double> part;   // per-block partial sums
     DevBuf<int> flag;      // device error flag
     DevBuf<int> cls_scratch;  // classification: extents, change flag, block counts
+    DevBuf<double> pc_b, pc_x, pc_part;  // pressure-correction loop: RHS, xe, DMAX partials
     DevBuf<Scal> sc;       // [0] inner/FP64 run, [1] outer (mixed)
     Scal *h_sc = nullptr;  // pinned mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
@@ -176,6 +177,10 @@ int h_fuse_pm = -1;
 // both precisions (CLOWNS_V2_32 / CLOWNS_V2_64 = 0 to switch off).
 bool h_v2_32 = true, h_v2_64 = true;
 template <typename T> bool use_v2() { return sizeof(T) == 4 ? h_v2_32 : h_v2_64; }
+// Two rows per thread in the SpMV, p update and x/r update (measured
+// separately from the PM stages; CLOWNS_V2VEC = 0/1).
+bool h_v2_vec = true;
+template <typename T> bool use_v2_vec() { return h_v2_vec; }
 
 template <typename T>
 void launch_pm2(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j, const T *in, T *out) {
@@ -260,9 +265,28 @@ int apply_pm(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, const T *in,
     return launch_pm_stage<T, 1>(s, g, sc, A, 3, t, out, dot, w, part);
 }
 
+template <typename T, int NL>
+void launch_spmv_v2(hipStream_t s, const Scal *sc, const Op<T> &A, int mode, const T *v0, const T *v1,
+                    const T *v2, T *vout, T *y) {
+    const int G2 = geo_of(A.N, 2).G;
+    switch (mode) {
+    case 0: k_spmv_v2<T, NL, 0><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, A.LL, A.D, v0, v1, v2, vout, y); break;
+    case 1: k_spmv_v2<T, NL, 1><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, A.LL, A.D, v0, v1, v2, vout, y); break;
+    case 2: k_spmv_v2<T, NL, 2><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, A.LL, A.D, v0, v1, v2, vout, y); break;
+    default: k_spmv_v2<T, NL, 3><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, A.LL, A.D, v0, v1, v2, vout, y); break;
+    }
+}
+
 template <typename T>
 void spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
           const T *v0, const T *v1, const T *v2, T *vout, T *y) {
+    if (use_v2_vec<T>()) {
+        if (A.NL == 6)
+            launch_spmv_v2<T, 6>(s, sc, A, mode, v0, v1, v2, vout, y);
+        else
+            launch_spmv_v2<T, 4>(s, sc, A, mode, v0, v1, v2, vout, y);
+        return;
+    }
     if (A.NL == 6)
         launch_spmv<T, 6>(s, g, sc, A, mode, v0, v1, v2, vout, y);
     else
@@ -286,8 +310,12 @@ int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc,
         for (; K <= kend; ++K) {
             // Q = PM*A*P, with P = R + BETA*(P - AMU*Q) fused for K > 1.
             if (K == 1 || !h->fuse_pupd) {
-                if (K > 1)
-                    k_pupd<T><<<geo_of(A.N, 1).G, BS, 0, s>>>(sc, A.N, pc, V.r.p, V.q.p);
+                if (K > 1) {
+                    if (use_v2_vec<T>())
+                        k_pupd_v2<T><<<geo_of(A.N, 2).G, BS, 0, s>>>(sc, A.N, pc, V.r.p, V.q.p);
+                    else
+                        k_pupd<T><<<geo_of(A.N, 1).G, BS, 0, s>>>(sc, A.N, pc, V.r.p, V.q.p);
+                }
                 spmv<T>(s, g, sc, A, 0, pc, nullptr, nullptr, nullptr, V.y.p);
             } else {
                 spmv<T>(s, g, sc, A, 1, pc, V.r.p, V.q.p, pn, V.y.p);
@@ -299,9 +327,14 @@ int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc,
             spmv<T>(s, g, sc, A, 2, nullptr, V.r.p, V.q.p, V.e.p, V.y.p);
             const int Gv = apply_pm<T>(s, g, sc, A, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
             k_fin_omega<<<1, FIN_BS, 0, s>>>(sc, part, Gv);
-            k_update<T><<<g.G, BS, 0, s>>>(sc, A.N, g.R, V.x.p, pc, V.e.p, V.v.p,
-                                          V.r.p, V.rh.p, part);
-            k_fin_beta<<<1, FIN_BS, 0, s>>>(sc, part, g.G, K);
+            int Gu = g.G;
+            if (use_v2_vec<T>()) {
+                Gu = geo_of(A.N, 2).G;
+                k_update_v2<T><<<Gu, BS, 0, s>>>(sc, A.N, V.x.p, pc, V.e.p, V.v.p, V.r.p, V.rh.p, part);
+            } else {
+                k_update<T><<<g.G, BS, 0, s>>>(sc, A.N, g.R, V.x.p, pc, V.e.p, V.v.p, V.r.p, V.rh.p, part);
+            }
+            k_fin_beta<<<1, FIN_BS, 0, s>>>(sc, part, Gu, K);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(h->h_sc + sc_index, sc, sizeof(Scal), hipMemcpyDeviceToHost, s));
@@ -595,6 +628,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_FUSE_PM")) h_fuse_pm = atoi(e) != 0 ? 1 : 0;
     if (const char *e = getenv("CLOWNS_V2_32")) h_v2_32 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
+    if (const char *e = getenv("CLOWNS_V2VEC")) h_v2_vec = atoi(e) != 0;
     *out = h;
     return CLOWNS_OK;
 }
@@ -607,7 +641,7 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     h->AL32.release(); h->D32.release();
     h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
     h->AL0.release(); h->D0.release(); h->LL0.release();
-    h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->cls_scratch.release(); h->sc.release();
+    h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->cls_scratch.release(); h->pc_b.release(); h->pc_x.release(); h->pc_part.release(); h->sc.release();
     if (h->h_sc) (void)hipHostFree(h->h_sc);
     if (h->ev0) { (void)hipEventDestroy(h->ev0); (void)hipEventDestroy(h->ev1); (void)hipEventDestroy(h->ev2); }
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -1056,6 +1090,64 @@ extern "C" int clowns_solver_classify(clowns_solver *h, const clowns_grid_desc *
     HIPCHK(hipStreamSynchronize(s));
     if (N > cap) return CLOWNS_EBADARG;
     *nrows = N;
+    return CLOWNS_OK;
+}
+
+extern "C" int clowns_solver_pressure_correction(clowns_solver *h, const clowns_grid_dev *g, double *d_un,
+                                                 double *d_p, double d0, int32_t max_passes, double eps,
+                                                 int32_t flags, clowns_pc_info *info) {
+    if (!h || !grid_ok(g) || !d_un || !d_p || max_passes < 1) return CLOWNS_EBADARG;
+    if (flags & ~CLOWNS_MIXED) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    clowns_grid_dev gg = *g;
+    gg.un = d_un;
+    GridDev G = grid_of(&gg);
+    const int N = g->nrows;
+    // the operator depends only on the flags, F and sp: assembled (fused with
+    // scaling and sort) once and resident for every pass
+    if ((rc = h->pc_b.ensure(N)) || (rc = h->pc_x.ensure(N))) return rc;
+    if ((rc = clowns_solver_assemble(h, &gg, 5.0, h->pc_b.p, 0))) return rc;
+    const int G1 = cdiv(N, 256);
+    if ((rc = h->pc_part.ensure((size_t)G1 + 1))) return rc;
+    double *dmax_dev = h->pc_part.p + G1;
+    clowns_pc_info out{};
+    HIPCHK(hipEventRecord(h->ev0, s));
+    int ik = 0;
+    for (;;) {
+        // cal_err_of_continuity: RHS and DMAX from the current velocities
+        k_rhs<<<G1, 256, 0, s>>>(G, h->pc_b.p, h->pc_part.p);
+        k_max_fin<<<1, 1024, 0, s>>>(h->pc_part.p, G1, dmax_dev);
+        double dmax = 0.0;
+        HIPCHK(hipMemcpyAsync(&dmax, dmax_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (ik == 0) out.dmax_first = dmax;
+        out.dmax_last = dmax;
+        if (dmax < d0) break;        // check_dmax (1sub-momentum.F90:437-441)
+        if (++ik > max_passes) {     // (1sub-momentum.F90:448-454)
+            out.ier = CLOWNS_NOCONV;
+            break;
+        }
+        clowns_info si{};
+        rc = clowns_solver_solve(h, h->pc_b.p, h->pc_x.p, eps, 10000, flags & CLOWNS_MIXED, &si);
+        if (rc) return rc;
+        out.iters += si.iters;
+        if (si.ier != 0 && si.ier != CLOWNS_NOCONV) {
+            out.ier = si.ier;
+            break;
+        }
+        k_pressure_update<<<G1, 256, 0, s>>>(N, g->row2cell, h->pc_x.p, d_p);
+        k_velocity_correction<<<cdiv(G.ncell, 256), 256, 0, s>>>(G, h->pc_x.p, d_un);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(h->ev1, s));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    out.passes = ik;
+    out.ms = ms;
+    if (info) *info = out;
     return CLOWNS_OK;
 }
 

@@ -154,7 +154,7 @@ def main():
     if args.profile_kernels:
         dg.assemble_into(sol, b=b)
         res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
-               (("k_spmv<double,6,0>", 0), ("k_pm<double,1,0>", 1), ("k_copy<double>", 4))}
+               (("k_spmv_v2<double,6,0>", 0), ("k_pm_v2<double,1,0>", 1), ("k_copy<double>", 4))}
         print(json.dumps({"rows": N, "avg_us": res}))
         return
 
@@ -201,12 +201,12 @@ def main():
     t_pm = sol.time_kernel(1, 50)
     t_copy = sol.time_kernel(4, 50)
     if 2 * t_spmv >= 8 * t_pm:
-        kname, kms, kbytes = "k_spmv<double,6,0>", t_spmv, ALG_BYTES["spmv"] * N
+        kname, kms, kbytes = "k_spmv_v2<double,6,0>", t_spmv, ALG_BYTES["spmv"] * N
     else:
-        kname, kms, kbytes = "k_pm<double,1,0>", t_pm, ALG_BYTES["pm_stage"] * N
+        kname, kms, kbytes = "k_pm_v2<double,1,0>", t_pm, ALG_BYTES["pm_stage"] * N
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "traffic_config2.json")
+    pmc = os.path.join(ROOT, "profiles", f"traffic_config{args.config}.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
             traffic = json.load(f).get(kname)

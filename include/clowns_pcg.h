@@ -192,6 +192,30 @@ int clowns_assemble(const clowns_grid_dev *g, int32_t lda, double *d_AL,
 int clowns_solver_assemble(clowns_solver *h, const clowns_grid_dev *g,
                            double parm, double *d_B, int32_t keep_raw);
 
+/* The SMAC pressure-correction loop of one time step on the device
+ * (src2LOWNS/1sub-momentum.F90:41-60 with check_dmax :419-463): the operator
+ * is assembled once (fused with PCGPME's scaling and sort) and stays
+ * resident; each pass computes the RHS and DMAX from d_un
+ * (cal_err_of_continuity), stops when DMAX < d0, solves for xe (x0 = 0,
+ * stopping rule eps; CLOWNS_MIXED selects mixed precision), updates
+ * d_p(cell) -= xe and corrects d_un (estimate_velocity2, :590-651; grids
+ * without pressure-boundary objects).  d_un is 3*ncell low-face velocities
+ * (used instead of g->un), d_p per cell.  More than max_passes solves ->
+ * info->ier = 3000 (the reference stops at ik > 1000). */
+typedef struct clowns_pc_info {
+    int32_t passes;      /* solves done */
+    int32_t iters;       /* total Bi-CGSTAB iterations */
+    int32_t ier;
+    int32_t reserved;
+    double dmax_first, dmax_last;  /* DMAX before the first and after the last pass */
+    double ms;           /* device time of the loop (HIP events) */
+} clowns_pc_info;
+
+int clowns_solver_pressure_correction(clowns_solver *h, const clowns_grid_dev *g,
+                                      double *d_un, double *d_p, double d0,
+                                      int32_t max_passes, double eps,
+                                      int32_t flags, clowns_pc_info *info);
+
 /* Library build string, e.g. "clowns_pcg gfx950 ...". */
 const char *clowns_version(void);
 

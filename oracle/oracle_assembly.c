@@ -410,3 +410,81 @@ int oracle_build_system(int nx, int ny, int nz, const int8_t *solid, const doubl
     free(g.F); free(g.fln); free(g.sfn);
     return N;
 }
+
+/*
+ * One SMAC pressure-correction pass after a solve (oracle):
+ *   p(mno2mn(r)) -= xe(r)            src2LOWNS/1sub-momentum.F90:53-58
+ *   estimate_velocity2               src2LOWNS/1sub-momentum.F90:590-651
+ *     un(l,c) = momentum_d(un, -xe(nno(l)), -xe(nnn), dxc, dt)
+ *             = un + dt*(pm - pc)/dxc, dxc = (dx(l,i) + dx(l,i-1))/2,
+ *     for every non-air, non-solid cell c and axis l with a(l,c) != 0 and a
+ *     low neighbour flagged >= 1; |un| < 1e-15 -> 0.  (lib/src/math_func.F90:251-255)
+ *   The reference reads xe(0) for an unnumbered cell; PCGPME never writes
+ *   it and the caller allocates it without a value, so here it is 0.
+ * and the RHS + DMAX of the next pass (1sub-momentum.F90:315-359) for the
+ * given row numbering (rows numbered once per time step, ik = 0).
+ * mno, row2cell, nf (flags) come from oracle_build_system; un is
+ * 3*ncell (low-face velocities), p per cell.
+ */
+void oracle_velocity_correction(int nx, int ny, int nz, const int8_t *solid, const int8_t *nf,
+                                const double *a, const double *dx, int dxstride, double dt,
+                                const int *mno, const double *xe, double *un) {
+    const int X = nx + 2, Y = ny + 2, Z = nz + 2;
+    const long ncell = (long)X * Y * Z;
+    const long step[3] = {1, X, (long)X * Y};
+    for (int k = 0; k < Z; ++k)
+        for (int j = 0; j < Y; ++j)
+            for (int i = 0; i < X; ++i) {
+                long c = i + (long)X * (j + (long)Y * k);
+                if (solid[c] || nf[c] == 0) continue;
+                int idx[3] = {i, j, k};
+                double xec = -(mno[c] ? xe[mno[c] - 1] : 0.0);
+                for (int l = 0; l < 3; ++l) {
+                    if (a[(size_t)l * ncell + c] == 0.0) continue;
+                    long m = c - step[l];
+                    int fm = solid[m] ? -1 : nf[m];
+                    if (fm >= 1) {
+                        double dxc = (dx[(size_t)l * dxstride + idx[l]] + dx[(size_t)l * dxstride + idx[l] - 1]) * 0.5;
+                        double pm = -(mno[m] ? xe[mno[m] - 1] : 0.0);
+                        double v = un[(size_t)l * ncell + c] + dt * (pm - xec) / dxc;
+                        if (fabs(v) < 1e-15) v = 0.0;
+                        un[(size_t)l * ncell + c] = v;
+                    }
+                }
+            }
+}
+
+/* RHS b (per row) and DMAX of a pass for a fixed numbering. */
+double oracle_rhs(int nx, int ny, int nz, const int8_t *solid, const int8_t *nf, const double *F,
+                  const double *fb, const double *a, const double *un, const double *dx, int dxstride,
+                  double dt, double mp, int N, const int *row2cell, double *B) {
+    const int X = nx + 2, Y = ny + 2;
+    const long ncell = (long)X * Y * (nz + 2);
+    double dmax = 0.0;
+    for (int r = 0; r < N; ++r) {
+        long c = row2cell[r];
+        B[r] = 0.0;
+        if (nf[c] != 1) continue;
+        int i = (int)(c % X), j = (int)((c / X) % Y), k = (int)(c / ((long)X * Y));
+        double rn;
+        if (fb[c] != 0.0) {
+            long cx = c + 1, cy = c + X, cz = c + (long)X * Y;
+            double axp = solid[cx] ? 0.0 : a[cx], ayp = solid[cy] ? 0.0 : a[ncell + cy];
+            double azp = solid[cz] ? 0.0 : a[2 * ncell + cz];
+            double up = solid[cx] ? 0.0 : un[cx], vp = solid[cy] ? 0.0 : un[ncell + cy];
+            double wp = solid[cz] ? 0.0 : un[2 * ncell + cz];
+            rn = ((up * axp - un[c] * a[c]) / dx[i] + (vp * ayp - un[ncell + c] * a[ncell + c]) / dx[dxstride + j] +
+                  (wp * azp - un[2 * ncell + c] * a[2 * ncell + c]) / dx[2 * dxstride + k]) / fb[c];
+        } else {
+            rn = -10000.0;
+        }
+        double ff = 1.0 - F[c];
+        if (ff > 0.001) ff = 0.001;
+        if (ff < -0.001) ff = -0.001;
+        double dr1 = rn + ff / dt;
+        double q = fabs(dr1 / dt);
+        if (!(q <= 1e-15 && q != 0.0)) B[r] = mp * dr1 / dt;
+        if (fabs(dr1) > dmax) dmax = fabs(dr1);
+    }
+    return dmax;
+}

@@ -152,3 +152,50 @@ subroutine ref_build_system(nx, ny, nz, solid, fvv, fbv, av, unv, dxv, xfv, stri
         Bb(r) = b(r)
     enddo
 end subroutine ref_build_system
+
+! One pressure-correction pass of the reference after ref_build_system on
+! the same grid (module state persists): xe(1:mnoe) = the given solution,
+! xe(0) = 0, then p(mno2mn(r)) -= xe(r) (1sub-momentum.F90:53-58),
+! estimate_velocity2 (1sub-momentum.F90:590-651), and the next pass's
+! cal_err_of_continuity with ik > 0 (RHS and DMAX for the same numbering).
+! Returns un (box layout, 3*ncell), p (per cell) and b, DMAX.
+subroutine ref_correction_pass(nx, ny, nz, xe_in, unv, pv, Bb, dmax_out) &
+        bind(C, name="ref_correction_pass")
+    use iso_c_binding
+    use variables, only: inne2, mnoe, ik, dmax
+    use arrays, only: in, jn, kn, un, xe, b, p, mno2mn
+    implicit none
+    integer(c_int), value :: nx, ny, nz
+    real(c_double), intent(in) :: xe_in(*)
+    real(c_double), intent(inout) :: unv(0:*), pv(0:*)
+    real(c_double), intent(out) :: Bb(*), dmax_out
+    interface
+        subroutine estimate_velocity2()
+        end subroutine
+        subroutine cal_err_of_continuity()
+        end subroutine
+    end interface
+    integer :: XX, YY, ncell, nn, l, c, r
+    XX = nx + 2 ; YY = ny + 2 ; ncell = XX * YY * (nz + 2)
+    do nn = 1, inne2
+        c = in(nn) + XX*(jn(nn) + YY*kn(nn))
+        p(nn) = pv(c)
+    enddo
+    xe(0) = 0d0
+    xe(1:mnoe) = xe_in(1:mnoe)
+    do r = 1, mnoe
+        p(mno2mn(r)) = p(mno2mn(r)) - xe(r)
+    enddo
+    call estimate_velocity2
+    ik = 1
+    call cal_err_of_continuity
+    do nn = 1, inne2
+        c = in(nn) + XX*(jn(nn) + YY*kn(nn))
+        pv(c) = p(nn)
+        do l = 0, 2
+            unv(l*ncell + c) = un(l, nn)
+        enddo
+    enddo
+    Bb(1:mnoe) = b(1:mnoe)
+    dmax_out = dmax
+end subroutine ref_correction_pass
