@@ -219,4 +219,86 @@ def config5(t=0.0, seed=2026):
     return config_solitary(1024, 512, 80, t=t, seed=seed)
 
 
+def dispersion_2dh(nx, ny, seed=2027, dx=(8.0, 12.0), depth=10.0, hump=None, a_disp=0.4,
+                   b_disp=1.0 / 15.0, g_acc=9.80665, nbumps=12):
+    """PCGPME_WILL inputs: the 2DH linear-dispersion matrix as disp_param
+    builds it (src2LOWNS/1sub-2DH-initial_reset.f90:472-505) on an nx*ny
+    grid of cells i=1..nx, j=1..ny, and a right-hand side PSI of the shape
+    disp_adjust gives it (1sub-2DH-momentum.f90:247-315, linear case
+    PSI = (A - B) g ZK^2 hxx).
+
+    Rows are the cells with bed ZK < 0, numbered i fastest (the reference
+    numbers them in its cell-list order, 1sub-2DH-initial_reset.f90:229-235).
+    Per row: x neighbours are skipped on the first and last column and y
+    neighbours on the first and last row; each existing neighbour (x-, x+,
+    y-, y+ order) takes the next slot, LL packed with zeros after the last,
+    AL = A*2*ZK*ZK/DX/(DX + DX_neighbour) and DIAG = -1 - sum(AL).  The bed
+    slopes from `depth` offshore to dry land over the last ~10% in x, with
+    seeded Gaussian bumps; spacings are seeded in `dx` per column/row.  The
+    free surface is a Gaussian hump (centre, radius, amplitude) cut to
+    exactly zero beyond three radii, so PSI is exactly zero there and the
+    solution decays through the output pass's 1e-14 limit.
+    Returns dict(AL, LL (NA,4) Fortran order, D, B, N, NA=N, NL=4, mnd)."""
+    rng = np.random.default_rng(seed)
+    X, Y = nx + 2, ny + 2
+    dx1 = np.broadcast_to(rng.uniform(dx[0], dx[1], size=X)[None, :], (Y, X)).copy()
+    dx2 = np.broadcast_to(rng.uniform(dx[0], dx[1], size=Y)[:, None], (Y, X)).copy()
+    x = np.concatenate([[0.0], np.cumsum(dx1[0, 1:])])  # cell positions (approximate centres)
+    y = np.concatenate([[0.0], np.cumsum(dx2[1:, 0])])
+    Lx, Ly = x[-1], y[-1]
+    xx, yy = np.meshgrid(x, y)
+    zk = -depth * (1.0 - 1.1 * xx / Lx)
+    for _ in range(nbumps):
+        cx, cy = rng.uniform(0, Lx), rng.uniform(0, Ly)
+        w = rng.uniform(0.03, 0.1) * min(Lx, Ly)
+        zk = zk + rng.uniform(0.05, 0.25) * depth * np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / w ** 2)
+    ii, jj = np.meshgrid(np.arange(X), np.arange(Y))
+    inner = (ii >= 1) & (ii <= nx) & (jj >= 1) & (jj <= ny)
+    wet = inner & (zk < 0.0)
+    mnd = np.zeros((Y, X), dtype=np.int64)
+    mnd[wet] = np.arange(1, int(wet.sum()) + 1)  # row-major over (j, i): i fastest
+    N = int(wet.sum())
+    # surface hump and PSI
+    if hump is None:
+        hump = (0.35 * Lx, 0.5 * Ly, 0.08 * min(Lx, Ly), 0.1 * depth)
+    cx, cy, rad, amp = hump
+    r2 = ((xx - cx) ** 2 + (yy - cy) ** 2) / rad ** 2
+    eta = np.where(r2 <= 9.0, amp * np.exp(-r2), 0.0)
+    hxx = np.zeros((Y, X))
+    for d, (sp, ax) in enumerate(((dx1, 1), (dx2, 0))):
+        em, ep = np.roll(eta, 1, axis=ax), np.roll(eta, -1, axis=ax)
+        sm, spp = np.roll(sp, 1, axis=ax), np.roll(sp, -1, axis=ax)
+        hm, hp = 0.5 * (sm + sp), 0.5 * (sp + spp)
+        term = 2.0 * (hm * ep - (hm + hp) * eta + hp * em) / (hm * hp * (hm + hp))
+        edge = (ii == 1) | (ii == nx) if d == 0 else (jj == 1) | (jj == ny)
+        hxx = hxx + np.where(edge, 0.0, term)
+    psi = (a_disp - b_disp) * g_acc * zk * zk * hxx
+    # matrix
+    rows = mnd[wet] - 1
+    ci, cj = ii[wet], jj[wet]
+    zc = zk[wet]
+    AL = np.zeros((N, 4), order="F")
+    LL = np.zeros((N, 4), dtype=np.int32, order="F")
+    slot = np.zeros(N, dtype=np.int64)
+    cand = []
+    for d, (di, dj) in enumerate(((-1, 0), (1, 0), (0, -1), (0, 1))):
+        sp = dx1 if d < 2 else dx2
+        skip = ((ci == 1) | (ci == nx)) if d < 2 else ((cj == 1) | (cj == ny))
+        nb = mnd[cj + dj, ci + di]
+        ok = (~skip) & (nb != 0)
+        own = sp[cj, ci]
+        other = sp[cj + dj, ci + di]
+        val = a_disp * 2.0 * zc * zc / own / (own + other)
+        cand.append((ok, nb, val))
+    for ok, nb, val in cand:
+        r = rows[ok]
+        s = slot[ok]
+        AL[r, s] = val[ok]
+        LL[r, s] = nb[ok]
+        slot[ok] += 1
+    D = -1.0 - (((AL[:, 0] + AL[:, 1]) + AL[:, 2]) + AL[:, 3])
+    B = np.ascontiguousarray(psi[wet])
+    return dict(AL=AL, LL=LL, D=D, B=B, N=N, NA=N, NL=4, mnd=mnd)
+
+
 CONFIGS = {1: config1, 2: config2, 3: config3}

@@ -5,7 +5,10 @@
 (src2LOWNS/pcgpme_para.F90:1-58, 250-257) on numpy arrays laid out as the
 Fortran caller holds them: AL/LL column-major (NA, NL), LL 1-based.  Like the
 reference it mutates AL, LL, D, B, X and EPS in place and returns
-(ITER, IER).  ``Solver`` is the device-resident form on torch tensors.
+(ITER, IER).  ``pcgpme_will`` does the same for the 2DH dispersion solve
+``PCGPME_WILL(AL,NL,NA,LL,D,N,N2,B,EPS,PARM,ITER)`` (src2LOWNS/pcgpme_2DH.f90:1-41):
+only B changes (right-hand side in, filtered solution out).  ``Solver`` is
+the device-resident form on torch tensors.
 """
 import ctypes
 
@@ -44,6 +47,22 @@ def pcgpme(AL, NL, NA, LL, D, N, N2, B, X, EPS, PARM, ITER):
                           _ptr(B), _ptr(X), _ptr(EPS), ctypes.byref(ctypes.c_double(PARM)),
                           ctypes.byref(it), ctypes.byref(ier))
     return it.value, ier.value
+
+
+def pcgpme_will(AL, NL, NA, LL, D, N, N2, B, EPS, PARM, ITER):
+    """Drop-in of PCGPME_WILL through ``clowns_pcgpme_will``.  B is
+    overwritten with the reference's output; returns IER (0, 3000 or a
+    CLOWNS_E* code -- the reference itself has no status argument)."""
+    _check_host("AL", AL, np.float64, (NA, NL), fortran=True)
+    _check_host("LL", LL, np.int32, (NA, NL), fortran=True)
+    _check_host("D", D, np.float64, (N,))
+    _check_host("B", B, np.float64, (N,))
+    ier = ctypes.c_int32(0)
+    i32 = lambda v: ctypes.byref(ctypes.c_int32(int(v)))
+    f64 = lambda v: ctypes.byref(ctypes.c_double(float(v)))
+    L.lib().clowns_pcgpme_will(_ptr(AL), i32(NL), i32(NA), _ptr(LL), _ptr(D), i32(N), i32(N2),
+                               _ptr(B), f64(EPS), f64(PARM), i32(ITER), ctypes.byref(ier))
+    return ier.value
 
 
 def _dptr(t):
@@ -109,6 +128,39 @@ class Solver:
         L.check(L.lib().clowns_solver_solve(self._h, _dptr(b), _dptr(x), float(eps), int(maxiter), flags,
                                             ctypes.byref(info)), "clowns_solver_solve")
         return x, info.as_dict()
+
+    def solve_will(self, b, out=None, eps=1e-10, iters=1000):
+        """PCGPME_WILL on the resident operator (set_matrix): zero start, EPS
+        and ITER as given, then the reference's output pass.  Returns
+        (out, info) with info["filter_rounds"] the device rounds of that pass
+        (-1: it ran on the host)."""
+        torch = self._torch
+        if out is None:
+            out = torch.empty_like(b)
+        info = L.Info()
+        rounds = ctypes.c_int32(0)
+        L.check(L.lib().clowns_solver_solve_will(self._h, _dptr(b), _dptr(out), float(eps), int(iters),
+                                                 ctypes.byref(info), ctypes.byref(rounds)),
+                "clowns_solver_solve_will")
+        d = info.as_dict()
+        d["filter_rounds"] = rounds.value
+        return out, d
+
+    def will_filter(self, LL, X, R):
+        """PCGPME_WILL's output pass alone: LL (NL, lda) int32 caller order,
+        X and R (N,) float64.  Returns (X_filtered, B, rounds)."""
+        torch = self._torch
+        NL, lda = LL.shape
+        N = X.shape[0]
+        for name, t, dt in (("LL", LL, torch.int32), ("X", X, torch.float64), ("R", R, torch.float64)):
+            if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
+                raise TypeError(f"{name} must be a contiguous CUDA {dt} tensor")
+        xo = torch.empty_like(X)
+        bo = torch.empty_like(R)
+        rounds = ctypes.c_int32(0)
+        L.check(L.lib().clowns_will_filter(self._h, _dptr(LL), NL, lda, N, _dptr(X), _dptr(R), _dptr(xo),
+                                           _dptr(bo), ctypes.byref(rounds)), "clowns_will_filter")
+        return xo, bo, rounds.value
 
     def solve_transpose(self, g, lam=None, eps=1e-10, maxiter=10000, warm=False):
         torch = self._torch

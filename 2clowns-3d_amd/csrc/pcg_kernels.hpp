@@ -43,7 +43,8 @@ struct Scal {
     int done;          // 1 = converged / exited: every later kernel returns
     int ier;
     int kout;          // ITER returned (the reference returns ITER+1 on 3000)
-    int pad_[3];
+    int beta_prod;     // 1: BETA = C1/(AMU*RQ) (pcgpme_2DH.f90:241), else C1/AMU/RQ
+    int pad_[2];
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -632,11 +633,12 @@ __device__ __forceinline__ double fin_sum(const double *__restrict__ p, int G) {
 }
 
 // Reset the run's scalars (also makes a stale `done` from an earlier run go).
-__global__ void k_reset(Scal *sc, double eps, int itermax) {
+__global__ void k_reset(Scal *sc, double eps, int itermax, int beta_prod) {
     if (threadIdx.x != 0) return;
     Scal s = {};
     s.EPS = eps;
     s.itermax = itermax;
+    s.beta_prod = beta_prod;
     *sc = s;
 }
 
@@ -709,7 +711,9 @@ k_fin_beta(Scal *sc, const double *__restrict__ part, int G, int K) {
         if (sc->AMU == 0.0 || c1 == 0.0) {
             stop = 1;
         } else {
-            sc->BETA = c1 / sc->AMU / sc->RQ;
+            // Same quantity, two roundings: pcgpme_para.F90:241 divides
+            // twice, pcgpme_2DH.f90:241 divides by the product.
+            sc->BETA = sc->beta_prod ? c1 / (sc->AMU * sc->RQ) : c1 / sc->AMU / sc->RQ;
             if (K >= sc->itermax) {
                 sc->ier = 3000;
                 sc->kout = K + 1;
@@ -806,6 +810,74 @@ k_grad_coef(int N, int lda, int lda_out, int NL, const int *__restrict__ LL,
         if (gAL) gAL[(size_t)k * lda_out + j] = (i >= 1 && i <= N) ? -l * x[i - 1] : 0.0;
     }
     if (gD) gD[j] = -l * x[j];
+}
+
+// -------------------------------------------------- PCGPME_WILL output --
+// The output pass of PCGPME_WILL (src2LOWNS/pcgpme_2DH.f90:255-268), serial
+// in the reference: row i zeroes |X(i)| <= 1e-14, then walks its caller-order
+// neighbours LL(i,1..NL) (skipping itself, stopping at the first 0) and
+// snaps X(i) to a neighbour within 1e-14; B(i) receives X(i) at every
+// neighbour that differs from it and otherwise keeps the final residual.
+// Row i reads the already filtered X of neighbours j < i and the raw X of
+// neighbours j > i.  On the device the filtered vector is the fixed point of
+// Y <- F(Y), F(Y)_i = row i's pass with neighbours j < i read from Y: the
+// dependence is on lower rows only, so the fixed point is unique, equals the
+// serial result, and is reached after (longest chain of snaps) + 1 rounds.
+constexpr double WILL_LIMIT = 1e-14;
+
+__device__ __forceinline__ double will_row(int i, int lda, int NL, const int *__restrict__ LL,
+                                           const double *__restrict__ X, const double *__restrict__ Y,
+                                           bool snap_y, bool &assigned) {
+    double xi = X[i];
+    if (fabs(xi) <= WILL_LIMIT) xi = 0.0;
+    assigned = false;
+    for (int k = 0; k < NL; ++k) {
+        const int j = LL[(size_t)k * lda + i];
+        if (j == i + 1) continue;
+        if (j == 0) break;
+        double xj;
+        if (j - 1 < i) {
+            xj = Y[j - 1];
+            if (snap_y && fabs(xj) <= WILL_LIMIT) xj = 0.0;
+        } else {
+            xj = X[j - 1];
+        }
+        if (xi == xj) continue;
+        if (fabs(xi - xj) <= WILL_LIMIT) xi = xj;
+        assigned = true;
+    }
+    return xi;
+}
+
+// One round: Yout = F(Yin).  Round 0 reads Yin = X with the zero snap applied
+// (the value a lower row has when it snaps to no neighbour).  flags[r] = 1 if
+// any row changed; a round whose predecessor changed nothing returns (both
+// buffers already hold the fixed point).
+__global__ void __launch_bounds__(BS)
+k_will_round(int N, int lda, int NL, const int *__restrict__ LL, const double *__restrict__ X,
+             const double *__restrict__ Yin, double *__restrict__ Yout, int *__restrict__ flags, int r) {
+    if (r > 0 && flags[r - 1] == 0) return;
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= N) return;
+    bool as;
+    const double y = will_row(i, lda, NL, LL, X, Yin, r == 0, as);
+    double prev = Yin[i];
+    if (r == 0 && fabs(prev) <= WILL_LIMIT) prev = 0.0;
+    Yout[i] = y;
+    if (__double_as_longlong(y) != __double_as_longlong(prev)) flags[r] = 1;
+}
+
+// Outputs from the fixed point Y: X' (optional) and B.
+__global__ void __launch_bounds__(BS)
+k_will_out(int N, int lda, int NL, const int *__restrict__ LL, const double *__restrict__ X,
+           const double *__restrict__ Y, const double *__restrict__ R, double *__restrict__ Xout,
+           double *__restrict__ B) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= N) return;
+    bool as;
+    const double y = will_row(i, lda, NL, LL, X, Y, false, as);
+    if (Xout) Xout[i] = y;
+    B[i] = as ? y : R[i];
 }
 
 } // namespace clowns

@@ -109,6 +109,7 @@ struct clowns_solver {
     DevBuf<double> stage_b, stage_x;  // clowns_pcgpme host staging
     DevBuf<double> part;   // per-block partial sums
     DevBuf<int> flag;      // device error flag
+    DevBuf<int> will_flags;  // PCGPME_WILL output pass: per-round change flags
     DevBuf<int> cls_scratch;  // classification: extents, change flag, block counts
     DevBuf<double> pc_b, pc_x, pc_part;  // pressure-correction loop: RHS, xe, DMAX partials
     DevBuf<Scal> sc;       // [0] inner/FP64 run, [1] outer (mixed)
@@ -120,6 +121,7 @@ struct clowns_solver {
     double tau_min = 1e-3;  // mixed: smallest relative tolerance of an inner solve
     bool cls_force_host = false;  // test hook: always take the exact host path
     int poll = 8;          // iterations between host polls of `done`
+    int will_max_rounds = 4096;  // PCGPME_WILL output pass: device rounds before the host pass
 };
 
 namespace {
@@ -437,8 +439,10 @@ template <typename T> int ensure_vecs(clowns_solver *h, VecSet<T> &V, int N) {
 
 // FP64 PCGPME on the resident operator `A` with row scales `Wsc`: exactly the
 // reference's sequence (setup :115-145, iteration :148-247).
+// beta_prod selects PCGPME_WILL's rounding of BETA (pcgpme_2DH.f90:241).
 int solve_fp64(clowns_solver *h, const Op<double> &A, const double *Wsc, const double *d_b,
-               double *d_x, double eps, int itermax, bool warm, clowns_info *info) {
+               double *d_x, double eps, int itermax, bool warm, clowns_info *info,
+               bool beta_prod = false) {
     hipStream_t s = h->stream;
     int rc;
     VecSet<double> &V = h->v64;
@@ -448,7 +452,7 @@ int solve_fp64(clowns_solver *h, const Op<double> &A, const double *Wsc, const d
     Scal *sc = h->sc.p;
     double *part = h->part.p;
     HIPCHK(hipEventRecord(h->ev0, s));
-    k_reset<<<1, 64, 0, s>>>(sc, eps, itermax);
+    k_reset<<<1, 64, 0, s>>>(sc, eps, itermax, beta_prod ? 1 : 0);
     // WK1 = W*B; X = x0.
     k_scale_b<<<cdiv(N, BS), BS, 0, s>>>(N, d_b, Wsc, V.p0.p, warm ? d_x : nullptr, V.x.p);
     // WK1 = PM*WK1 (:123).
@@ -503,8 +507,8 @@ int solve_mixed(clowns_solver *h, const Op<double> &A, const Op<float> &A32, con
     Scal *sc = h->sc.p, *outer = h->sc.p + 1;
     double *part = h->part.p;
     HIPCHK(hipEventRecord(h->ev0, s));
-    k_reset<<<1, 64, 0, s>>>(outer, eps, itermax);
-    k_reset<<<1, 64, 0, s>>>(sc, eps, itermax);
+    k_reset<<<1, 64, 0, s>>>(outer, eps, itermax, 0);
+    k_reset<<<1, 64, 0, s>>>(sc, eps, itermax, 0);
     // bs = W*b ; BN = ||PM*bs||^2.
     k_scale_b<<<cdiv(N, BS), BS, 0, s>>>(N, d_b, Wsc, h->bs.p, warm ? d_x : nullptr, V.x.p);
     const int Gb = apply_pm<double>(s, g, sc, A, h->bs.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
@@ -547,7 +551,7 @@ int solve_mixed(clowns_solver *h, const Op<double> &A, const Op<float> &A32, con
             if (total_inner >= itermax) break;
             // Inner solve of (M A) d = r/||r|| to relative tolerance tau.
             double tau = std::max(tau_min, 0.3 * eps / err);
-            k_reset<<<1, 64, 0, s>>>(sc, tau, std::max(1, itermax - total_inner));
+            k_reset<<<1, 64, 0, s>>>(sc, tau, std::max(1, itermax - total_inner), 0);
             k_rhs32<<<cdiv(N, BS), BS, 0, s>>>(N, outer, V.r.p, F.p0.p);
             fill<float>(s, F.x.p, (size_t)N, 0.f);
             k_init<float><<<g.G, BS, 0, s>>>(sc, N, g.R, F.p0.p, nullptr, F.r.p, F.rh.p, part);
@@ -558,7 +562,7 @@ int solve_mixed(clowns_solver *h, const Op<double> &A, const Op<float> &A32, con
             ++refinements;
             // x += ||r|| d ; r = PM*(bs - A x) ; ERR.
             k_correct<<<cdiv(N, BS), BS, 0, s>>>(N, outer, F.x.p, V.x.p);
-            k_reset<<<1, 64, 0, s>>>(sc, eps, itermax);
+            k_reset<<<1, 64, 0, s>>>(sc, eps, itermax, 0);
             spmv<double>(s, g, sc, A, 3, V.x.p, h->bs.p, nullptr, nullptr, V.y.p);
             const int Gr = apply_pm<double>(s, g, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
             k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, Gr);
@@ -625,6 +629,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_FUSE_PUPD")) h->fuse_pupd = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_TAU_MIN")) h->tau_min = std::max(1e-8, atof(e));
     if (const char *e = getenv("CLOWNS_CLASSIFY_HOST")) h->cls_force_host = atoi(e) != 0;
+    if (const char *e = getenv("CLOWNS_WILL_MAX_ROUNDS")) h->will_max_rounds = std::max(1, atoi(e));
     if (const char *e = getenv("CLOWNS_FUSE_PM")) h_fuse_pm = atoi(e) != 0 ? 1 : 0;
     if (const char *e = getenv("CLOWNS_V2_32")) h_v2_32 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
@@ -641,7 +646,7 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     h->AL32.release(); h->D32.release();
     h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
     h->AL0.release(); h->D0.release(); h->LL0.release();
-    h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->cls_scratch.release(); h->pc_b.release(); h->pc_x.release(); h->pc_part.release(); h->sc.release();
+    h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->will_flags.release(); h->cls_scratch.release(); h->pc_b.release(); h->pc_x.release(); h->pc_part.release(); h->sc.release();
     if (h->h_sc) (void)hipHostFree(h->h_sc);
     if (h->ev0) { (void)hipEventDestroy(h->ev0); (void)hipEventDestroy(h->ev1); (void)hipEventDestroy(h->ev2); }
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -786,6 +791,183 @@ extern "C" void clowns_pcgpme(double *AL, const int32_t *NL_, const int32_t *NA_
     *IER = info.ier;
 }
 
+// ================================================================ PCGPME_WILL
+
+namespace {
+
+// Device rounds of the output pass before the exact host pass takes over (a
+// chain of snaps longer than this is the only way to reach it); the solver's
+// will_max_rounds may lower it.
+constexpr int WILL_MAX_ROUNDS = 4096;
+
+// PCGPME_WILL's output pass in the reference's serial order
+// (pcgpme_2DH.f90:255-268), for the capped case.
+void will_host(int N, int lda, int NL, const std::vector<int> &LL, std::vector<double> &X,
+               std::vector<double> &B) {
+    for (int i = 0; i < N; ++i) {
+        if (std::fabs(X[i]) <= WILL_LIMIT) X[i] = 0.0;
+        for (int k = 0; k < NL; ++k) {
+            const int j = LL[(size_t)k * lda + i];
+            if (j == i + 1) continue;
+            if (j == 0) break;
+            if (X[i] == X[j - 1]) continue;
+            if (std::fabs(X[i] - X[j - 1]) <= WILL_LIMIT) X[i] = X[j - 1];
+            B[i] = X[i];
+        }
+    }
+}
+
+// X' and B of the output pass from the solution X and the final residual R
+// (B may be R).  LL: caller order, 1-based, column stride lda, entries in
+// [0, N] (checked by the caller).  *rounds: device rounds, or -1 when the
+// host pass ran.
+int will_filter(clowns_solver *h, const int *d_LL, int NL, int lda, int N, const double *d_X,
+                const double *d_R, double *d_Xout, double *d_B, int *rounds) {
+    hipStream_t s = h->stream;
+    int rc;
+    if ((rc = ensure_vecs(h, h->v64, N)) || (rc = h->will_flags.ensure(WILL_MAX_ROUNDS))) return rc;
+    double *buf[2] = {h->v64.y.p, h->v64.t.p};
+    const int cap = std::max(1, std::min(WILL_MAX_ROUNDS, h->will_max_rounds));
+    fill<int>(s, h->will_flags.p, WILL_MAX_ROUNDS, 0);
+    const int G = cdiv(N, BS);
+    int launched = 0, batch = 4, r0 = -1;
+    int hf[256];
+    while (launched < cap) {
+        const int nb = std::min(batch, cap - launched);
+        for (int r = launched; r < launched + nb; ++r)
+            k_will_round<<<G, BS, 0, s>>>(N, lda, NL, d_LL, d_X, r == 0 ? d_X : buf[(r - 1) & 1],
+                                          buf[r & 1], h->will_flags.p, r);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(hf, h->will_flags.p + launched, (size_t)nb * sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int r = 0; r < nb && r0 < 0; ++r)
+            if (hf[r] == 0) r0 = launched + r;
+        launched += nb;
+        if (r0 >= 0) break;
+        batch = std::min(256, batch * 2);
+    }
+    if (r0 >= 0) {
+        k_will_out<<<G, BS, 0, s>>>(N, lda, NL, d_LL, d_X, buf[r0 & 1], d_R, d_Xout, d_B);
+        HIPCHK(hipGetLastError());
+        if (rounds) *rounds = r0 + 1;
+        return CLOWNS_OK;
+    }
+    std::vector<int> LL((size_t)lda * NL);
+    std::vector<double> X(N), B(N);
+    HIPCHK(hipMemcpy2DAsync(LL.data(), (size_t)lda * sizeof(int), d_LL, (size_t)lda * sizeof(int),
+                            (size_t)N * sizeof(int), NL, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(X.data(), d_X, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(B.data(), d_R, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    will_host(N, lda, NL, LL, X, B);
+    if (d_Xout)
+        HIPCHK(hipMemcpyAsync(d_Xout, X.data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_B, B.data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (rounds) *rounds = -1;
+    return CLOWNS_OK;
+}
+
+// PCGPME_WILL on the resident operator: zero start, the reference's EPS and
+// ITER as given (no defaults), BETA as C1/(AMU*RQ), then the output pass over
+// the caller-order LL kept by load_matrix.  d_out receives B.
+int solve_will(clowns_solver *h, const double *d_b, double *d_out, double *d_xout, double eps,
+               int iter, clowns_info *info, int *rounds) {
+    if (!h->raw_valid) return CLOWNS_EBADARG;
+    int rc;
+    const int N = h->N;
+    if ((rc = h->stage_x.ensure(N))) return rc;
+    Op<double> A{N, h->NL, h->lda, h->AL.p, h->LL.p, h->D.p};
+    clowns_info inf{};
+    if ((rc = solve_fp64(h, A, h->W.p, d_b, h->stage_x.p, eps, iter, false, &inf, true))) return rc;
+    // No iteration ran: the reference falls through to IER = 3000
+    // (pcgpme_2DH.f90:248-249); a zero right-hand side returns at once.
+    if (iter <= 0 && !(h->h_sc[0].done && h->h_sc[0].BN == 0.0)) inf.ier = CLOWNS_NOCONV;
+    if ((rc = will_filter(h, h->LL0.p, h->NL, h->lda, N, h->stage_x.p, h->v64.r.p, d_xout, d_out,
+                          rounds)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (info) *info = inf;
+    return CLOWNS_OK;
+}
+
+} // namespace
+
+extern "C" void clowns_pcgpme_will(const double *AL, const int32_t *NL_, const int32_t *NA_,
+                                   const int32_t *LL, const double *D, const int32_t *N_,
+                                   const int32_t *N2_, double *B, const double *EPS,
+                                   const double *PARM, const int32_t *ITER, int32_t *IER) {
+    if (!IER) return;
+    if (!AL || !NL_ || !NA_ || !LL || !D || !N_ || !N2_ || !B || !EPS || !PARM || !ITER) {
+        *IER = CLOWNS_EBADARG;
+        return;
+    }
+    const int NL = *NL_, NA = *NA_, N = *N_, N2 = *N2_;
+    if (check_nl(NL)) {
+        *IER = CLOWNS_EBADNL;
+        return;
+    }
+    if (N < 1 || NA < N || N2 < 64) {
+        *IER = CLOWNS_EBADARG;
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        *IER = CLOWNS_EHIP;
+        return;
+    }
+    clowns_solver *h = default_solver(dev);
+    if (!h || set_device(h)) {
+        *IER = CLOWNS_EHIP;
+        return;
+    }
+    hipStream_t s = h->stream;
+    // AL, LL and D are inputs only (the reference scales copies,
+    // pcgpme_2DH.f90:56-76); keep_raw holds the caller-order LL.
+    int rc = load_matrix(h, AL, NL, NA, LL, D, N, *PARM, hipMemcpyHostToDevice, true);
+    if (!rc) rc = h->stage_b.ensure(N);
+    if (!rc && hipMemcpyAsync(h->stage_b.p, B, (size_t)N * sizeof(double), hipMemcpyHostToDevice, s))
+        rc = CLOWNS_EHIP;
+    clowns_info info{};
+    if (!rc) rc = solve_will(h, h->stage_b.p, h->stage_b.p, nullptr, *EPS, *ITER, &info, nullptr);
+    if (!rc && (hipMemcpyAsync(B, h->stage_b.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s) ||
+                hipStreamSynchronize(s)))
+        rc = CLOWNS_EHIP;
+    *IER = rc ? rc : info.ier;
+}
+
+extern "C" int clowns_solver_solve_will(clowns_solver *h, const double *d_b, double *d_out,
+                                        double eps, int32_t iter, clowns_info *info,
+                                        int32_t *filter_rounds) {
+    if (!h || !d_b || !d_out || h->N < 1) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    return solve_will(h, d_b, d_out, nullptr, eps, iter, info, filter_rounds);
+}
+
+extern "C" int clowns_will_filter(clowns_solver *h, const int32_t *d_LL, int32_t NL, int32_t lda,
+                                  int32_t N, const double *d_X, const double *d_R, double *d_Xout,
+                                  double *d_B, int32_t *rounds) {
+    if (!h || !d_LL || !d_X || !d_R || !d_B || N < 1 || lda < N || NL < 1 || NL > 8)
+        return CLOWNS_EBADARG;
+    if (d_Xout == d_X) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    if ((rc = h->flag.ensure(1))) return rc;
+    fill<int>(s, h->flag.p, 1, 0);
+    k_check_ll<<<cdiv(N, BS), BS, 0, s>>>(N, lda, NL, d_LL, h->flag.p);
+    int hbad = 0;
+    HIPCHK(hipMemcpyAsync(&hbad, h->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hbad) return CLOWNS_EBADARG;
+    if ((rc = will_filter(h, d_LL, NL, lda, N, d_X, d_R, d_Xout, d_B, rounds))) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    return CLOWNS_OK;
+}
+
 extern "C" int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32_t reps,
                                          double *avg_ms) {
     if (!h || h->N < 1 || reps < 1 || !avg_ms) return CLOWNS_EBADARG;
@@ -798,7 +980,7 @@ extern "C" int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32
         if ((rc = ensure_f32(h)) || (rc = ensure_vecs(h, h->v32, N))) return rc;
     }
     Scal *sc = h->sc.p;
-    k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1);
+    k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1, 0);
     const Geo g = geo_of(N, h->R);
     Op<double> A{N, h->NL, h->lda, h->AL.p, h->LL.p, h->D.p};
     Op<float> A32{N, h->NL, h->lda, h->AL32.p, h->LL.p, h->D32.p};

@@ -33,3 +33,39 @@ subroutine PCGPME(AL,NL,NA,LL,D,N,N2,B,X,EPS,PARM,ITER,IER)
     end interface
     call clowns_pcgpme(AL,NL,NA,LL,D,N,N2,B,X,EPS,PARM,ITER,IER)
 end subroutine PCGPME
+
+! Replaces PCGPME_WILL (src2LOWNS/pcgpme_2DH.f90:1,28-41), the 2DH dispersion
+! solve called by disp_adjust (1sub-2DH-momentum.f90:325-326), with the same
+! name and argument list.  B returns what the reference leaves in it (the
+! filtered solution, pcgpme_2DH.f90:255-268); AL, LL, D are not modified.
+! The reference has no status argument: on non-convergence it writes a line
+! to unit 6 (pcgpme_2DH.f90:248), as this shim does; a library error stops.
+subroutine PCGPME_WILL(AL,NL,NA,LL,D,N,N2,B,EPS,PARM,ITER)
+    use iso_c_binding, only: c_double, c_int32_t
+    implicit none
+    integer, intent(in) :: N, N2, NA, NL
+    real(8), intent(in) :: PARM, EPS
+    integer, intent(in) :: ITER
+    real(8), dimension(NA,NL), intent(in) :: AL
+    real(8), dimension(N), intent(in) :: D
+    real(8), dimension(N), intent(inout) :: B
+    integer, dimension(NA,NL), intent(in) :: LL
+    integer :: IER
+    interface
+        subroutine clowns_pcgpme_will(AL,NL,NA,LL,D,N,N2,B,EPS,PARM,ITER,IER) &
+                bind(C, name="clowns_pcgpme_will")
+            import :: c_double, c_int32_t
+            real(c_double) :: AL(*), D(*), B(*)
+            integer(c_int32_t) :: NL, NA, N, N2, ITER, IER
+            integer(c_int32_t) :: LL(*)
+            real(c_double) :: EPS, PARM
+        end subroutine clowns_pcgpme_will
+    end interface
+    call clowns_pcgpme_will(AL,NL,NA,LL,D,N,N2,B,EPS,PARM,ITER,IER)
+    if (IER == 3000) then
+        write(6,*) 'PCGPME_WILL: no convergence in ITER =', ITER
+    else if (IER /= 0) then
+        write(6,*) 'PCGPME_WILL: clowns_pcgpme_will failed, IER =', IER
+        stop 1
+    end if
+end subroutine PCGPME_WILL

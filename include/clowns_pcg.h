@@ -216,6 +216,43 @@ int clowns_solver_pressure_correction(clowns_solver *h, const clowns_grid_dev *g
                                       int32_t max_passes, double eps,
                                       int32_t flags, clowns_pc_info *info);
 
+/* ---- 2DH dispersion solve: PCGPME_WILL ----------------------------------
+ * Replaces PCGPME_WILL(AL,NL,NA,LL,D,N,N2,B,EPS,PARM,ITER)
+ * (src2LOWNS/pcgpme_2DH.f90:1-41, interface 0mod-type_interface.F90:156-164),
+ * called by disp_adjust (1sub-2DH-momentum.f90:325-326) with NL = 4,
+ * NA = N, N2 = 64, PARM = 5, ITER = 1000.  The same PM-Bi-CGSTAB as PCGPME
+ * with the reference's differences: AL, LL, D are inputs only; X starts at
+ * zero; EPS and ITER are used as given (no defaults; ITER <= 0 runs no
+ * iteration); BETA = C1/(AMU*RQ); and the output pass
+ * (pcgpme_2DH.f90:255-268): |X(i)| <= 1e-14 -> 0, X(i) snapped to a
+ * neighbour within 1e-14 in caller order, serial over i, and B(i) = X(i) for
+ * every row with a differing neighbour, else B(i) keeps the final scaled
+ * residual.  B: right-hand side in, that output out.  IER (not in the
+ * reference's argument list, which has no status) gets 0 / 3000 / CLOWNS_E*. */
+void clowns_pcgpme_will(const double *AL, const int32_t *NL, const int32_t *NA,
+                        const int32_t *LL, const double *D, const int32_t *N,
+                        const int32_t *N2, double *B, const double *EPS,
+                        const double *PARM, const int32_t *ITER, int32_t *IER);
+
+/* PCGPME_WILL on the operator last passed to clowns_solver_set_matrix (or
+ * clowns_solver_assemble with keep_raw): d_b unscaled right-hand side, d_out
+ * the reference's B on return (may be d_b).  *filter_rounds (may be NULL):
+ * device rounds of the output pass, -1 if it ran on the host (a chain of
+ * more than 4096 successive snaps). */
+int clowns_solver_solve_will(clowns_solver *h, const double *d_b, double *d_out,
+                             double eps, int32_t iter, clowns_info *info,
+                             int32_t *filter_rounds);
+
+/* The output pass of PCGPME_WILL alone, on the device: from the solution
+ * d_X and final residual d_R (length N) and the caller-order LL (column
+ * stride lda, entries in [0, N], NL <= 8), writes the filtered X to d_Xout
+ * (may be NULL; must not be d_X) and B to d_B (may be d_R).  Exact: the
+ * result equals the serial pass bit for bit. */
+int clowns_will_filter(clowns_solver *h, const int32_t *d_LL, int32_t NL,
+                       int32_t lda, int32_t N, const double *d_X,
+                       const double *d_R, double *d_Xout, double *d_B,
+                       int32_t *rounds);
+
 /* Library build string, e.g. "clowns_pcg gfx950 ...". */
 const char *clowns_version(void);
 

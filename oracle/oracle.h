@@ -12,4 +12,10 @@ void oracle_ldusub(const double *AL, int NL, int NA, const int *LL, int N,
 int oracle_pcgpme(double *AL, int NL, int NA, int *LL, double *D, int N,
                   int N2, double *B, double *X, double *EPS, double PARM,
                   int *ITER, int *IER);
+void oracle_will_filter(const int *LL, int NL, int NA, int N, double *X,
+                        double *B);
+int oracle_pcgpme_will(const double *AL, int NL, int NA, const int *LL,
+                       const double *D, int N, int N2, double *B, double EPS,
+                       double PARM, int ITER, int *IER, int *K_out,
+                       double *X_raw, double *X_out);
 #endif

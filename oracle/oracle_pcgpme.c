@@ -163,19 +163,18 @@ void oracle_ldusub(const double *AL, int NL, int NA, const int *LL, int N,
 /* Deterministic chunk-ordered combination of per-thread partial sums. */
 #define MAXT 1024
 
-/* PCGPME(AL,NL,NA,LL,D,N,N2,B,X,EPS,PARM,ITER,IER), pcgpme_para.F90:1-257.
- * Same arguments (by value for scalars), same in-place mutation of AL, LL, D
- * and B, same exits.  Returns 0, or -1 if NL is not 4 or 6 (reference: STOP)
- * or N2 < 64 (the reference would index past X). */
-int oracle_pcgpme(double *AL, int NL, int NA, int *LL, double *D, int N,
-                  int N2, double *B, double *X, double *EPS, double PARM,
-                  int *ITER, int *IER) {
-    if (NL != 6 && NL != 4) return -1;
-    if (N2 < 64 || g_nthr > MAXT) return -1;
+/* The preconditioned iteration shared by PCGPME and PCGPME_WILL on an
+ * already scaled and sorted system: setup pcgpme_para.F90:115-145
+ * (pcgpme_2DH.f90:121-150), iteration :148-247 (:153-247).  The two
+ * differ only in BETA's rounding: beta_prod = 0 gives C1/AMU/RQ
+ * (pcgpme_para.F90:241), 1 gives C1/(AMU*RQ) (pcgpme_2DH.f90:241).
+ * itermax <= 0 runs no iteration.  Returns 0, or -2 if out of memory. */
+static int pcgpme_iterate(const double *AL, int NL, int NA, const int *LL,
+                          const double *D, int N, int N2, double *B, double *X,
+                          double eps, int itermax, int beta_prod, int *K_out,
+                          double *err_out, int *ier_out) {
     const double ZERO = 0.0;
-    if (EPS[0] <= 1e-15) EPS[0] = 1e-9;                           /* :51 */
-    if (*ITER <= 0) *ITER = (int)(10.0 * (sqrt((double)N) + 1.0)); /* :52 */
-    *IER = 0;
+    int IER = 0;
     double ERRP = ZERO, ERR = ZERO;
     size_t L = (size_t)N + (size_t)N2;
     double *WK = (double *)calloc(L * 5, sizeof(double));
@@ -188,7 +187,6 @@ int oracle_pcgpme(double *AL, int NL, int NA, int *LL, double *D, int N,
     double *WK1 = WK, *WK2 = WK + L, *WK3 = WK + 2 * L, *WK4 = WK + 3 * L,
            *WK5 = WK + 4 * L;
 
-    oracle_scale_sort(AL, NL, NA, LL, D, N, B, PARM);
     for (int i = 0; i < N; ++i) WK1[i] = B[i];
 
     oracle_ldusub(AL, NL, NA, LL, N, WK1, WK1, W);   /* B = PM*B   :123 */
@@ -221,7 +219,7 @@ int oracle_pcgpme(double *AL, int NL, int NA, int *LL, double *D, int N,
     }
 
     int K;
-    for (K = 1; K <= *ITER; ++K) {
+    for (K = 1; K <= itermax; ++K) {
         oracle_axsub(AL, NL, NA, LL, D, N, WK1, WK3);  /* Q = PM*A*P :150 */
         oracle_ldusub(AL, NL, NA, LL, N, WK3, WK3, W);
         double RQ = ZERO;
@@ -286,11 +284,11 @@ int oracle_pcgpme(double *AL, int NL, int NA, int *LL, double *D, int N,
         }
         ERR = sqrt(RN / BN);                                    /* :215 */
         if (K >= 10 && fabs(ERR - ERRP) <= ERR * 1.0e-6) goto done; /* :228 */
-        if (ERR <= EPS[0]) goto done;                           /* :232 */
+        if (ERR <= eps) goto done;                              /* :232 */
         ERRP = ERR;
         if (AMU == ZERO) goto done;                             /* :234 */
         if (C1 == ZERO) goto done;                              /* :240 */
-        double BETA = C1 / AMU / RQ;                            /* :241 */
+        double BETA = beta_prod ? C1 / (AMU * RQ) : C1 / AMU / RQ; /* :241 */
 #pragma omp parallel num_threads(nt)
         {
             int lo, hi;
@@ -299,13 +297,98 @@ int oracle_pcgpme(double *AL, int NL, int NA, int *LL, double *D, int N,
                 WK1[i] = B[i] + BETA * (WK1[i] - AMU * WK3[i]);
         }
     }
-    *IER = 3000;                                                /* :251 */
+    IER = 3000;                                                 /* :251 */
 done:
-    *ITER = K;                                                  /* :254 */
-    EPS[0] = ERR;
+    *K_out = K;
+    *err_out = ERR;
+    *ier_out = IER;
     free(WK);
     free(W);
     return 0;
+}
+
+/* PCGPME(AL,NL,NA,LL,D,N,N2,B,X,EPS,PARM,ITER,IER), pcgpme_para.F90:1-257.
+ * Same arguments (by value for scalars), same in-place mutation of AL, LL, D
+ * and B, same exits.  Returns 0, or -1 if NL is not 4 or 6 (reference: STOP)
+ * or N2 < 64 (the reference would index past X). */
+int oracle_pcgpme(double *AL, int NL, int NA, int *LL, double *D, int N,
+                  int N2, double *B, double *X, double *EPS, double PARM,
+                  int *ITER, int *IER) {
+    if (NL != 6 && NL != 4) return -1;
+    if (N2 < 64 || g_nthr > MAXT) return -1;
+    if (EPS[0] <= 1e-15) EPS[0] = 1e-9;                           /* :51 */
+    if (*ITER <= 0) *ITER = (int)(10.0 * (sqrt((double)N) + 1.0)); /* :52 */
+    oracle_scale_sort(AL, NL, NA, LL, D, N, B, PARM);
+    int K = 0, ier = 0;
+    double err = 0.0;
+    int rc = pcgpme_iterate(AL, NL, NA, LL, D, N, N2, B, X, EPS[0], *ITER, 0,
+                            &K, &err, &ier);
+    if (rc) return rc;
+    *IER = ier;
+    *ITER = K;                                                  /* :254 */
+    EPS[0] = err;
+    return 0;
+}
+
+/* The output pass of PCGPME_WILL, pcgpme_2DH.f90:255-268, serial over i in
+ * the caller's LL (not the sorted copy): zero |X(i)| <= 1e-14, then for each
+ * neighbour in slot order -- skip itself, stop at the first 0, skip an equal
+ * value, snap X(i) to a value within 1e-14 -- and store X(i) into B(i). */
+void oracle_will_filter(const int *LL, int NL, int NA, int N, double *X,
+                        double *B) {
+    const double L_Limit = 1e-14;                               /* :47 */
+    for (int i = 0; i < N; ++i) {
+        if (fabs(X[i]) <= L_Limit) X[i] = 0.0;
+        for (int k = 0; k < NL; ++k) {
+            int j = A2(LL, i, k, NA);
+            if (j == i + 1) continue;
+            if (j == 0) break;
+            if (X[i] == X[j - 1]) continue;
+            if (fabs(X[i] - X[j - 1]) <= L_Limit) X[i] = X[j - 1];
+            B[i] = X[i];
+        }
+    }
+}
+
+/* PCGPME_WILL(AL,NL,NA,LL,D,N,N2,B,EPS,PARM,ITER), pcgpme_2DH.f90:1-374:
+ * AL, LL, D untouched (scaled and sorted copies, :56-76), X from zero, EPS
+ * and ITER as given, BETA = C1/(AMU*RQ), then the output pass into B.
+ * Returns 0 / -1 (bad NL or N2) / -2 (memory).  Not in the reference's
+ * argument list, for the checks: *IER the local IER, *K_out the last K,
+ * X_raw / X_out (N entries each, may be NULL) X before and after the
+ * output pass. */
+int oracle_pcgpme_will(const double *AL, int NL, int NA, const int *LL,
+                       const double *D, int N, int N2, double *B, double EPS,
+                       double PARM, int ITER, int *IER, int *K_out,
+                       double *X_raw, double *X_out) {
+    if (NL != 6 && NL != 4) return -1;
+    if (N2 < 64 || g_nthr > MAXT || NA < N) return -1;
+    size_t nm = (size_t)NA * (size_t)NL;
+    double *ALT = (double *)malloc(nm * sizeof(double));
+    int *LLT = (int *)malloc(nm * sizeof(int));
+    double *DT = (double *)malloc((size_t)N * sizeof(double));
+    double *X = (double *)calloc((size_t)N + (size_t)N2, sizeof(double));
+    if (!ALT || !LLT || !DT || !X) {
+        free(ALT); free(LLT); free(DT); free(X);
+        return -2;
+    }
+    memcpy(ALT, AL, nm * sizeof(double));
+    memcpy(LLT, LL, nm * sizeof(int));
+    memcpy(DT, D, (size_t)N * sizeof(double));
+    oracle_scale_sort(ALT, NL, NA, LLT, DT, N, B, PARM);
+    int K = 0, ier = 0;
+    double err = 0.0;
+    int rc = pcgpme_iterate(ALT, NL, NA, LLT, DT, N, N2, B, X, EPS, ITER, 1,
+                            &K, &err, &ier);
+    if (!rc) {
+        if (X_raw) memcpy(X_raw, X, (size_t)N * sizeof(double));
+        oracle_will_filter(LL, NL, NA, N, X, B);
+        if (IER) *IER = ier;
+        if (K_out) *K_out = K;
+        if (X_out) memcpy(X_out, X, (size_t)N * sizeof(double));
+    }
+    free(ALT); free(LLT); free(DT); free(X);
+    return rc;
 }
 
 /* Fortran-mangled entry with the reference's by-reference argument list. */

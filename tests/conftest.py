@@ -110,10 +110,11 @@ def call_ref(s, iters=10000, eps=1e-10, nthr=1, parm=5.0, x0=None):
     it = ctypes.c_int(iters)
     ier = ctypes.c_int(-1)
     i32 = lambda v: ctypes.byref(ctypes.c_int(v))
-    lib.ref_set_threads(nthr)
-    run_big_stack(lambda: lib.pcgpme_(ptr(c["AL"]), i32(c["NL"]), i32(c["NA"]), ptr(c["LL"]), ptr(c["D"]),
+    # the OpenMP thread count is a per-thread setting: set it in the thread
+    # that calls the reference
+    run_big_stack(lambda: (lib.ref_set_threads(nthr), lib.pcgpme_(ptr(c["AL"]), i32(c["NL"]), i32(c["NA"]), ptr(c["LL"]), ptr(c["D"]),
                                       i32(N), i32(64), ptr(c["B"]), ptr(X), ptr(EPS),
-                                      ctypes.byref(ctypes.c_double(parm)), ctypes.byref(it), ctypes.byref(ier)),
+                                      ctypes.byref(ctypes.c_double(parm)), ctypes.byref(it), ctypes.byref(ier))),
                   120 * (N + 64) + 64 * c["NA"] + (64 << 20))
     c.update(X=X, EPS=EPS, ITER=it.value, IER=ier.value)
     return c
@@ -140,3 +141,35 @@ def apply_A(s, x):
         m = col > 0
         y[m] += s["AL"][:N, k][m] * x[col[m] - 1]
     return y
+
+
+def call_oracle_will(s, iters=1000, eps=1e-10, nthr=1, parm=5.0):
+    """oracle_pcgpme_will on copies of s; returns dict(B, X_raw, X, IER, K,
+    AL/LL/D as left after the call)."""
+    c = copy_system(s)
+    N = c["N"]
+    xr, xo = np.zeros(N), np.zeros(N)
+    ier, k = ctypes.c_int(-1), ctypes.c_int(-1)
+    o = oracle()
+    o.oracle_set_threads(nthr)
+    rc = o.oracle_pcgpme_will(ptr(c["AL"]), c["NL"], c["NA"], ptr(c["LL"]), ptr(c["D"]), N, 64, ptr(c["B"]),
+                              ctypes.c_double(eps), ctypes.c_double(parm), int(iters), ctypes.byref(ier),
+                              ctypes.byref(k), ptr(xr), ptr(xo))
+    c.update(X_raw=xr, X=xo, IER=ier.value, K=k.value, rc=rc)
+    return c
+
+
+def call_ref_will(s, iters=1000, eps=1e-10, nthr=1, parm=5.0):
+    """The flang build of PCGPME_WILL (src2LOWNS/pcgpme_2DH.f90) on copies
+    of s; returns dict(B, AL/LL/D as left after the call)."""
+    lib = ref_pcgpme_lib()
+    c = copy_system(s)
+    N = c["N"]
+    i32 = lambda v: ctypes.byref(ctypes.c_int(v))
+    f64 = lambda v: ctypes.byref(ctypes.c_double(v))
+    # PCGPME_WILL takes its chunking from omp_get_num_threads()
+    # (pcgpme_2DH.f90:55-57): set the count in the thread that calls it
+    run_big_stack(lambda: (lib.ref_set_threads(nthr), lib.pcgpme_will_(ptr(c["AL"]), i32(c["NL"]), i32(c["NA"]), ptr(c["LL"]), ptr(c["D"]),
+                                           i32(N), i32(64), ptr(c["B"]), f64(eps), f64(parm), i32(iters))),
+                  200 * (N + 64) + 64 * c["NA"] + (64 << 20))
+    return c

@@ -51,6 +51,8 @@ def test_fortran_shim_defines_pcgpme():
     out = subprocess.run(["nm", obj], capture_output=True, text=True).stdout
     assert re.search(r"\bT pcgpme_\b", out), out  # PCGPME, default Fortran mangling
     assert re.search(r"\bU clowns_pcgpme\b", out), out
+    assert re.search(r"\bT pcgpme_will_\b", out), out  # PCGPME_WILL
+    assert re.search(r"\bU clowns_pcgpme_will\b", out), out
 
 
 def test_python_mirror_rejects_bad_layouts():

@@ -20,7 +20,7 @@ import pytest
 from conftest import (ROOT, call_oracle, call_ref, copy_system, dense_matrix, oracle, ptr)
 from _systems import box_system
 
-GOLDEN = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "pcgpme_*.npz")))
+GOLDEN = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "pcgpme_box*.npz")))
 
 
 def _from_golden(z):
