@@ -821,8 +821,9 @@ k_grad_coef(int N, int lda, int lda_out, int NL, const int *__restrict__ LL,
 // Row i reads the already filtered X of neighbours j < i and the raw X of
 // neighbours j > i.  On the device the filtered vector is the fixed point of
 // Y <- F(Y), F(Y)_i = row i's pass with neighbours j < i read from Y: the
-// dependence is on lower rows only, so the fixed point is unique, equals the
-// serial result, and is reached after (longest chain of snaps) + 1 rounds.
+// dependence is on lower rows only, so the fixed point is unique and equals
+// the serial result; the rounds below reach it after at most (longest chain
+// of snaps) + 1 rounds, each touching only the rows whose inputs changed.
 constexpr double WILL_LIMIT = 1e-14;
 
 __device__ __forceinline__ double will_row(int i, int lda, int NL, const int *__restrict__ LL,
@@ -849,22 +850,111 @@ __device__ __forceinline__ double will_row(int i, int lda, int NL, const int *__
     return xi;
 }
 
-// One round: Yout = F(Yin).  Round 0 reads Yin = X with the zero snap applied
-// (the value a lower row has when it snaps to no neighbour).  flags[r] = 1 if
-// any row changed; a round whose predecessor changed nothing returns (both
-// buffers already hold the fixed point).
+// Start of the rounds: Y = X with the zero limit applied (the value of a row
+// that snaps to no neighbour).
 __global__ void __launch_bounds__(BS)
-k_will_round(int N, int lda, int NL, const int *__restrict__ LL, const double *__restrict__ X,
-             const double *__restrict__ Yin, double *__restrict__ Yout, int *__restrict__ flags, int r) {
-    if (r > 0 && flags[r - 1] == 0) return;
+k_will_init(int N, const double *__restrict__ X, double *__restrict__ Y) {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i >= N) return;
-    bool as;
-    const double y = will_row(i, lda, NL, LL, X, Yin, r == 0, as);
-    double prev = Yin[i];
-    if (r == 0 && fabs(prev) <= WILL_LIMIT) prev = 0.0;
-    Yout[i] = y;
-    if (__double_as_longlong(y) != __double_as_longlong(prev)) flags[r] = 1;
+    const double x = X[i];
+    Y[i] = fabs(x) <= WILL_LIMIT ? 0.0 : x;
+}
+
+// Readers of each row as CSR (off[0..N], readers[off[j] .. off[j+1])): the
+// rows m > j whose pass reads row j (j+1 in a slot before m's first zero).
+__global__ void __launch_bounds__(BS)
+k_will_rdeg(int N, int lda, int NL, const int *__restrict__ LL, int *__restrict__ deg) {
+    const int m = blockIdx.x * BS + threadIdx.x;
+    if (m >= N) return;
+    for (int k = 0; k < NL; ++k) {
+        const int j = LL[(size_t)k * lda + m];
+        if (j == m + 1) continue;
+        if (j == 0) break;
+        if (j - 1 < m) atomicAdd(&deg[j - 1], 1);
+    }
+}
+
+// Per-block sums of deg over blocks of WILL_SCAN rows (then scanned by
+// k_cls_scan_blocks).
+constexpr int WILL_SCAN = 2048;
+__global__ void __launch_bounds__(BS)
+k_will_rsum(int N, const int *__restrict__ deg, int *__restrict__ bsum) {
+    const int base = blockIdx.x * WILL_SCAN;
+    int n = 0;
+    for (int u = threadIdx.x; u < WILL_SCAN; u += BS)
+        if (base + u < N) n += deg[base + u];
+    __shared__ int lds[BS];
+    lds[threadIdx.x] = n;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = lds[0];
+}
+
+// off = exclusive scan of deg: block start from the scanned block sums, then
+// a scan of each 256-row chunk in LDS.  off[N] = the total.
+__global__ void __launch_bounds__(BS)
+k_will_roff(int N, const int *__restrict__ deg, const int *__restrict__ bstart, int *__restrict__ off) {
+    const int base = blockIdx.x * WILL_SCAN;
+    __shared__ int lds[BS];
+    int run = bstart[blockIdx.x];
+    for (int u = 0; u < WILL_SCAN; u += BS) {
+        const int i = base + u + threadIdx.x;
+        const int d = i < N ? deg[i] : 0;
+        lds[threadIdx.x] = d;
+        __syncthreads();
+        for (int o = 1; o < BS; o <<= 1) {
+            const int v = threadIdx.x >= o ? lds[threadIdx.x - o] : 0;
+            __syncthreads();
+            lds[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (i < N) off[i] = run + lds[threadIdx.x] - d;
+        run += lds[BS - 1];
+        __syncthreads();
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[N] = bstart[gridDim.x];
+}
+
+__global__ void __launch_bounds__(BS)
+k_will_rfill(int N, int lda, int NL, const int *__restrict__ LL, const int *__restrict__ off,
+             int *__restrict__ fillc, int *__restrict__ readers) {
+    const int m = blockIdx.x * BS + threadIdx.x;
+    if (m >= N) return;
+    for (int k = 0; k < NL; ++k) {
+        const int j = LL[(size_t)k * lda + m];
+        if (j == m + 1) continue;
+        if (j == 0) break;
+        if (j - 1 < m) readers[off[j - 1] + atomicAdd(&fillc[j - 1], 1)] = m;
+    }
+}
+
+// One round, in place over a candidate list: round 0 evaluates every row,
+// round r > 0 the rows pushed in round r-1.  A row whose value changes
+// stores it and pushes its readers (deduplicated by `stamp`) onto the next
+// list.  Rows read the current Y of their lower neighbours, this round's or
+// the last one's: every row is evaluated again after the last change of each
+// row it reads, so when a round pushes nothing, Y = F(Y): the unique fixed
+// point.  cnt[r] = length of round r's list.
+__global__ void __launch_bounds__(BS)
+k_will_front(int N, int lda, int NL, const int *__restrict__ LL, const double *__restrict__ X,
+             double *Y, const int *__restrict__ off, const int *__restrict__ readers,
+             const int *__restrict__ cand_in, int *__restrict__ cand_out, int *__restrict__ stamp,
+             int *__restrict__ cnt, int r) {
+    const int n = r == 0 ? N : cnt[r];
+    for (int t = blockIdx.x * BS + threadIdx.x; t < n; t += gridDim.x * BS) {
+        const int i = r == 0 ? t : cand_in[t];
+        bool as;
+        const double y = will_row(i, lda, NL, LL, X, Y, false, as);
+        if (__double_as_longlong(y) == __double_as_longlong(Y[i])) continue;
+        Y[i] = y;
+        for (int e = off[i]; e < off[i + 1]; ++e) {
+            const int m = readers[e];
+            if (atomicMax(&stamp[m], r + 1) < r + 1) cand_out[atomicAdd(&cnt[r + 1], 1)] = m;
+        }
+    }
 }
 
 // Outputs from the fixed point Y: X' (optional) and B.

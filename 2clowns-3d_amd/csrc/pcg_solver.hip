@@ -109,7 +109,10 @@ struct clowns_solver {
     DevBuf<double> stage_b, stage_x;  // clowns_pcgpme host staging
     DevBuf<double> part;   // per-block partial sums
     DevBuf<int> flag;      // device error flag
-    DevBuf<int> will_flags;  // PCGPME_WILL output pass: per-round change flags
+    DevBuf<int> will_flags;  // PCGPME_WILL output pass: per-round change flags / counts
+    DevBuf<int> will_cand;   // PCGPME_WILL output pass: candidate lists (2N) + stamps (N)
+    DevBuf<int> will_rev;    // PCGPME_WILL output pass: readers of each row (CSR)
+    bool will_rev_valid = false;  // will_rev holds the readers of the resident LL0
     DevBuf<int> cls_scratch;  // classification: extents, change flag, block counts
     DevBuf<double> pc_b, pc_x, pc_part;  // pressure-correction loop: RHS, xe, DMAX partials
     DevBuf<Scal> sc;       // [0] inner/FP64 run, [1] outer (mixed)
@@ -409,6 +412,7 @@ int load_matrix(clowns_solver *h, const double *AL, int NL, int lda, const int *
     h->have32 = false;
     h->have_t = false;
     h->raw_valid = keep_raw;
+    h->will_rev_valid = false;
     return CLOWNS_OK;
 }
 
@@ -646,7 +650,7 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     h->AL32.release(); h->D32.release();
     h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
     h->AL0.release(); h->D0.release(); h->LL0.release();
-    h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->will_flags.release(); h->cls_scratch.release(); h->pc_b.release(); h->pc_x.release(); h->pc_part.release(); h->sc.release();
+    h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->will_flags.release(); h->will_cand.release(); h->will_rev.release(); h->cls_scratch.release(); h->pc_b.release(); h->pc_x.release(); h->pc_part.release(); h->sc.release();
     if (h->h_sc) (void)hipHostFree(h->h_sc);
     if (h->ev0) { (void)hipEventDestroy(h->ev0); (void)hipEventDestroy(h->ev1); (void)hipEventDestroy(h->ev2); }
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -819,36 +823,56 @@ void will_host(int N, int lda, int NL, const std::vector<int> &LL, std::vector<d
 
 // X' and B of the output pass from the solution X and the final residual R
 // (B may be R).  LL: caller order, 1-based, column stride lda, entries in
-// [0, N] (checked by the caller).  *rounds: device rounds, or -1 when the
-// host pass ran.
+// [0, N] (checked by the caller).  reuse_rev: will_rev already holds this
+// LL's readers.  *rounds: device rounds, or -1 when the host pass ran.
 int will_filter(clowns_solver *h, const int *d_LL, int NL, int lda, int N, const double *d_X,
-                const double *d_R, double *d_Xout, double *d_B, int *rounds) {
+                const double *d_R, double *d_Xout, double *d_B, int *rounds, bool reuse_rev) {
     hipStream_t s = h->stream;
     int rc;
-    if ((rc = ensure_vecs(h, h->v64, N)) || (rc = h->will_flags.ensure(WILL_MAX_ROUNDS))) return rc;
-    double *buf[2] = {h->v64.y.p, h->v64.t.p};
+    const int nb = cdiv(N, WILL_SCAN);
+    const size_t nrev = (size_t)2 * N + 1 + nb + 1 + (size_t)N * NL;
+    if ((rc = ensure_vecs(h, h->v64, N)) || (rc = h->will_flags.ensure(WILL_MAX_ROUNDS + 1)) ||
+        (rc = h->will_cand.ensure((size_t)3 * N)) || (rc = h->will_rev.ensure(nrev)))
+        return rc;
+    int *off = h->will_rev.p, *deg = off + N + 1, *bsum = deg + N, *readers = bsum + nb + 1;
+    if (!reuse_rev) {
+        fill<int>(s, deg, (size_t)N, 0);
+        k_will_rdeg<<<cdiv(N, BS), BS, 0, s>>>(N, lda, NL, d_LL, deg);
+        k_will_rsum<<<nb, BS, 0, s>>>(N, deg, bsum);
+        k_cls_scan_blocks<<<1, 1024, 0, s>>>(nb, bsum);
+        k_will_roff<<<nb, BS, 0, s>>>(N, deg, bsum, off);
+        fill<int>(s, deg, (size_t)N, 0);
+        k_will_rfill<<<cdiv(N, BS), BS, 0, s>>>(N, lda, NL, d_LL, off, deg, readers);
+        HIPCHK(hipGetLastError());
+    }
+    double *Y = h->v64.y.p;
+    int *cand[2] = {h->will_cand.p, h->will_cand.p + N}, *stamp = h->will_cand.p + 2 * (size_t)N;
     const int cap = std::max(1, std::min(WILL_MAX_ROUNDS, h->will_max_rounds));
-    fill<int>(s, h->will_flags.p, WILL_MAX_ROUNDS, 0);
-    const int G = cdiv(N, BS);
-    int launched = 0, batch = 4, r0 = -1;
+    fill<int>(s, h->will_flags.p, WILL_MAX_ROUNDS + 1, 0);
+    fill<int>(s, stamp, (size_t)N, 0);
+    k_will_init<<<cdiv(N, BS), BS, 0, s>>>(N, d_X, Y);
+    const int Gf = std::min(cdiv(N, BS), 1024);
+    int launched = 0, batch = 2, r0 = -1;
     int hf[256];
     while (launched < cap) {
-        const int nb = std::min(batch, cap - launched);
-        for (int r = launched; r < launched + nb; ++r)
-            k_will_round<<<G, BS, 0, s>>>(N, lda, NL, d_LL, d_X, r == 0 ? d_X : buf[(r - 1) & 1],
-                                          buf[r & 1], h->will_flags.p, r);
+        const int nr = std::min(batch, cap - launched);
+        for (int r = launched; r < launched + nr; ++r)
+            k_will_front<<<r == 0 ? cdiv(N, BS) : Gf, BS, 0, s>>>(N, lda, NL, d_LL, d_X, Y, off, readers,
+                                                                cand[r & 1], cand[(r + 1) & 1], stamp,
+                                                                h->will_flags.p, r);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(hf, h->will_flags.p + launched, (size_t)nb * sizeof(int),
+        // will_flags[r + 1]: rows pushed by round r; none = fixed point
+        HIPCHK(hipMemcpyAsync(hf, h->will_flags.p + launched + 1, (size_t)nr * sizeof(int),
                               hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        for (int r = 0; r < nb && r0 < 0; ++r)
+        for (int r = 0; r < nr && r0 < 0; ++r)
             if (hf[r] == 0) r0 = launched + r;
-        launched += nb;
+        launched += nr;
         if (r0 >= 0) break;
         batch = std::min(256, batch * 2);
     }
     if (r0 >= 0) {
-        k_will_out<<<G, BS, 0, s>>>(N, lda, NL, d_LL, d_X, buf[r0 & 1], d_R, d_Xout, d_B);
+        k_will_out<<<cdiv(N, BS), BS, 0, s>>>(N, lda, NL, d_LL, d_X, Y, d_R, d_Xout, d_B);
         HIPCHK(hipGetLastError());
         if (rounds) *rounds = r0 + 1;
         return CLOWNS_OK;
@@ -885,8 +909,9 @@ int solve_will(clowns_solver *h, const double *d_b, double *d_out, double *d_xou
     // (pcgpme_2DH.f90:248-249); a zero right-hand side returns at once.
     if (iter <= 0 && !(h->h_sc[0].done && h->h_sc[0].BN == 0.0)) inf.ier = CLOWNS_NOCONV;
     if ((rc = will_filter(h, h->LL0.p, h->NL, h->lda, N, h->stage_x.p, h->v64.r.p, d_xout, d_out,
-                          rounds)))
+                          rounds, h->will_rev_valid)))
         return rc;
+    h->will_rev_valid = true;
     HIPCHK(hipStreamSynchronize(h->stream));
     if (info) *info = inf;
     return CLOWNS_OK;
@@ -963,7 +988,8 @@ extern "C" int clowns_will_filter(clowns_solver *h, const int32_t *d_LL, int32_t
     HIPCHK(hipMemcpyAsync(&hbad, h->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (hbad) return CLOWNS_EBADARG;
-    if ((rc = will_filter(h, d_LL, NL, lda, N, d_X, d_R, d_Xout, d_B, rounds))) return rc;
+    if ((rc = will_filter(h, d_LL, NL, lda, N, d_X, d_R, d_Xout, d_B, rounds, false))) return rc;
+    h->will_rev_valid = false;  // will_rev now holds the caller's LL
     HIPCHK(hipStreamSynchronize(s));
     return CLOWNS_OK;
 }
@@ -1139,6 +1165,7 @@ extern "C" int clowns_solver_assemble(clowns_solver *h, const clowns_grid_dev *g
     h->have32 = false;
     h->have_t = false;
     h->raw_valid = keep_raw != 0;
+    h->will_rev_valid = false;
     HIPCHK(hipStreamSynchronize(s));
     return CLOWNS_OK;
 }

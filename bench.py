@@ -13,6 +13,15 @@ The same steps are then run in mixed precision (FP32 inner solve, FP64
 refinement), reported under "mixed" with the error against the FP64 solution.
 
 Prints one JSON line.  `value` is seconds per step (lower is better).
+
+`--will NXxNY` measures the second call site of the same solver instead: the
+2DH dispersion solve PCGPME_WILL (src2LOWNS/pcgpme_2DH.f90, called once per
+time step by disp_adjust, 1sub-2DH-momentum.f90:325-326) on the 2DH
+linear-dispersion matrix of an NX x NY grid (deep water, 400-500 m cells).
+The matrix is constant in time (disp_param builds it once), so it stays
+resident; a step is one solve_will on a new right-hand side already in HBM
+(solve + the reference's output pass).  The drop-in (host arrays in and out,
+matrix upload included) is timed beside it.
 """
 import argparse
 import ctypes
@@ -52,7 +61,143 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only assemble and run the timed kernel loops (for rocprofv3 --pmc)")
+    ap.add_argument("--will", default=None, metavar="NXxNY",
+                    help="measure the 2DH dispersion solve PCGPME_WILL on an NX x NY grid instead")
     return ap.parse_args()
+
+
+# Algorithmic bytes per row of one PCGPME_WILL iteration (NL = 4): 2 SpMV
+# (AL 32 + D 8 + x 8 + y 8 = 56, the e-fused one 72), 8 single-term PM stages
+# (24 each) plus the two dot vectors read by the last stage of each (16),
+# p update (32), x/r update with the dots (56).
+WILL_ITER_BYTES = 56 + 72 + 8 * 24 + 16 + 32 + 56
+
+
+def will_cpu_baseline(s, threads, eps, iters):
+    """The reference's own PCGPME_WILL (flang build in oracle/_ref) on the
+    same system at `threads` OpenMP threads; the C oracle if that was not built."""
+    import numpy as np
+    N, NA = s["N"], s["NA"]
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref_pcgpme.so")
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    AL, LL, D = s["AL"], s["LL"], s["D"]
+    times = []
+    for _ in range(2):
+        B = s["B"].copy()
+        if os.path.exists(ref):
+            lib = ctypes.CDLL(ref)
+            kind = "reference"
+            i32 = lambda v: ctypes.byref(ctypes.c_int(v))
+            f64 = lambda v: ctypes.byref(ctypes.c_double(v))
+
+            def call():
+                lib.ref_set_threads(threads)
+                lib.pcgpme_will_(P(AL), i32(4), i32(NA), P(LL), P(D), i32(N), i32(64), P(B), f64(eps), f64(5.0),
+                                 i32(iters))
+        else:
+            orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+            kind = "port"
+            ier, k = ctypes.c_int(), ctypes.c_int()
+
+            def call():
+                orc.oracle_set_threads(threads)
+                orc.oracle_pcgpme_will(P(AL), 4, NA, P(LL), P(D), N, 64, P(B), ctypes.c_double(eps),
+                                       ctypes.c_double(5.0), iters, ctypes.byref(ier), ctypes.byref(k), None, None)
+        out = {}
+
+        def body():
+            t0 = time.perf_counter()
+            call()
+            out["t"] = time.perf_counter() - t0
+
+        old = threading.stack_size()
+        threading.stack_size(max(32 << 20, 200 * (N + 64) + 64 * NA + (64 << 20)))
+        th = threading.Thread(target=body)
+        th.start()
+        th.join()
+        threading.stack_size(old)
+        times.append(out["t"])
+    return {"value": min(times), "unit": "s/solve", "cores": threads, "kind": kind,
+            "sample": (f"the same 2DH system (N={N}) on the host; "
+                       f"{'flang build of pcgpme_2DH.f90' if kind == 'reference' else 'C oracle of PCGPME_WILL'} "
+                       f"run to convergence at {threads} OpenMP threads, best of 2")}
+
+
+def bench_will(args):
+    import numpy as np
+    import torch
+    import clowns3d
+    from clowns3d import Solver, configs
+    nx, ny = (int(v) for v in args.will.lower().split("x"))
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    s = configs.dispersion_2dh(nx, ny, depth=4000.0, dx=(400.0, 500.0))
+    N = s["N"]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sol = Solver(0)
+    sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
+    if args.profile_kernels:
+        res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
+               (("k_spmv_v2<double,4,0>", 0), ("k_pm_v2<double,1,0>", 1), ("k_copy<double>", 4))}
+        print(json.dumps({"rows": N, "avg_us": res}))
+        return
+    rng = np.random.default_rng(7)
+    # a new right-hand side per step: the base PSI scaled, as the surface evolves
+    rhs = [t(s["B"] * rng.uniform(0.5, 1.5)) for _ in range(args.steps + args.warmup)]
+    out = torch.empty(N, dtype=torch.float64, device=dev)
+    for k in range(args.warmup):
+        sol.solve_will(rhs[k], out, eps=args.eps, iters=1000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = [sol.solve_will(rhs[args.warmup + k], out, eps=args.eps, iters=1000)[1] for k in range(args.steps)]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    assert all(i["ier"] == 0 for i in infos), infos
+    its = infos[-1]["iters"]
+    # drop-in: host arrays, matrix upload, copy back
+    c = {k: (v.copy(order="F") if isinstance(v, np.ndarray) else v) for k, v in s.items()}
+    t1 = time.perf_counter()
+    ier = clowns3d.pcgpme_will(c["AL"], 4, c["NA"], c["LL"], c["D"], N, 64, c["B"], args.eps, 5.0, 1000)
+    t_drop = time.perf_counter() - t1
+    # dominant kernel, HIP events on the solver's stream
+    t_spmv = sol.time_kernel(0, 50)
+    t_pm = sol.time_kernel(1, 50)
+    t_copy = sol.time_kernel(4, 50)
+    if 2 * t_spmv >= 8 * t_pm:
+        kname, kms, kbytes = "k_spmv_v2<double,4,0>", t_spmv, 56 * N
+    else:
+        kname, kms, kbytes = "k_pm_v2<double,1,0>", t_pm, ALG_BYTES["pm_stage"] * N
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"traffic_will{nx}x{ny}.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get(kname)
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = will_cpu_baseline(s, threads, args.eps, 1000)
+    solve_gbs = N * its * WILL_ITER_BYTES / (infos[-1]["ms"] * 1e-3) / 1e9
+    res = {
+        "metric": "2DH dispersion solve (PCGPME_WILL) s/solve to 1e-10 rel. residual; HBM GB/s",
+        "value": dt, "unit": "s/solve", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"2DH linear-dispersion matrix {nx}x{ny} (depth 4000 m, 400-500 m cells, "
+                               f"Gaussian surface hump), resident operator, PCGPME_WILL to {args.eps:g} + output pass",
+                   "rows": N, "iterations": its, "filter_rounds": infos[-1]["filter_rounds"],
+                   "solve_ms_hip_events": infos[-1]["ms"], "solve_alg_GBps": solve_gbs,
+                   "solve_alg_frac": solve_gbs / PEAK_HBM_GBS,
+                   "drop_in_ms_host_arrays": t_drop * 1e3, "drop_in_ier": ier},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "kernel": kname,
+                     "alg_bytes_per_launch": kbytes, "avg_us": kms * 1e3,
+                     "copy_GBps": 16 * N / (t_copy * 1e-3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        res["speedup_vs_cpu"] = cpu["value"] / dt
+    print(json.dumps(res))
 
 
 def cpu_baseline(g, iters_full, n_iter_hi, threads):
@@ -137,6 +282,9 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU z-slab design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.will:
+        bench_will(args)
+        return
     import numpy as np
     import torch
     import clowns3d

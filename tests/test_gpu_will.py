@@ -45,21 +45,22 @@ def serial_filter(LL, X, R):
     return Xs, Bs
 
 
-def tiny_field(nx, ny, seed):
-    """A 2D neighbour list with dropped links, self slots and early zeros,
-    and an X of a few values 0.35e-14 apart (snaps, equalities, chains)."""
+def tiny_field(nx, ny, seed, drop=0.1):
+    """A 2D neighbour list with self slots, early zeros and (drop > 0)
+    dropped links, and an X of a few values 0.35e-14 apart (snaps,
+    equalities, chains)."""
     rng = np.random.default_rng(seed)
     N = nx * ny
-    LL = np.zeros((N, 4), np.int32)
+    LL = np.zeros((N, 6), np.int32)
     for i in range(N):
         y, x = divmod(i, nx)
         c = [(y + dy) * nx + (x + dx) + 1 for dy, dx in ((0, -1), (0, 1), (-1, 0), (1, 0))
-             if 0 <= x + dx < nx and 0 <= y + dy < ny and rng.random() < 0.9]
+             if 0 <= x + dx < nx and 0 <= y + dy < ny and rng.random() >= drop]
         if rng.random() < 0.05:
             c.insert(int(rng.integers(0, len(c) + 1)), i + 1)
         if c and rng.random() < 0.05:
             c.insert(int(rng.integers(0, len(c))), 0)
-        LL[i, :min(4, len(c))] = c[:4]
+        LL[i, :len(c)] = c
     X = 1e-13 * rng.integers(-2, 3, N) + 0.35e-14 * rng.integers(0, 6, N)
     R = rng.standard_normal(N)
     return LL, X, R
@@ -73,17 +74,18 @@ def _filter_gpu(sol, LL, X, R):
     return xo.cpu().numpy(), bo.cpu().numpy(), rounds
 
 
-@pytest.mark.parametrize("case", ["dispersion", "tiny40x30", "tiny97x61"])
+@pytest.mark.parametrize("case", ["dispersion", "tiny40x30", "tiny97x61", "sym80x50", "dispersion_deep"])
 def test_output_pass_bit_exact(case):
     from clowns3d import Solver, configs
-    if case == "dispersion":
-        s = configs.dispersion_2dh(96, 64, seed=43)
+    if case.startswith("dispersion"):
+        kw = dict(depth=4000.0, dx=(400.0, 500.0)) if case.endswith("deep") else {}
+        s = configs.dispersion_2dh(160, 128, seed=43, **kw)
         o = call_oracle_will(s, iters=1000)
         LL, X = np.ascontiguousarray(s["LL"]), o["X_raw"]
         R = np.random.default_rng(1).standard_normal(s["N"])
     else:
-        nx, ny = map(int, case[4:].split("x"))
-        LL, X, R = tiny_field(nx, ny, seed=nx)
+        nx, ny = map(int, case[4:].split("x") if case.startswith("tiny") else case[3:].split("x"))
+        LL, X, R = tiny_field(nx, ny, seed=nx, drop=0.1 if case.startswith("tiny") else 0.0)
     Xs, Bs = serial_filter(LL, X, R)
     sol = Solver(0)
     Xg, Bg, rounds = _filter_gpu(sol, LL, X, R)
