@@ -125,6 +125,8 @@ struct clowns_solver {
     bool cls_force_host = false;  // test hook: always take the exact host path
     int poll = 8;          // iterations between host polls of `done`
     int will_max_rounds = 4096;  // PCGPME_WILL output pass: device rounds before the host pass
+    int fuse64 = -1, fuse32 = -1;      // pairwise PM fusion chosen by tune_fuse ...
+    int fuse64_N = 0, fuse32_N = 0;    // ... for operators of this many rows
 };
 
 namespace {
@@ -168,13 +170,17 @@ template <typename T> struct Op {
     const T *AL;
     const int *LL;
     const T *D;
+    bool fuse = false;  // PM stages fused pairwise (k_pm2) in apply_pm
 };
 
-// Pairwise fusion of single-term PM stages (k_pm2).  Measured on MI355X at
-// 4.2M rows: FP32 inner solve 8% faster per iteration, FP64 1.5% slower (the
-// three dependent loads of the recomputed gather cost more than the 8-byte
-// stream they save), so it is on for FP32 only.  CLOWNS_FUSE_PM=0/1 forces
-// both precisions off/on.
+// Pairwise fusion of single-term PM stages (k_pm2) saves one pass over the
+// intermediate vector but recomputes the first stage at every gather point
+// (three dependent loads).  Which form is faster depends on the operator:
+// measured on MI355X, fused is 9% (FP64) and 12% (FP32) faster per solve at
+// 2.45M rows, but at 13.1M rows (512x512 planes) the fused FP32 pair takes
+// 83 us against 2 x 31 us.  So each solver times both forms on its operator
+// the first time it solves at a given size (tune_fuse) and keeps the faster.
+// CLOWNS_FUSE_PM=0/1 forces both precisions off/on.
 int h_fuse_pm = -1;
 // Two rows per thread in the PM stages (k_pm_v2, k_pm2_v2): 8/16-byte
 // accesses, twice the bytes in flight per memory instruction.  Measured on
@@ -246,7 +252,7 @@ int launch_pm_stage(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j,
 template <typename T>
 int apply_pm(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, const T *in,
              T *out, T *y, T *t, int dot, const T *w, double *part) {
-    if (h_fuse_pm == 1 || (h_fuse_pm < 0 && sizeof(T) == 4)) {
+    if (A.fuse) {
         // stages (1,2) and (3,4) fused pairwise; 5/6 (two-term) carries the dots
         launch_pm2<T>(s, g, sc, A, 0, in, t);
         if (A.NL == 6) {
@@ -268,6 +274,38 @@ int apply_pm(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, const T *in,
     launch_pm_stage<T, 1>(s, g, sc, A, 1, t, y, 0, w, part);
     launch_pm_stage<T, 1>(s, g, sc, A, 2, y, t, 0, w, part);
     return launch_pm_stage<T, 1>(s, g, sc, A, 3, t, out, dot, w, part);
+}
+
+// Time the fused pair against two single stages on the operator (one
+// warm-up and three launches each, HIP events on the solver's stream) and
+// keep the faster form for operators of this size.  V's y, t, e are scratch
+// here; the run's scalars are reset (kernels skip work once `done` is set).
+template <typename T> bool tune_fuse(clowns_solver *h, const Op<T> &A, VecSet<T> &V) {
+    if (h_fuse_pm >= 0) return h_fuse_pm == 1;
+    int &dec = sizeof(T) == 4 ? h->fuse32 : h->fuse64;
+    int &decN = sizeof(T) == 4 ? h->fuse32_N : h->fuse64_N;
+    if (dec >= 0 && decN == A.N) return dec == 1;
+    hipStream_t s = h->stream;
+    Scal *sc = h->sc.p;
+    const Geo g = geo_of(A.N, h->R);
+    k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1, 0);
+    auto fused = [&]() { launch_pm2<T>(s, g, sc, A, 0, V.y.p, V.t.p); };
+    auto single = [&]() {
+        launch_pm_stage<T, 1>(s, g, sc, A, 0, V.y.p, V.t.p, 0, nullptr, h->part.p);
+        launch_pm_stage<T, 1>(s, g, sc, A, 1, V.t.p, V.e.p, 0, nullptr, h->part.p);
+    };
+    float ms[2] = {0.f, 0.f};
+    for (int form = 0; form < 2; ++form) {
+        form == 0 ? fused() : single();
+        if (hipEventRecord(h->ev0, s) != hipSuccess) return false;
+        for (int k = 0; k < 3; ++k) form == 0 ? fused() : single();
+        if (hipEventRecord(h->ev1, s) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess ||
+            hipEventElapsedTime(&ms[form], h->ev0, h->ev1) != hipSuccess)
+            return false;
+    }
+    dec = ms[0] < ms[1] ? 1 : 0;
+    decN = A.N;
+    return dec == 1;
 }
 
 template <typename T, int NL>
@@ -444,13 +482,15 @@ template <typename T> int ensure_vecs(clowns_solver *h, VecSet<T> &V, int N) {
 // FP64 PCGPME on the resident operator `A` with row scales `Wsc`: exactly the
 // reference's sequence (setup :115-145, iteration :148-247).
 // beta_prod selects PCGPME_WILL's rounding of BETA (pcgpme_2DH.f90:241).
-int solve_fp64(clowns_solver *h, const Op<double> &A, const double *Wsc, const double *d_b,
+int solve_fp64(clowns_solver *h, const Op<double> &A_in, const double *Wsc, const double *d_b,
                double *d_x, double eps, int itermax, bool warm, clowns_info *info,
                bool beta_prod = false) {
     hipStream_t s = h->stream;
     int rc;
     VecSet<double> &V = h->v64;
-    if ((rc = ensure_vecs(h, V, A.N)) || (rc = h->bs.ensure(n_pad(A.N)))) return rc;
+    if ((rc = ensure_vecs(h, V, A_in.N)) || (rc = h->bs.ensure(n_pad(A_in.N)))) return rc;
+    Op<double> A = A_in;
+    A.fuse = tune_fuse(h, A, V);
     const int N = A.N;
     const Geo g = geo_of(N, h->R);
     Scal *sc = h->sc.p;
@@ -496,16 +536,20 @@ int solve_fp64(clowns_solver *h, const Op<double> &A, const double *Wsc, const d
 // x += d with the true preconditioned residual r = M (W b - W A x) recomputed
 // in FP64 after each inner solve; the outer stopping value is the
 // reference's sqrt(RN/BN) evaluated on that true residual.
-int solve_mixed(clowns_solver *h, const Op<double> &A, const Op<float> &A32, const double *Wsc,
+int solve_mixed(clowns_solver *h, const Op<double> &A_in, const Op<float> &A32_in, const double *Wsc,
                 const double *d_b, double *d_x, double eps, int itermax, bool warm,
                 clowns_info *info) {
     hipStream_t s = h->stream;
     int rc;
     VecSet<double> &V = h->v64;
     VecSet<float> &F = h->v32;
-    if ((rc = ensure_vecs(h, V, A.N)) || (rc = ensure_vecs(h, F, A.N)) ||
-        (rc = h->bs.ensure(n_pad(A.N))))
+    if ((rc = ensure_vecs(h, V, A_in.N)) || (rc = ensure_vecs(h, F, A_in.N)) ||
+        (rc = h->bs.ensure(n_pad(A_in.N))))
         return rc;
+    Op<double> A = A_in;
+    Op<float> A32 = A32_in;
+    A.fuse = tune_fuse(h, A, V);
+    A32.fuse = tune_fuse(h, A32, F);
     const int N = A.N;
     const Geo g = geo_of(N, h->R);
     Scal *sc = h->sc.p, *outer = h->sc.p + 1;

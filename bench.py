@@ -302,7 +302,8 @@ def main():
     if args.profile_kernels:
         dg.assemble_into(sol, b=b)
         res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
-               (("k_spmv_v2<double,6,0>", 0), ("k_pm_v2<double,1,0>", 1), ("k_copy<double>", 4))}
+               (("k_spmv_v2<double,6,0>", 0), ("k_pm_v2<double,1,0>", 1), ("k_copy<double>", 4),
+                ("k_spmv_v2<float,6,0>", 2), ("k_pm_v2<float,1,0>", 3), ("k_pm2_v2<float>", 6))}
         print(json.dumps({"rows": N, "avg_us": res}))
         return
 
