@@ -73,6 +73,7 @@ SIGNATURES = {
     "clowns_pcgpme_will": (None, [_P] * 12),
     "clowns_solver_solve_will": (ctypes.c_int, [_P, _P, _P, _D, _I, ctypes.POINTER(Info), ctypes.POINTER(_I)]),
     "clowns_will_filter": (ctypes.c_int, [_P, _P, _I, _I, _I, _P, _P, _P, _P, ctypes.POINTER(_I)]),
+    "clowns_solver_index_bits": (ctypes.c_int, [_P]),
     "clowns_version": (ctypes.c_char_p, []),
 }
 

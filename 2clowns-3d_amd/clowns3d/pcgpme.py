@@ -129,6 +129,11 @@ class Solver:
                                             ctypes.byref(info)), "clowns_solver_solve")
         return x, info.as_dict()
 
+    def index_bits(self):
+        """16 if the resident operator's kernels read 16-bit coded neighbour
+        indices, else 32."""
+        return L.lib().clowns_solver_index_bits(self._h)
+
     def solve_will(self, b, out=None, eps=1e-10, iters=1000):
         """PCGPME_WILL on the resident operator (set_matrix): zero start, EPS
         and ITER as given, then the reference's output pass.  Returns

@@ -328,11 +328,119 @@ __device__ __forceinline__ typename V2<T>::t ld2(const T *p) {
     return *reinterpret_cast<const typename V2<T>::t *>(p);
 }
 
-template <typename T, int NT, int DOT>
+// Neighbour index sources for the two-row kernels, 0-based, ghosts (the
+// reference's N+1+mod(I,64), pcgpme_para.F90:80) at the zero rows >= N.
+// Ix32: the sorted LL itself.  Ix16: the same indices coded in 16 bits per
+// slot (k_ll16_build): an offset from the row when |offset| < 16384, else an
+// offset from the row block's nearest far offset of that sign (z
+// neighbours, up to a plane away), or the ghost code.  Half the index bytes
+// of every SpMV and PM stage; the gathered element is the same.
+struct Ix32 {
+    const int *L;
+    __device__ __forceinline__ Ix32 col(size_t off) const { return {L + off}; }
+    __device__ __forceinline__ int2 pair(int i) const {
+        const int2 j = *reinterpret_cast<const int2 *>(L + i);
+        return make_int2(j.x - 1, j.y - 1);
+    }
+    __device__ __forceinline__ int one(int i) const { return L[i] - 1; }
+};
+
+constexpr int LL16_ROWS = 512;  // rows per base block (two-row kernels: one block)
+constexpr int LL16_GHOST = -32768;
+
+// Decoded indices are clamped to [0, N] (N: a zero ghost row), so a coding
+// error can only give a wrong value, never an access outside the vectors.
+__device__ __forceinline__ int ll16_dec(int v, int i, int2 b, int N) {
+    int j;
+    if (v >= 16384) j = i + b.x + (v - 16384);
+    else if (v >= -16384) j = i + v;
+    else if (v == LL16_GHOST) j = N;
+    else j = i + b.y + (v + 16385);
+    return min(max(j, 0), N);
+}
+
+struct Ix16 {
+    const short *L;
+    const int2 *base;
+    int N;
+    __device__ __forceinline__ Ix16 col(size_t off) const { return {L + off, base, N}; }
+    __device__ __forceinline__ int2 pair(int i) const {
+        const short2 v = *reinterpret_cast<const short2 *>(L + i);
+        const int2 b = base[i / LL16_ROWS];
+        return make_int2(ll16_dec(v.x, i, b, N), ll16_dec(v.y, i + 1, b, N));
+    }
+    __device__ __forceinline__ int one(int i) const { return ll16_dec(L[i], i, base[i / LL16_ROWS], N); }
+};
+
+// Build the 16-bit form of the sorted LL: one block per LL16_ROWS rows finds
+// its nearest far offsets of each sign, then codes every slot.  *bad = 1 if
+// an offset does not fit (the operator then keeps the 32-bit path).
 __global__ void __launch_bounds__(BS)
-k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0,
-        const int *__restrict__ l0, const T *__restrict__ a1,
-        const int *__restrict__ l1, const T *__restrict__ in,
+k_ll16_build(int N, int lda, int NL, const int *__restrict__ LL, short *__restrict__ L16,
+             int2 *__restrict__ base, int *__restrict__ bad) {
+    const int r0 = blockIdx.x * LL16_ROWS;
+    int bp = 0x7fffffff, bn = -0x7fffffff - 1;
+    for (int u = threadIdx.x; u < LL16_ROWS; u += BS) {
+        const int i = r0 + u;
+        if (i >= N) break;
+        for (int k = 0; k < NL; ++k) {
+            const int j = LL[(size_t)k * lda + i];
+            if (j < 1 || j > N) continue;
+            const int off = j - 1 - i;
+            if (off >= 16384) bp = min(bp, off);
+            if (off < -16384) bn = max(bn, off);
+        }
+    }
+    __shared__ int lp[BS], ln[BS];
+    lp[threadIdx.x] = bp;
+    ln[threadIdx.x] = bn;
+    __syncthreads();
+    for (int st = BS / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            lp[threadIdx.x] = min(lp[threadIdx.x], lp[threadIdx.x + st]);
+            ln[threadIdx.x] = max(ln[threadIdx.x], ln[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    bp = lp[0];
+    bn = ln[0];
+    if (threadIdx.x == 0) base[blockIdx.x] = make_int2(bp, bn);
+    int fail = 0;
+    for (int u = threadIdx.x; u < LL16_ROWS; u += BS) {
+        const int i = r0 + u;
+        if (i >= N) break;
+        for (int k = 0; k < NL; ++k) {
+            const int j = LL[(size_t)k * lda + i];
+            int v;
+            if (j > N) {
+                v = LL16_GHOST;
+            } else if (j < 1) {
+                v = 0;
+                fail = 1;
+            } else {
+                const int off = j - 1 - i;
+                if (off >= 16384) {
+                    const int d = off - bp;
+                    fail |= d > 16383;
+                    v = 16384 + d;
+                } else if (off >= -16384) {
+                    v = off;
+                } else {
+                    const int d = bn - off;
+                    fail |= d > 16382;
+                    v = -16385 - d;
+                }
+            }
+            L16[(size_t)k * lda + i] = (short)v;
+        }
+    }
+    if (fail) atomicOr(bad, 1);
+}
+
+template <typename T, int NT, int DOT, typename IX>
+__global__ void __launch_bounds__(BS)
+k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
+        const T *__restrict__ a1, IX l1, const T *__restrict__ in,
         T *__restrict__ out, const T *__restrict__ w, double *__restrict__ part) {
     if (sc->done) return;
     const int lb = logical_block();
@@ -340,15 +448,15 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0,
     double s0 = 0.0, s1 = 0.0;
     if (i + 1 < N) {
         const auto c0 = ld2(a0 + i);
-        const auto j0 = ld2(l0 + i);
+        const int2 j0 = l0.pair(i);
         const auto xi = ld2(in + i);
-        T o0 = xi.x - c0.x * in[j0.x - 1];
-        T o1 = xi.y - c0.y * in[j0.y - 1];
+        T o0 = xi.x - c0.x * in[j0.x];
+        T o1 = xi.y - c0.y * in[j0.y];
         if (NT == 2) {
             const auto c1 = ld2(a1 + i);
-            const auto j1 = ld2(l1 + i);
-            o0 = o0 - c1.x * in[j1.x - 1];
-            o1 = o1 - c1.y * in[j1.y - 1];
+            const int2 j1 = l1.pair(i);
+            o0 = o0 - c1.x * in[j1.x];
+            o1 = o1 - c1.y * in[j1.y];
         }
         typename V2<T>::t o;
         o.x = o0;
@@ -368,8 +476,8 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0,
             s0 += (double)o1 * (double)o1;
         }
     } else if (i < N) {
-        T o = in[i] - a0[i] * in[l0[i] - 1];
-        if (NT == 2) o = o - a1[i] * in[l1[i] - 1];
+        T o = in[i] - a0[i] * in[l0.one(i)];
+        if (NT == 2) o = o - a1[i] * in[l1.one(i)];
         out[i] = o;
         if (DOT == 1 || DOT == 2) s0 += (double)w[i] * (double)o;
         if (DOT == 2) s1 += (double)o * (double)o;
@@ -386,41 +494,39 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0,
     }
 }
 
-template <typename T>
+template <typename T, typename IX>
 __global__ void __launch_bounds__(BS)
-k_pm2_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0,
-         const int *__restrict__ l0, const T *__restrict__ a1,
-         const int *__restrict__ l1, const T *__restrict__ in,
-         T *__restrict__ out) {
+k_pm2_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
+         const T *__restrict__ a1, IX l1, const T *__restrict__ in, T *__restrict__ out) {
     if (sc->done) return;
     const int i = (logical_block() * BS + threadIdx.x) * 2;
-    auto one = [&](int r, T c0, int jj0, T c1, int m, T xr) -> T {
-        const T o1i = xr - c0 * in[jj0 - 1];
+    // stage 1 at row r, and at its stage-2 neighbour m (a ghost m reads 0)
+    auto one = [&](T c0, int jj0, T c1, int m, T xr) -> T {
+        const T o1i = xr - c0 * in[jj0];
         T o1m = T(0);
-        if (m - 1 < N) o1m = in[m - 1] - a0[m - 1] * in[l0[m - 1] - 1];
+        if (m < N) o1m = in[m] - a0[m] * in[l0.one(m)];
         return o1i - c1 * o1m;
     };
     if (i + 1 < N) {
         const auto c0 = ld2(a0 + i);
-        const auto j0 = ld2(l0 + i);
+        const int2 j0 = l0.pair(i);
         const auto c1 = ld2(a1 + i);
-        const auto j1 = ld2(l1 + i);
+        const int2 j1 = l1.pair(i);
         const auto xi = ld2(in + i);
         typename V2<T>::t o;
-        o.x = one(i, c0.x, j0.x, c1.x, j1.x, xi.x);
-        o.y = one(i + 1, c0.y, j0.y, c1.y, j1.y, xi.y);
+        o.x = one(c0.x, j0.x, c1.x, j1.x, xi.x);
+        o.y = one(c0.y, j0.y, c1.y, j1.y, xi.y);
         *reinterpret_cast<typename V2<T>::t *>(out + i) = o;
     } else if (i < N) {
-        out[i] = one(i, a0[i], l0[i], a1[i], l1[i], in[i]);
+        out[i] = one(a0[i], l0.one(i), a1[i], l1.one(i), in[i]);
     }
 }
 
 // SpMV with two consecutive rows per thread (same per-row arithmetic).
-template <typename T, int NL, int MODE>
+template <typename T, int NL, int MODE, typename IX>
 __global__ void __launch_bounds__(BS)
-k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
-          const int *__restrict__ LL, const T *__restrict__ D,
-          const T *__restrict__ v0, const T *__restrict__ v1,
+k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL, IX LL,
+          const T *__restrict__ D, const T *__restrict__ v0, const T *__restrict__ v1,
           const T *__restrict__ v2, T *__restrict__ vout, T *__restrict__ y) {
     if (sc->done) return;
     T c0 = T(0), c1 = T(0);
@@ -441,7 +547,7 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
         int2 l[NL];
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
-            l[k] = ld2(LL + (size_t)k * lda + i);
+            l[k] = LL.col((size_t)k * lda).pair(i);
             a[k] = ld2(AL + (size_t)k * lda + i);
         }
         const auto d = ld2(D + i);
@@ -462,8 +568,8 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
         T acc0 = x0 * d.x, acc1 = x1 * d.y;
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
-            acc0 = acc0 + a[k].x * X(l[k].x - 1);
-            acc1 = acc1 + a[k].y * X(l[k].y - 1);
+            acc0 = acc0 + a[k].x * X(l[k].x);
+            acc1 = acc1 + a[k].y * X(l[k].y);
         }
         typename V2<T>::t o;
         if (MODE == 1 || MODE == 2) {
@@ -484,7 +590,7 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
         const T xi = X(i);
         T acc = xi * D[i];
 #pragma unroll
-        for (int k = 0; k < NL; ++k) acc = acc + AL[(size_t)k * lda + i] * X(LL[(size_t)k * lda + i] - 1);
+        for (int k = 0; k < NL; ++k) acc = acc + AL[(size_t)k * lda + i] * X(LL.col((size_t)k * lda).one(i));
         if (MODE == 1 || MODE == 2) vout[i] = xi;
         y[i] = MODE == 3 ? v1[i] - acc : acc;
     }

@@ -137,8 +137,9 @@ def bench_will(args):
     sol = Solver(0)
     sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
     if args.profile_kernels:
+        ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
         res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
-               (("k_spmv_v2<double,4,0>", 0), ("k_pm_v2<double,1,0>", 1), ("k_copy<double>", 4))}
+               ((f"k_spmv_v2<double,4,0,{ix}>", 0), (f"k_pm_v2<double,1,0,{ix}>", 1), ("k_copy<double>", 4))}
         print(json.dumps({"rows": N, "avg_us": res}))
         return
     rng = np.random.default_rng(7)
@@ -163,10 +164,11 @@ def bench_will(args):
     t_spmv = sol.time_kernel(0, 50)
     t_pm = sol.time_kernel(1, 50)
     t_copy = sol.time_kernel(4, 50)
+    ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
     if 2 * t_spmv >= 8 * t_pm:
-        kname, kms, kbytes = "k_spmv_v2<double,4,0>", t_spmv, 56 * N
+        kname, kms, kbytes = f"k_spmv_v2<double,4,0,{ix}>", t_spmv, 56 * N
     else:
-        kname, kms, kbytes = "k_pm_v2<double,1,0>", t_pm, ALG_BYTES["pm_stage"] * N
+        kname, kms, kbytes = f"k_pm_v2<double,1,0,{ix}>", t_pm, ALG_BYTES["pm_stage"] * N
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = None
     pmc = os.path.join(ROOT, "profiles", f"traffic_will{nx}x{ny}.json")
@@ -301,9 +303,10 @@ def main():
 
     if args.profile_kernels:
         dg.assemble_into(sol, b=b)
+        ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
         res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
-               (("k_spmv_v2<double,6,0>", 0), ("k_pm_v2<double,1,0>", 1), ("k_copy<double>", 4),
-                ("k_spmv_v2<float,6,0>", 2), ("k_pm_v2<float,1,0>", 3), ("k_pm2_v2<float>", 6))}
+               ((f"k_spmv_v2<double,6,0,{ix}>", 0), (f"k_pm_v2<double,1,0,{ix}>", 1), ("k_copy<double>", 4),
+                (f"k_spmv_v2<float,6,0,{ix}>", 2), (f"k_pm_v2<float,1,0,{ix}>", 3), (f"k_pm2_v2<float,{ix}>", 6))}
         print(json.dumps({"rows": N, "avg_us": res}))
         return
 
@@ -349,10 +352,11 @@ def main():
     t_spmv = sol.time_kernel(0, 50)
     t_pm = sol.time_kernel(1, 50)
     t_copy = sol.time_kernel(4, 50)
+    ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
     if 2 * t_spmv >= 8 * t_pm:
-        kname, kms, kbytes = "k_spmv_v2<double,6,0>", t_spmv, ALG_BYTES["spmv"] * N
+        kname, kms, kbytes = f"k_spmv_v2<double,6,0,{ix}>", t_spmv, ALG_BYTES["spmv"] * N
     else:
-        kname, kms, kbytes = "k_pm_v2<double,1,0>", t_pm, ALG_BYTES["pm_stage"] * N
+        kname, kms, kbytes = f"k_pm_v2<double,1,0,{ix}>", t_pm, ALG_BYTES["pm_stage"] * N
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = None
     pmc = os.path.join(ROOT, "profiles", f"traffic_config{args.config}.json")

@@ -253,6 +253,12 @@ int clowns_will_filter(clowns_solver *h, const int32_t *d_LL, int32_t NL,
                        const double *d_R, double *d_Xout, double *d_B,
                        int32_t *rounds);
 
+/* Width of the neighbour indices the resident operator's solve kernels read:
+ * 16 when the sorted LL was coded in 16 bits (every offset fits; always for
+ * box grids of up to 16383 cells a line), else 32; negative CLOWNS_E* on
+ * error.  No counterpart in the reference (diagnostics). */
+int clowns_solver_index_bits(clowns_solver *h);
+
 /* Library build string, e.g. "clowns_pcg gfx950 ...". */
 const char *clowns_version(void);
 
