@@ -7,5 +7,5 @@ synthetic workloads and exposes the adjoint as a torch.autograd.Function.
 from ._lib import ClownsError, Info, LIB_PATH, lib  # noqa: F401
 from .pcgpme import Solver, pcgpme, pcgpme_will  # noqa: F401
 from . import configs  # noqa: F401,E402
-from .assembly import DeviceGrid, classify  # noqa: F401,E402
+from .assembly import DeviceGrid, classify, set_dx  # noqa: F401,E402
 from .autograd import PCGSolve, pcg_solve  # noqa: F401,E402

@@ -75,6 +75,16 @@ def classify(g):
     return out
 
 
+def set_dx(x):
+    """set_dx of one axis (src2LOWNS/1sub-main.F90:53-74) through
+    clowns_set_dx: x holds the face coordinates x(ls-2 .. le+1).  Returns
+    (x, dx) -- x with a zero last face extrapolated as the reference does."""
+    x = np.array(x, dtype=np.float64)
+    dx = np.zeros_like(x)
+    L.check(L.lib().clowns_set_dx(_p(x), len(x), _p(dx)), "clowns_set_dx")
+    return x, dx
+
+
 class DeviceGrid:
     def __init__(self, g, device=0, classified=None):
         import torch

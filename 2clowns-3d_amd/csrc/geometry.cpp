@@ -10,6 +10,8 @@
 //   number_rows     src2LOWNS/1sub-momentum.F90:278-296 (ik == 0 block)
 //   surface_pos     set_initial_surface_position,
 //                   src2LOWNS/1sub-normal-ini.F90:23-40 (normal component)
+//   clowns_set_dx   set_dx, src2LOWNS/1sub-main.F90:53-74, with round,
+//                   lib/src/xy_calc.f90:101-112 (start-up, per axis)
 //
 // Box convention: cells i=0..nx+1, j=0..ny+1, k=0..nz+1 with a solid outer
 // ring, index c = i + X*(j + Y*k), X = nx+2, Y = ny+2 (k-major: the row
@@ -312,3 +314,27 @@ extern "C" int clowns_classify(const clowns_grid_desc *g, const int8_t *solid, d
     *nrows = N;
     return CLOWNS_OK;
 }
+
+namespace {
+// Fortran NINT: nearest integer, halves away from zero (v - trunc(v) is
+// exact in binary floating point, so the half test is too).
+double nint_away(double v) {
+    const double t = std::trunc(v);
+    return std::fabs(v - t) >= 0.5 ? t + (v > 0 ? 1.0 : -1.0) : t;
+}
+} // namespace
+
+extern "C" int clowns_set_dx(double *x, int32_t m, double *dx) {
+    if (!x || !dx || m < 4) return CLOWNS_EBADARG;
+    // positions in x[]/dx[] of the reference's indices ls and le
+    const int ls = 2, le = m - 2;
+    if (x[le + 1] == 0.0) x[le + 1] = x[le] + x[le] - x[le - 1];
+    // decimals guessed from the first cell size
+    int dec = 0;
+    while (dec <= 10 && !((x[ls + 1] - x[ls]) / std::pow(10.0, 4 - dec) > 1.0)) ++dec;
+    const double A = std::pow(10.0, dec);
+    for (int i = 0; i <= le; ++i) dx[i] = nint_away((x[i + 1] - x[i]) * A) / A;
+    dx[le + 1] = dx[le];
+    return CLOWNS_OK;
+}
+

@@ -253,6 +253,14 @@ int clowns_will_filter(clowns_solver *h, const int32_t *d_LL, int32_t NL,
                        const double *d_R, double *d_Xout, double *d_B,
                        int32_t *rounds);
 
+/* Replaces set_dx (src2LOWNS/1sub-main.F90:53-74) for one axis, with the
+ * reference's round (lib/src/xy_calc.f90:101-112): x[0..m) are the face
+ * coordinates x(ls-2 .. le+1) (m = le-ls+4 >= 4), dx[0..m) receives
+ * dx(ls-2 .. le+1).  A zero x(le+1) is extrapolated in place first; dx is
+ * rounded to the number of decimals the first cell size suggests.  Host
+ * function, start-up geometry. */
+int clowns_set_dx(double *x, int32_t m, double *dx);
+
 /* Width of the neighbour indices the resident operator's solve kernels read:
  * 16 when the sorted LL was coded in 16 bits (every offset fits; always for
  * box grids of up to 16383 cells a line), else 32; negative CLOWNS_E* on

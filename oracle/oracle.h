@@ -18,4 +18,5 @@ int oracle_pcgpme_will(const double *AL, int NL, int NA, const int *LL,
                        const double *D, int N, int N2, double *B, double EPS,
                        double PARM, int ITER, int *IER, int *K_out,
                        double *X_raw, double *X_out);
+int oracle_set_dx(double *x, int m, double *dx);
 #endif

@@ -488,3 +488,32 @@ double oracle_rhs(int nx, int ny, int nz, const int8_t *solid, const int8_t *nf,
     }
     return dmax;
 }
+
+/* round(num, dec), lib/src/xy_calc.f90:101-112: dfloat(nint(num*10**dec))
+ * / 10**dec, nint rounding half away from zero. */
+static double ref_round(double num, int dec) {
+    double A = pow(10.0, (double)dec);
+    double v = num * A;
+    double t = trunc(v);
+    double r = t;
+    if (fabs(v - t) >= 0.5) r = t + (v > 0 ? 1.0 : -1.0);
+    return r / A;
+}
+
+/* set_dx for one axis, src2LOWNS/1sub-main.F90:53-74.  x[0..m) holds the
+ * face coordinates x(ls-2 .. le+1) (m = le-ls+4 >= 4), dx[0..m) receives
+ * dx(ls-2 .. le+1).  x(le+1) = 0 is extrapolated first (:60-62); the number
+ * of decimals is the first dec in 0..10 with (x(ls+1)-x(ls))/10**(4-dec) > 1
+ * (:63-66; 11 if none); dx(i) = round(x(i+1)-x(i), dec) for i = ls-2..le and
+ * dx(le+1) = dx(le) (:67-71).  Returns 0, or -1 if m < 4. */
+int oracle_set_dx(double *x, int m, double *dx) {
+    if (m < 4) return -1;
+    const int ls = 2, le = m - 2; /* array positions of x(ls), x(le) */
+    if (x[le + 1] == 0.0) x[le + 1] = x[le] + x[le] - x[le - 1];
+    int dec;
+    for (dec = 0; dec <= 10; ++dec)
+        if ((x[ls + 1] - x[ls]) / pow(10.0, (double)(4 - dec)) > 1.0) break;
+    for (int i = 0; i <= le; ++i) dx[i] = ref_round(x[i + 1] - x[i], dec);
+    dx[le + 1] = dx[le];
+    return 0;
+}

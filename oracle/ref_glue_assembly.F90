@@ -199,3 +199,41 @@ subroutine ref_correction_pass(nx, ny, nz, xe_in, unv, pv, Bb, dmax_out) &
     Bb(1:mnoe) = b(1:mnoe)
     dmax_out = dmax
 end subroutine ref_correction_pass
+
+! set_dx (src2LOWNS/1sub-main.F90:53-74) on one axis: x(ls-2 .. le+1) in,
+! dx(ls-2 .. le+1) out, with ls = 1 and le = m - 3 on axis 0.  The module
+! arrays are allocated over that whole range (the reference's own
+! allocation, 0mod-arrays.F90:205-206, starts at 1).
+subroutine ref_set_dx(m, xv, dxv) bind(C, name="ref_set_dx")
+    use iso_c_binding
+    use arrays, only: ls, le, dx, x
+    implicit none
+    integer(c_int), value :: m
+    real(c_double), intent(inout) :: xv(0:m-1)
+    real(c_double), intent(out) :: dxv(0:m-1)
+    interface
+        subroutine set_dx()
+        end subroutine
+    end interface
+    integer :: i
+    if (allocated(dx)) deallocate(dx)
+    if (allocated(x)) deallocate(x)
+    allocate(dx(0:2, -1:m-2), x(0:2, -1:m-2))
+    dx = 0d0
+    x = 0d0
+    ls = [1, 1, 1]
+    le = [m - 3, 2, 2]
+    do i = 0, m - 1
+        x(0, i - 1) = xv(i)
+    enddo
+    ! axes 1 and 2: a small uniform grid so their loops stay in bounds
+    do i = -1, 3
+        x(1, i) = dble(i)
+        x(2, i) = dble(i)
+    enddo
+    call set_dx()
+    do i = 0, m - 1
+        dxv(i) = dx(0, i - 1)
+        xv(i) = x(0, i - 1)
+    enddo
+end subroutine ref_set_dx
