@@ -95,6 +95,7 @@ struct clowns_solver {
     DevBuf<short> LL16;    // LL coded in 16 bits (k_ll16_build), valid if ll16_ok
     DevBuf<int2> base16;
     bool ll16_ok = false;
+    int ll16_wide = 0;     // row blocks that stay on the 32-bit LL
     DevBuf<float> AL32, D32;
     bool have32 = false;
     // Transposed operator (adjoint), scaled + sorted the same way.
@@ -181,7 +182,7 @@ template <typename T> struct Op {
 // The index source of column j for the two-row kernels.
 template <typename T> bool has16(const Op<T> &A) { return A.LL16 != nullptr; }
 template <typename T> Ix32 ix32(const Op<T> &A, size_t off) { return Ix32{A.LL + off}; }
-template <typename T> Ix16 ix16(const Op<T> &A, size_t off) { return Ix16{A.LL16 + off, A.base16, A.N}; }
+template <typename T> Ix16 ix16(const Op<T> &A, size_t off) { return Ix16{A.LL16 + off, A.LL + off, A.base16, A.N}; }
 
 // Pairwise fusion of single-term PM stages (k_pm2) saves one pass over the
 // intermediate vector but recomputes the first stage at every gather point
@@ -511,19 +512,22 @@ int ensure_f32(clowns_solver *h) {
 // when every offset fits (always, for box grids up to 16383 cells a line).
 int build_ll16(clowns_solver *h) {
     h->ll16_ok = false;
+    h->ll16_wide = 0;
     if (!h_ll16 || !h_v2_32 || !h_v2_64 || !h_v2_vec) return CLOWNS_OK;
     const int N = h->N, nblk = cdiv(N, LL16_ROWS);
     int rc;
-    if ((rc = h->LL16.ensure((size_t)h->lda * h->NL)) || (rc = h->base16.ensure(nblk)) || (rc = h->flag.ensure(1)))
+    if ((rc = h->LL16.ensure((size_t)h->lda * h->NL)) || (rc = h->base16.ensure(nblk)) || (rc = h->flag.ensure(2)))
         return rc;
     hipStream_t s = h->stream;
-    fill<int>(s, h->flag.p, 1, 0);
-    k_ll16_build<<<nblk, BS, 0, s>>>(N, h->lda, h->NL, h->LL.p, h->LL16.p, h->base16.p, h->flag.p);
+    fill<int>(s, h->flag.p, 2, 0);
+    k_ll16_build<<<nblk, BS, 0, s>>>(N, h->lda, h->NL, h->LL.p, h->LL16.p, h->base16.p, h->flag.p,
+                                     h->flag.p + 1);
     HIPCHK(hipGetLastError());
-    int bad = 0;
-    HIPCHK(hipMemcpyAsync(&bad, h->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    int res[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(res, h->flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    h->ll16_ok = bad == 0;
+    h->ll16_ok = res[0] == 0;
+    h->ll16_wide = res[1];
     return CLOWNS_OK;
 }
 
