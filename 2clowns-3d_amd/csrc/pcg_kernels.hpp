@@ -333,10 +333,11 @@ __device__ __forceinline__ typename V2<T>::t ld2(const T *p) {
 // Ix32: the sorted LL itself.  Ix16: the same indices coded in 16 bits per
 // slot (k_ll16_build): an offset from the row when |offset| < 16384, else an
 // offset from the row block's nearest far offset of that sign (z
-// neighbours, up to a plane away), or the ghost code.  A block whose far
-// offsets spread too far for that (rare: a few blocks under a ragged
-// surface) is marked and read from the 32-bit LL.  Half the index bytes of
-// every SpMV and PM stage; the gathered element is the same.
+// neighbours, up to a plane away), or the ghost code, or -- for the rare
+// slot whose far offset spreads too far from its block's (under a ragged
+// surface) -- the wide code, which reads that slot from the 32-bit LL.  Half
+// the index bytes of every SpMV and PM stage; the gathered element is the
+// same.
 struct Ix32 {
     const int *L;
     __device__ __forceinline__ Ix32 col(size_t off) const { return {L + off}; }
@@ -349,46 +350,39 @@ struct Ix32 {
 
 constexpr int LL16_ROWS = 512;  // rows per base block (two-row kernels: one block)
 constexpr int LL16_GHOST = -32768;
-constexpr int LL16_WIDE = -0x7fffffff - 1;  // base.x of a block read from the 32-bit LL
-
-// Decoded indices are clamped to [0, N] (N: a zero ghost row), so a coding
-// error can only give a wrong value, never an access outside the vectors.
-__device__ __forceinline__ int ll16_dec(int v, int i, int2 b, int N) {
-    int j;
-    if (v >= 16384) j = i + b.x + (v - 16384);
-    else if (v >= -16384) j = i + v;
-    else if (v == LL16_GHOST) j = N;
-    else j = i + b.y + (v + 16385);
-    return min(max(j, 0), N);
-}
+constexpr int LL16_WIDE = -32767;
 
 struct Ix16 {
     const short *L;
-    const int *L32;  // the same column, 32-bit (blocks marked LL16_WIDE)
+    const int *L32;  // the same column, 32-bit (slots coded LL16_WIDE)
     const int2 *base;
     int N;
     __device__ __forceinline__ Ix16 col(size_t off) const { return {L + off, L32 + off, base, N}; }
+    // Decoded indices are clamped to [0, N] (N: a zero ghost row), so a
+    // coding error can only give a wrong value, never an access outside the
+    // vectors.
+    // Selects rather than branches for the common codes.
+    __device__ __forceinline__ int dec(int v, int i, int2 b) const {
+        int j = i + v;
+        j = v >= 16384 ? i + b.x + (v - 16384) : j;
+        j = v < -16384 ? i + b.y + (v + 16385) : j;
+        j = v == LL16_GHOST ? N : j;
+        if (v == LL16_WIDE) j = L32[i] - 1;
+        return min(max(j, 0), N);
+    }
     __device__ __forceinline__ int2 pair(int i) const {
         const int2 b = base[i / LL16_ROWS];
-        if (b.x == LL16_WIDE) {
-            const int2 j = *reinterpret_cast<const int2 *>(L32 + i);
-            return make_int2(j.x - 1, j.y - 1);
-        }
         const short2 v = *reinterpret_cast<const short2 *>(L + i);
-        return make_int2(ll16_dec(v.x, i, b, N), ll16_dec(v.y, i + 1, b, N));
+        return make_int2(dec(v.x, i, b), dec(v.y, i + 1, b));
     }
-    __device__ __forceinline__ int one(int i) const {
-        const int2 b = base[i / LL16_ROWS];
-        if (b.x == LL16_WIDE) return L32[i] - 1;
-        return ll16_dec(L[i], i, b, N);
-    }
+    __device__ __forceinline__ int one(int i) const { return dec(L[i], i, base[i / LL16_ROWS]); }
 };
 
 // Build the 16-bit form of the sorted LL: one block per LL16_ROWS rows finds
-// its nearest far offsets of each sign, then codes every slot; a block with
-// an offset that does not fit is marked LL16_WIDE and counted in *nwide.
-// *bad = 1 for an index outside [1, N] that is not a ghost (cannot happen
-// after the checks and the sort; the operator then keeps the 32-bit path).
+// its nearest far offsets of each sign, then codes every slot; *nwide counts
+// the slots left on the 32-bit LL.  *bad = 1 for an index outside [1, N]
+// that is not a ghost (cannot happen after the checks and the sort; the
+// operator then keeps the 32-bit path).
 __global__ void __launch_bounds__(BS)
 k_ll16_build(int N, int lda, int NL, const int *__restrict__ LL, short *__restrict__ L16,
              int2 *__restrict__ base, int *__restrict__ bad, int *__restrict__ nwide) {
@@ -418,7 +412,6 @@ k_ll16_build(int N, int lda, int NL, const int *__restrict__ LL, short *__restri
     }
     bp = lp[0];
     bn = ln[0];
-    __syncthreads();
     int wide = 0, invalid = 0;
     for (int u = threadIdx.x; u < LL16_ROWS; u += BS) {
         const int i = r0 + u;
@@ -435,30 +428,21 @@ k_ll16_build(int N, int lda, int NL, const int *__restrict__ LL, short *__restri
                 const int off = j - 1 - i;
                 if (off >= 16384) {
                     const int d = off - bp;
-                    wide |= d > 16383;
-                    v = 16384 + d;
+                    v = d <= 16383 ? 16384 + d : LL16_WIDE;
                 } else if (off >= -16384) {
                     v = off;
                 } else {
                     const int d = bn - off;
-                    wide |= d > 16382;
-                    v = -16385 - d;
+                    v = d <= 16381 ? -16385 - d : LL16_WIDE;
                 }
+                wide += v == LL16_WIDE;
             }
             L16[(size_t)k * lda + i] = (short)v;
         }
     }
     if (invalid) atomicOr(bad, 1);
-    lp[threadIdx.x] = wide;
-    __syncthreads();
-    for (int st = BS / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) lp[threadIdx.x] |= lp[threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        base[blockIdx.x] = make_int2(lp[0] ? LL16_WIDE : bp, bn);
-        if (lp[0]) atomicAdd(nwide, 1);
-    }
+    if (wide) atomicAdd(nwide, wide);
+    if (threadIdx.x == 0) base[blockIdx.x] = make_int2(bp, bn);
 }
 
 template <typename T, int NT, int DOT, typename IX>

@@ -95,7 +95,7 @@ struct clowns_solver {
     DevBuf<short> LL16;    // LL coded in 16 bits (k_ll16_build), valid if ll16_ok
     DevBuf<int2> base16;
     bool ll16_ok = false;
-    int ll16_wide = 0;     // row blocks that stay on the 32-bit LL
+    int ll16_wide = 0;     // slots read from the 32-bit LL (LL16_WIDE)
     DevBuf<float> AL32, D32;
     bool have32 = false;
     // Transposed operator (adjoint), scaled + sorted the same way.
