@@ -604,6 +604,25 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
     }
 }
 
+// E = R - ALP*Q (pcgpme_para.F90:173) as its own pass, two rows per thread.
+template <typename T>
+__global__ void __launch_bounds__(BS)
+k_eupd_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ r, const T *__restrict__ q,
+          T *__restrict__ e) {
+    if (sc->done) return;
+    const int i = (logical_block() * BS + threadIdx.x) * 2;
+    const T alp = (T)sc->ALP;
+    if (i + 1 < N) {
+        const auto rv = ld2(r + i), qv = ld2(q + i);
+        typename V2<T>::t o;
+        o.x = rv.x - alp * qv.x;
+        o.y = rv.y - alp * qv.y;
+        *reinterpret_cast<typename V2<T>::t *>(e + i) = o;
+    } else if (i < N) {
+        e[i] = r[i] - alp * q[i];
+    }
+}
+
 // x/r update with two consecutive rows per thread.
 template <typename T>
 __global__ void __launch_bounds__(BS)
