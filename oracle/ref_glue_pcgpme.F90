@@ -13,3 +13,10 @@ subroutine ref_set_threads(n) bind(C, name="ref_set_threads")
     NTHR = n
     call omp_set_num_threads(n)
 end subroutine ref_set_threads
+
+! Flush unit 6: the reference writes progress lines there (e.g. 'bn0=' on
+! IER = 3000, pcgpme_para.F90:250); a caller that silences stdout around a
+! call flushes them while silenced.
+subroutine ref_flush() bind(C, name="ref_flush")
+    flush(6)
+end subroutine ref_flush
