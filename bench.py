@@ -46,6 +46,8 @@ ALG_BYTES = {
     "pm_pair": 32,     # fused pair of stages: AL_j 8 + AL_j+1 8 + in 8 + out 8
     "iteration": 512,
     "setup_cold": 288,
+    "iteration32": 256,   # FP32 inner iteration of the mixed solve
+    "refine64": 216,      # FP64 true residual: SpMV with b - Ax (80) + one PM application (136)
 }
 
 
@@ -374,8 +376,12 @@ def main():
     if not args.no_mixed:
         t32, infos32 = run(True)
         i32 = infos32[-1]
+        mixed_bytes = N * (i32["iters"] * ALG_BYTES["iteration32"] + (i32["refinements"] + 1) * ALG_BYTES["refine64"])
+        mixed_gbs = mixed_bytes / (i32["ms"] * 1e-3) / 1e9
         mixed = {"value": t32, "ms_per_step": t32 * 1e3, "inner_iters": i32["iters"],
                  "refinements": i32["refinements"], "err": i32["err"], "ier": i32["ier"],
+                 "solve_ms_hip_events": i32["ms"], "solve_alg_GBps": mixed_gbs,
+                 "solve_alg_frac": mixed_gbs / PEAK_HBM_GBS,
                  "rel_l2_vs_tight_fp64": ((x - xt).norm() / xt.norm()).item(),
                  "rel_l2_vs_fp64": ((x - x64).norm() / x64.norm()).item(), "speedup_vs_fp64": t64 / t32}
 
