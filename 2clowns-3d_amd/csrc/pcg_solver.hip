@@ -204,6 +204,8 @@ template <typename T> Ix16 ix16(const Op<T> &A, size_t off) { return Ix16{A.LL16
 // the first time it solves at a given size (tune_fuse) and keeps the faster.
 // CLOWNS_FUSE_PM=0/1 forces both precisions off/on.
 int h_fuse_pm = -1;
+// CLOWNS_VERBOSE = 1: report per-operator tuning decisions on stderr.
+bool h_verbose = false;
 // Two rows per thread in the PM stages (k_pm_v2, k_pm2_v2): 8/16-byte
 // accesses, twice the bytes in flight per memory instruction.  Measured on
 // MI355X: single FP64 PM stage 4-5% faster (configurations 2 and 3); on for
@@ -318,10 +320,13 @@ int apply_pm(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, const T *in,
     return launch_pm_stage<T, 1>(s, g, sc, A, 3, t, out, dot, w, part);
 }
 
-// Time the fused pair against two single stages on the operator (one
-// warm-up and three launches each, HIP events on the solver's stream) and
-// keep the faster form for operators of this size.  V's y, t, e are scratch
-// here; the run's scalars are reset (kernels skip work once `done` is set).
+// Time the first four PM stages of an LDUSUB application as two fused pairs
+// and as four single stages, on the operator, as the solve runs them
+// (alternating vectors; HIP events on the solver's stream; the two forms
+// interleaved three times, the best of each kept), and keep the faster form
+// for operators of this size.  NL = 4 operators time stages 1-2 only (their
+// apply fuses one pair).  V's y, t, e are scratch here; the run's scalars are
+// reset (kernels skip work once `done` is set).
 template <typename T> bool tune_fuse(clowns_solver *h, const Op<T> &A, VecSet<T> &V) {
     if (h_fuse_pm >= 0) return h_fuse_pm == 1;
     int &dec = sizeof(T) == 4 ? h->fuse32 : h->fuse64;
@@ -330,23 +335,40 @@ template <typename T> bool tune_fuse(clowns_solver *h, const Op<T> &A, VecSet<T>
     hipStream_t s = h->stream;
     Scal *sc = h->sc.p;
     const Geo g = geo_of(A.N, h->R);
+    const int npairs = A.NL == 6 ? 2 : 1;
     k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1, 0);
-    auto fused = [&]() { launch_pm2<T>(s, g, sc, A, 0, V.y.p, V.t.p); };
+    auto fused = [&]() {
+        launch_pm2<T>(s, g, sc, A, 0, V.y.p, V.t.p);
+        if (npairs == 2) launch_pm2<T>(s, g, sc, A, 2, V.t.p, V.e.p);
+    };
     auto single = [&]() {
         launch_pm_stage<T, 1>(s, g, sc, A, 0, V.y.p, V.t.p, 0, nullptr, h->part.p);
         launch_pm_stage<T, 1>(s, g, sc, A, 1, V.t.p, V.e.p, 0, nullptr, h->part.p);
+        if (npairs == 2) {
+            launch_pm_stage<T, 1>(s, g, sc, A, 2, V.e.p, V.t.p, 0, nullptr, h->part.p);
+            launch_pm_stage<T, 1>(s, g, sc, A, 3, V.t.p, V.e.p, 0, nullptr, h->part.p);
+        }
     };
-    float ms[2] = {0.f, 0.f};
-    for (int form = 0; form < 2; ++form) {
-        form == 0 ? fused() : single();
-        if (hipEventRecord(h->ev0, s) != hipSuccess) return false;
-        for (int k = 0; k < 3; ++k) form == 0 ? fused() : single();
-        if (hipEventRecord(h->ev1, s) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess ||
-            hipEventElapsedTime(&ms[form], h->ev0, h->ev1) != hipSuccess)
-            return false;
+    float best[2] = {1e30f, 1e30f};
+    fused();
+    single();
+    for (int round = 0; round < 3; ++round) {
+        for (int form = 0; form < 2; ++form) {
+            float ms = 0.f;
+            if (hipEventRecord(h->ev0, s) != hipSuccess) return false;
+            form == 0 ? fused() : single();
+            if (hipEventRecord(h->ev1, s) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess ||
+                hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess)
+                return false;
+            best[form] = std::min(best[form], ms);
+        }
     }
-    dec = ms[0] < ms[1] ? 1 : 0;
+    dec = best[0] < best[1] ? 1 : 0;
     decN = A.N;
+    if (h_verbose)
+        fprintf(stderr, "clowns_pcg: N=%d %s PM stages 1-%d: fused pairs %.2f us, singles %.2f us -> %s\n", A.N,
+                sizeof(T) == 4 ? "FP32" : "FP64", 2 * npairs, 1e3f * best[0], 1e3f * best[1],
+                dec ? "fused" : "single");
     return dec == 1;
 }
 
@@ -783,6 +805,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2VEC")) h_v2_vec = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_LL16")) h_ll16 = atoi(e) != 0;
+    if (const char *e = getenv("CLOWNS_VERBOSE")) h_verbose = atoi(e) != 0;
     *out = h;
     return CLOWNS_OK;
 }
