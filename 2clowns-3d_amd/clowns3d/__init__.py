@@ -9,3 +9,4 @@ from .pcgpme import Solver, pcgpme, pcgpme_will  # noqa: F401
 from . import configs  # noqa: F401,E402
 from .assembly import DeviceGrid, classify, set_dx  # noqa: F401,E402
 from .autograd import PCGSolve, pcg_solve  # noqa: F401,E402
+from . import decks  # noqa: F401,E402

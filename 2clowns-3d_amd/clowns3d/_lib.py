@@ -18,6 +18,7 @@ CLOWNS_EBADARG = 4003
 CLOWNS_EHIP = 4004
 CLOWNS_ENOMEM = 4005
 CLOWNS_ESTRUCT = 4006
+CLOWNS_EDECK = 4007
 
 CLOWNS_MIXED = 1
 CLOWNS_WARM_START = 2
@@ -30,6 +31,7 @@ ERRORS = {
     CLOWNS_EHIP: "HIP runtime error",
     CLOWNS_ENOMEM: "device allocation failed",
     CLOWNS_ESTRUCT: "operator pattern is not structurally symmetric",
+    CLOWNS_EDECK: "unreadable case file, or cells outside the box model",
 }
 
 
@@ -75,6 +77,11 @@ SIGNATURES = {
     "clowns_will_filter": (ctypes.c_int, [_P, _P, _I, _I, _I, _P, _P, _P, _P, ctypes.POINTER(_I)]),
     "clowns_solver_index_bits": (ctypes.c_int, [_P]),
     "clowns_set_dx": (ctypes.c_int, [_P, _I, _P]),
+    "clowns_deck_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P]),
+    "clowns_deck_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _P] + [_P] * 8),
+    "clowns_deck_write_data": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _P, _P, _P, _D, _D, _I]),
+    "clowns_deck_faces": (ctypes.c_int, [ctypes.c_char_p, _P, _P]),
+    "clowns_deck_write_grid": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _I]),
     "clowns_version": (ctypes.c_char_p, []),
 }
 

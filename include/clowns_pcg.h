@@ -39,6 +39,7 @@ extern "C" {
 #define CLOWNS_EHIP 4004        /* HIP runtime error / no device             */
 #define CLOWNS_ENOMEM 4005      /* device allocation failed                  */
 #define CLOWNS_ESTRUCT 4006     /* transpose: pattern not structurally symmetric */
+#define CLOWNS_EDECK 4007       /* unreadable deck file, or cells outside the box model */
 
 /* Replaces PCGPME (src2LOWNS/pcgpme_para.F90:1, argument declarations
  * :35-46).  AL(NA,NL), LL(NA,NL) column-major, LL 1-based (0 = none);
@@ -260,6 +261,61 @@ int clowns_will_filter(clowns_solver *h, const int32_t *d_LL, int32_t NL,
  * rounded to the number of decimals the first cell size suggests.  Host
  * function, start-up geometry. */
 int clowns_set_dx(double *x, int32_t m, double *dx);
+
+/* ---- the reference's case files -------------------------------------
+ * Fortran unformatted sequential files (4-byte record lengths around each
+ * record; big-endian in the reference build, either order read).  Grid
+ * file: read_data (src2LOWNS/data_io.F90:98-128), ".xyz" (name ends in 'z':
+ * nf/nfb dense in the data file) or ".xyzn" (list of nf = -1 cells,
+ * output_xyzn :1531-1560).  Data file: read_data :130-156, :219-300 /
+ * output_data :708-783 (T, g, nu; flags when dense; U, F, P of the numbered
+ * cells; dt).  Open-ratio file: read_fdata :304-330 (a0, fb0).  The box model
+ * covers cells with nf = -1 and nfb = 0 (walls, solid objects); other cells
+ * give CLOWNS_EDECK. */
+typedef struct clowns_deck_desc {
+    int32_t is, js, ks, ie, je, ke;  /* the grid file's extents */
+    int32_t nx, ny, nz;              /* box interior: ie-is, je-js, ke-ks */
+    int32_t stride;                  /* max(nx,ny,nz)+2: axis array stride */
+    int32_t sparse;                  /* 1: .xyzn grid file, 0: .xyz */
+    int32_t big_endian;              /* byte order of the grid file */
+    int32_t inne;                    /* numbered (non-solid) cells */
+    int32_t has_dt0;                 /* t0 > 0 and the data file ends with dt0 */
+    double t0, g, nu, dt0;           /* the data file's header (with t0 = 0 the
+                                        reference takes g, nu from its own
+                                        parameter file instead) */
+} clowns_deck_desc;
+
+/* Read the headers and the cell flags (box dims, numbering count). */
+int clowns_deck_open(const char *grid_path, const char *data_path,
+                     clowns_deck_desc *d);
+/* Fill the box arrays (ncell = (nx+2)(ny+2)(nz+2), axis arrays
+ * [3][stride]): solid, F, un[3][ncell] (the data file's U), p, a[3][ncell]
+ * and fb (from fdata_path, or 1 for numbered cells and 0 for solid ones when
+ * it is NULL or ""), x = low-face coordinates X(l, ls-1+c) and dx from
+ * set_dx (clowns_set_dx).  d from clowns_deck_open; dt0 is filled in. */
+int clowns_deck_read(const char *grid_path, const char *data_path,
+                     const char *fdata_path, clowns_deck_desc *d,
+                     int8_t *solid, double *F, double *un, double *p,
+                     double *a, double *fb, double *x, double *dx);
+/* The grid file's faces as stored: faces[l*(stride+2) + q] = X(l, ls-2+q),
+ * q = 0 .. n_l+3 (read_data, src2LOWNS/data_io.F90:113-116); the rest 0. */
+int clowns_deck_faces(const char *grid_path, const clowns_deck_desc *d,
+                      double *faces);
+/* Write a sparse (.xyzn) grid file in output_xyzn's format
+ * (src2LOWNS/data_io.F90:1505-1560, iflag = 0): the extents of d, faces as
+ * clowns_deck_faces returns them, and every solid box cell as nf = -1,
+ * nfb = 0. */
+int clowns_deck_write_grid(const char *path, const clowns_deck_desc *d,
+                           const int8_t *solid, const double *faces,
+                           int32_t big_endian);
+/* Write a data file in output_data's format from box arrays: nf/nb the cell
+ * flags (clowns_classify), un, F, p; t and dt its first and last records;
+ * d->sparse selects the .xyzn form. */
+int clowns_deck_write_data(const char *path, const clowns_deck_desc *d,
+                           const int8_t *nf, const int8_t *nb,
+                           const double *un, const double *F,
+                           const double *p, double t, double dt,
+                           int32_t big_endian);
 
 /* Width of the neighbour indices the resident operator's solve kernels read:
  * 16 when the sorted LL was coded in 16 bits (every offset fits; always for

@@ -1,0 +1,181 @@
+"""Synthetic 2CLOWNS-3D case files for the deck tests: a box with a solid
+ring and a solid block inside, written as Fortran unformatted sequential
+records (4-byte length, payload, length) in either byte order, in the dense
+(".xyz" grid, nf/nfb in the data file) or the sparse (".xyzn" grid listing
+the nf = -1 cells, data file listing cells with state) form that
+src2LOWNS/data_io.F90 reads (read_data :87-300, read_fdata :304-330).
+
+Along with the files, the generator returns what a reader must produce,
+per box cell c = i + X*(j + Y*k) (i, j, k counted from the reference's
+is-1, js-1, ks-1): solid, U, F, P, a, fb, the flags nf/nb the reference
+keeps for each numbered cell, and the coordinates.
+"""
+import struct
+
+import numpy as np
+
+
+def _rec(f, fmt, vals, be):
+    e = ">" if be else "<"
+    payload = struct.pack(e + fmt, *vals) if fmt else b""
+    m = struct.pack(e + "i", len(payload))
+    f.write(m + payload + m)
+
+
+def make_case(tmp, nx, ny, nz, seed, sparse, be=False, lo=(3, 3, 3), t0=12.5, with_dt0=True, obstacle=True,
+              object_cell=False, fdata=True, sparse_frac=0.6):
+    rng = np.random.default_rng(seed)
+    X, Y, Z = nx + 2, ny + 2, nz + 2
+    nc = X * Y * Z
+    is_, js, ks = lo
+    ie, je, ke = is_ + nx, js + ny, ks + nz
+    solid = np.zeros((Z, Y, X), np.int8)
+    solid[0], solid[-1] = 1, 1
+    solid[:, 0], solid[:, -1] = 1, 1
+    solid[:, :, 0], solid[:, :, -1] = 1, 1
+    if obstacle:
+        solid[1:1 + max(1, nz // 2), Y // 3:Y // 3 + 2, X // 2:X // 2 + 2] = 1
+    solid = solid.ravel()
+    nf = np.where(solid == 1, -1, rng.integers(0, 3, nc)).astype(np.int32)
+    nfb = np.where(solid == 1, 0, rng.integers(0, 2, nc)).astype(np.int32)
+    if object_cell:  # a boundary object the box model does not cover
+        c = 1 + X * (1 + Y * 1)
+        nf[c], nfb[c] = -1, 7
+    # numbering: j outer, k, i inner over nf >= 0 (data_io.F90:160-176)
+    cells = [i + X * (j + Y * k) for j in range(Y) for k in range(Z) for i in range(X) if nf[i + X * (j + Y * k)] >= 0]
+    cells = np.array(cells, np.int64)
+    inne = len(cells)
+    # coordinates X(l, ls-2 .. le+1): n+4 faces, cell sizes on a 0.01 grid
+    coords = []
+    for n, base in zip((nx, ny, nz), (-3.0, 10.0, -1.5)):
+        h = rng.integers(20, 90, n + 3) / 100.0
+        coords.append(np.concatenate([[base], base + np.cumsum(h)]))
+    U = np.zeros((3, nc))
+    F = np.zeros(nc)
+    P = np.zeros(nc)
+    A = np.where(solid == 1, 0.0, 1.0)[None].repeat(3, 0)
+    FB = np.where(solid == 1, 0.0, 1.0)
+    nf_keep = nf.copy()
+    nb_keep = nfb.copy()
+    listed = np.ones(inne, bool)
+    if sparse:
+        listed = rng.random(inne) < sparse_frac
+    for q, c in enumerate(cells):
+        if listed[q]:
+            U[:, c] = rng.standard_normal(3)
+            F[c] = rng.random()
+            P[c] = rng.standard_normal() * 100
+    if sparse:
+        # the data records carry the flags of the cells they list; the others
+        # keep the grid's nf = 0, nfb = 0
+        for q, c in enumerate(cells):
+            if listed[q]:
+                nf_keep[c] = int(rng.integers(0, 3))
+                nb_keep[c] = int(rng.integers(0, 2))
+            else:
+                nf_keep[c] = nb_keep[c] = 0
+    a_listed = rng.random(inne) < 0.5
+    for q, c in enumerate(cells):
+        if not sparse or a_listed[q]:
+            A[:, c] = rng.integers(0, 101, 3) / 100.0
+            FB[c] = rng.integers(1, 101) / 100.0
+    g, nu, dt0 = 9.8, 1.0e-6, 0.0025
+    ext = (is_, js, ks, ie, je, ke)
+    gname = tmp / ("case.xyzn" if sparse else "case.xyz")
+    dname = tmp / "case.data"
+    fname = tmp / "case.fdata0"
+    with open(gname, "wb") as f:
+        _rec(f, "6i", ext, be)
+        for xs in coords:
+            _rec(f, f"{len(xs)}d", xs, be)
+        if sparse:
+            sol = [(i, j, k) for i in range(X) for j in range(Y) for k in range(Z) if nf[i + X * (j + Y * k)] == -1]
+            _rec(f, "i", [len(sol)], be)
+            for i, j, k in sol:
+                c = i + X * (j + Y * k)
+                _rec(f, "5i", [i + is_ - 1, k + ks - 1, j + js - 1, nf[c], nfb[c]], be)
+    with open(dname, "wb") as f:
+        _rec(f, "3d", [t0, g, nu], be)
+        if not sparse:
+            # (((NF3(I,k,j),K=ks-1,KE),J=js-1,je),I=is-1,IE): k fastest, i slowest
+            order = [i + X * (j + Y * k) for i in range(X) for j in range(Y) for k in range(Z)]
+            _rec(f, f"{nc}i", nf[order], be)
+            _rec(f, f"{nc}i", nfb[order], be)
+            for c in cells:
+                _rec(f, "5d", [*U[:, c], F[c], P[c]], be)
+        else:
+            idx = np.nonzero(listed)[0]
+            _rec(f, "2i", [len(idx), 0], be)
+            for q in idx:
+                c = cells[q]
+                _rec(f, "i5d2i", [q + 1, *U[:, c], F[c], P[c], nf_keep[c], nb_keep[c]], be)
+        if with_dt0:
+            _rec(f, "d", [dt0], be)
+    if fdata:
+        with open(fname, "wb") as f:
+            if not sparse:
+                for c in cells:
+                    _rec(f, "4d", [*A[:, c], FB[c]], be)
+            else:
+                idx = np.nonzero(a_listed)[0]
+                _rec(f, "2i", [len(idx), 0], be)
+                for q in idx:
+                    c = cells[q]
+                    _rec(f, "i4d", [q + 1, *A[:, c], FB[c]], be)
+    return dict(grid=gname, data=dname, fdata=fname if fdata else None, ext=ext, nx=nx, ny=ny, nz=nz,
+                X=X, Y=Y, Z=Z, nc=nc, solid=solid, U=U, F=F, P=P, A=A, FB=FB, nf=nf_keep, nb=nb_keep,
+                cells=cells, inne=inne, coords=coords, t0=t0, g=g, nu=nu, dt0=dt0)
+
+
+def write_grid_case(tmp, g, nf, nb, p, sparse=False, be=False, lo=(3, 3, 3), t0=1.0, g_acc=9.8, nu=1.0e-6):
+    """Case files holding a configs.Grid as it is: grid file (faces
+    X(l, ls-2 .. le+1) continuing g.xf), data file (nf/nb the flags per box
+    cell, -1 for solid ones; U, F, p; dt0 = g.dt) and open-ratio file
+    (g.a, g.fb).  Returns the three paths."""
+    X, Y, Z = g.X, g.Y, g.Z
+    nc = g.ncell
+    nf = np.asarray(nf).ravel().astype(np.int32)
+    nb = np.asarray(nb).ravel().astype(np.int32)
+    un = g.un.reshape(3, -1)
+    F = g.F.ravel()
+    a = g.a.reshape(3, -1)
+    fb = g.fb.ravel()
+    p = np.asarray(p).ravel()
+    is_, js, ks = lo
+    ext = (is_, js, ks, is_ + g.nx, js + g.ny, ks + g.nz)
+    cells = [i + X * (j + Y * k) for j in range(Y) for k in range(Z) for i in range(X) if nf[i + X * (j + Y * k)] >= 0]
+    gname = tmp / ("case.xyzn" if sparse else "case.xyz")
+    dname, fname = tmp / "case.data", tmp / "case.fdata0"
+    with open(gname, "wb") as f:
+        _rec(f, "6i", ext, be)
+        for l, n in enumerate((g.nx, g.ny, g.nz)):
+            h = g.xf[l, 1] - g.xf[l, 0]
+            faces = np.concatenate([[g.xf[l, 0] - h], g.xf[l, :n + 2], [g.xf[l, n + 1] + h]])
+            _rec(f, f"{n + 4}d", faces, be)
+        if sparse:
+            sol = [(i, j, k) for i in range(X) for j in range(Y) for k in range(Z) if nf[i + X * (j + Y * k)] == -1]
+            _rec(f, "i", [len(sol)], be)
+            for i, j, k in sol:
+                _rec(f, "5i", [i + is_ - 1, k + ks - 1, j + js - 1, -1, 0], be)
+    with open(dname, "wb") as f:
+        _rec(f, "3d", [t0, g_acc, nu], be)
+        if not sparse:
+            order = [i + X * (j + Y * k) for i in range(X) for j in range(Y) for k in range(Z)]
+            _rec(f, f"{nc}i", nf[order], be)
+            _rec(f, f"{nc}i", np.where(nf < 0, 0, nb)[order], be)
+            for c in cells:
+                _rec(f, "5d", [*un[:, c], F[c], p[c]], be)
+        else:
+            _rec(f, "2i", [len(cells), 0], be)
+            for q, c in enumerate(cells):
+                _rec(f, "i5d2i", [q + 1, *un[:, c], F[c], p[c], nf[c], nb[c]], be)
+        _rec(f, "d", [g.dt], be)
+    with open(fname, "wb") as f:
+        if not sparse:
+            for c in cells:
+                _rec(f, "4d", [*a[:, c], fb[c]], be)
+        else:
+            _rec(f, "2i", [len(cells), 0], be)
+            for q, c in enumerate(cells):
+                _rec(f, "i4d", [q + 1, *a[:, c], fb[c]], be)
+    return gname, dname, fname

@@ -1,0 +1,327 @@
+"""The case readers and the data-file writer (clowns_deck_open/read/
+write_data) against the reference's own read_data, read_fdata and
+output_data (src2LOWNS/data_io.F90:87-364, :708-783) built with flang into
+oracle/_ref/libref_decks.so, on synthetic dense (".xyz") and sparse
+(".xyzn") cases.
+
+Bar, all bit-exact:
+- every field the reader fills (U, F, P, a, fb, solid, coordinates, header,
+  dt0) equals what the fixture wrote, in both byte orders;
+- the reference's reader numbers the same cells in the same order and reads
+  the same values into them;
+- the writer's file is byte-identical to output_data's on the same state,
+  and reading it back returns the same fields;
+- cells outside the box model (an nf = -1 cell with nfb != 0) and truncated
+  files give CLOWNS_EDECK.
+The flang runtime of the reference build reads and writes the host byte
+order (no main program sets its conversion), so the comparisons with it
+use little-endian files; the big-endian form, the one the reference's own
+build writes, is checked against the same fixture.
+"""
+import ctypes
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from _deckfiles import make_case
+from conftest import ROOT, ptr
+
+REF_DECKS_SO = os.path.join(ROOT, "oracle", "_ref", "libref_decks.so")
+CASES = [("dense", False), ("sparse", True)]
+
+
+def _ref_lib():
+    if not os.path.exists(REF_DECKS_SO):
+        pytest.skip("flang build of the reference (oracle/_ref) not present")
+    return ctypes.CDLL(REF_DECKS_SO, mode=os.RTLD_LAZY)
+
+
+def ref_read(case, data=None):
+    """The reference's read_data + read_fdata on the case; returns per
+    numbered cell its box index and values."""
+    lib = _ref_lib()
+    gp, dp = str(case["grid"]).encode(), str(data or case["data"]).encode()
+    fp = str(case["fdata"]).encode()
+    nmax = case["nc"]
+    S = max(case["nx"], case["ny"], case["nz"]) + 8
+    ext = np.zeros(6, np.int32)
+    counts = np.zeros(2, np.int32)
+    hdr = np.zeros(4)
+    xo = np.zeros((3, S))
+    cells = np.zeros((nmax, 3), np.int32)
+    uo = np.zeros((nmax, 3))
+    fo, po, fbo = np.zeros(nmax), np.zeros(nmax), np.zeros(nmax)
+    ao = np.zeros((nmax, 3))
+    flags = np.zeros((nmax, 2), np.int32)
+    lib.ref_read_deck(gp, len(gp), dp, len(dp), fp, len(fp), ptr(ext), ptr(counts), ptr(hdr), ptr(xo), S, nmax,
+                      ptr(cells), ptr(uo), ptr(fo), ptr(po), ptr(ao), ptr(fbo), ptr(flags))
+    n = int(counts[0])
+    is_, js, ks = ext[:3]
+    X, Y = case["X"], case["Y"]
+    i, j, k = cells[:n, 0] - (is_ - 1), cells[:n, 1] - (js - 1), cells[:n, 2] - (ks - 1)
+    return dict(ext=ext, inne=n, inne2=int(counts[1]), hdr=hdr, xo=xo, box=i + X * (j + Y * k),
+                U=uo[:n].T, F=fo[:n], P=po[:n], A=ao[:n].T, FB=fbo[:n], flags=flags[:n])
+
+
+def ref_write(path, t, dt):
+    """output_data on the state the last ref_read left."""
+    lib = _ref_lib()
+    p = str(path).encode()
+    lib.ref_write_data(p, len(p), ctypes.c_double(t), ctypes.c_double(dt))
+
+
+def ours(case, data=None, fdata="fixture"):
+    from clowns3d import decks
+    fd = case["fdata"] if fdata == "fixture" else fdata
+    return decks.read_case(case["grid"], data or case["data"], fd)
+
+
+def _check_fields(case, g, p, d):
+    assert (d.nx, d.ny, d.nz) == (case["nx"], case["ny"], case["nz"])
+    assert (d.is_, d.js, d.ks, d.ie, d.je, d.ke) == case["ext"]
+    assert d.inne == case["inne"]
+    assert (d.t0, d.g, d.nu) == (case["t0"], case["g"], case["nu"])
+    assert np.array_equal(g.solid.ravel(), case["solid"])
+    assert np.array_equal(g.un.reshape(3, -1), case["U"])
+    assert np.array_equal(g.F.ravel(), case["F"])
+    assert np.array_equal(p.ravel(), case["P"])
+    if case["fdata"] is not None:
+        assert np.array_equal(g.a.reshape(3, -1), case["A"])
+        assert np.array_equal(g.fb.ravel(), case["FB"])
+    for l, n in enumerate((case["nx"], case["ny"], case["nz"])):
+        # x[l, c] = X(l, ls-1+c) for the box cells c = 0 .. n+1
+        assert np.array_equal(g.xf[l, :n + 2], case["coords"][l][1:n + 3])
+
+
+@pytest.mark.parametrize("name,sparse", CASES)
+@pytest.mark.parametrize("be", [False, True])
+def test_reader_matches_fixture(tmp_path, name, sparse, be):
+    from clowns3d import set_dx
+    case = make_case(tmp_path, 7, 5, 4, seed=3 + sparse, sparse=sparse, be=be)
+    g, p, d = ours(case)
+    _check_fields(case, g, p, d)
+    assert d.big_endian == int(be) and d.sparse == int(sparse)
+    assert d.has_dt0 == 1 and d.dt0 == case["dt0"] and g.dt == case["dt0"]
+    for l, n in enumerate((case["nx"], case["ny"], case["nz"])):
+        _, dx = set_dx(case["coords"][l])
+        assert np.array_equal(g.dx[l, :n + 2], dx[1:n + 3])
+
+
+@pytest.mark.parametrize("name,sparse", CASES)
+def test_reader_without_fdata_defaults(tmp_path, name, sparse):
+    case = make_case(tmp_path, 5, 4, 3, seed=11, sparse=sparse, fdata=False)
+    g, p, d = ours(case, fdata=None)
+    open_ = case["solid"] == 0
+    assert np.array_equal(g.a.reshape(3, -1), np.where(open_, 1.0, 0.0)[None].repeat(3, 0))
+    assert np.array_equal(g.fb.ravel(), np.where(open_, 1.0, 0.0))
+
+
+@pytest.mark.parametrize("name,sparse", CASES)
+@pytest.mark.parametrize("shape,lo", [((7, 5, 4), (3, 3, 3)), ((12, 9, 6), (5, 3, 4))])
+def test_reader_matches_reference(tmp_path, name, sparse, shape, lo):
+    case = make_case(tmp_path, *shape, seed=sum(shape) + sparse, sparse=sparse, lo=lo)
+    r = ref_read(case)
+    g, p, d = ours(case)
+    assert tuple(r["ext"]) == case["ext"]
+    assert r["inne"] == r["inne2"] == d.inne
+    assert np.array_equal(r["box"], case["cells"])  # same numbering, same order
+    assert tuple(r["hdr"]) == (d.t0, d.g, d.nu, d.dt0)
+    c = r["box"]
+    assert np.array_equal(r["U"], g.un.reshape(3, -1)[:, c])
+    assert np.array_equal(r["F"], g.F.ravel()[c])
+    assert np.array_equal(r["P"], p.ravel()[c])
+    assert np.array_equal(r["A"], g.a.reshape(3, -1)[:, c])
+    assert np.array_equal(r["FB"], g.fb.ravel()[c])
+    assert np.array_equal(r["flags"][:, 0], case["nf"][c])
+    assert np.array_equal(r["flags"][:, 1], case["nb"][c])
+    for l, n in enumerate(shape):
+        assert np.array_equal(r["xo"][l, 1:n + 3], g.xf[l, :n + 2])
+    # every cell the reference does not number is solid in the box model
+    rest = np.setdiff1d(np.arange(case["nc"]), c)
+    assert np.all(g.solid.ravel()[rest] == 1) and np.all(g.solid.ravel()[c] == 0)
+
+
+def test_dense_restart_time_zero(tmp_path):
+    """t0 = 0: the fields are still read, dt0 is not (data_io.F90:283-291)."""
+    case = make_case(tmp_path, 6, 5, 3, seed=2, sparse=False, t0=0.0)
+    g, p, d = ours(case)
+    _check_fields(case, g, p, d)
+    assert d.has_dt0 == 0 and d.dt0 == 0.0
+    r = ref_read(case)
+    c = r["box"]
+    assert r["hdr"][3] == 0.0
+    assert np.array_equal(r["P"], p.ravel()[c]) and np.array_equal(r["U"], g.un.reshape(3, -1)[:, c])
+
+
+@pytest.mark.parametrize("name,sparse", CASES)
+def test_writer_byte_identical_to_reference(tmp_path, name, sparse):
+    from clowns3d import decks
+    case = make_case(tmp_path, 8, 6, 5, seed=21 + sparse, sparse=sparse)
+    g, p, d = ours(case)
+    nf = case["nf"].astype(np.int8)
+    nb = case["nb"].astype(np.int8)
+    rng = np.random.default_rng(4)
+    # a new (solved) pressure on the open cells
+    p2 = p.ravel() + rng.standard_normal(case["nc"]) * np.where(case["solid"] == 0, 1.0, 0.0)
+    t, dt = 37.25, 0.00125
+    ref_read(case)
+    # put the new pressure into the reference's state through a data file
+    # it reads: ours, written once, read by the reference
+    mine = tmp_path / "mine.data"
+    decks.write_data(mine, d, nf, nb, g.un.reshape(3, -1), g.F.ravel(), p2, t, dt, big_endian=False)
+    r = ref_read(case, data=mine)
+    c = r["box"]
+    U = g.un.reshape(3, -1)
+    expect = p2[c]
+    if sparse:  # cells without state are not listed and read back as zero
+        dropped = (nf[c] == 0) & ((U[:, c] ** 2).sum(0) == 0) & (g.F.ravel()[c] < 1e-13)
+        assert dropped.any() and not dropped.all()
+        expect = np.where(dropped, 0.0, expect)
+    assert np.array_equal(r["P"], expect)
+    assert np.array_equal(r["U"], g.un.reshape(3, -1)[:, r["box"]])
+    theirs = tmp_path / "theirs.data"
+    ref_write(theirs, t, dt)
+    assert mine.read_bytes() == theirs.read_bytes()
+
+
+@pytest.mark.parametrize("name,sparse", CASES)
+def test_writer_round_trip_big_endian(tmp_path, name, sparse):
+    from clowns3d import decks
+    case = make_case(tmp_path, 6, 7, 4, seed=31 + sparse, sparse=sparse, be=True)
+    g, p, d = ours(case)
+    out = tmp_path / "out.data"
+    decks.write_data(out, d, case["nf"], case["nb"], g.un.reshape(3, -1), g.F, p * 2.0, 5.0, 0.5, big_endian=True)
+    raw = out.read_bytes()
+    assert raw[:4] == (24).to_bytes(4, "big")
+    g2, p2, d2 = ours(case, data=out)
+    assert d2.t0 == 5.0 and d2.dt0 == 0.5 and d2.big_endian == 1
+    assert np.array_equal(p2, p * 2.0)
+    assert np.array_equal(g2.un, g.un) and np.array_equal(g2.F, g.F)
+
+
+def test_sparse_writer_drops_stateless_cells(tmp_path):
+    """The .xyzn data form lists only cells with nf != 0, U != 0 or F >=
+    1e-13 (data_io.F90:745-759)."""
+    from clowns3d import decks
+    case = make_case(tmp_path, 5, 4, 3, seed=8, sparse=True)
+    g, p, d = ours(case)
+    nc = case["nc"]
+    un = np.zeros((3, nc))
+    F = np.zeros(nc)
+    nf = np.where(case["solid"] == 1, -1, 0).astype(np.int8)
+    nb = np.zeros(nc, np.int8)
+    c = case["cells"]
+    F[c[0]] = 1e-13            # kept: F >= 1e-13
+    F[c[1]] = 0.99e-13         # dropped
+    un[1, c[2]] = 1e-100       # kept: |U|^2 != 0
+    nf[c[3]] = 2               # kept: nf != 0
+    un[0, c[4]] = 1e-300       # dropped: the rule tests |U|^2, which underflows to 0
+    out = tmp_path / "out.data"
+    decks.write_data(out, d, nf, nb, un, F, np.arange(nc, dtype=float), 1.0, 0.1, big_endian=False)
+    raw = out.read_bytes()
+    icc = int.from_bytes(raw[32 + 4:32 + 8], "little")
+    assert icc == 3
+    g2, p2, _ = ours(case, data=out)
+    kept = [c[0], c[2], c[3]]
+    assert np.array_equal(p2.ravel()[kept], np.array(kept, float))
+    assert p2.ravel()[c[1]] == 0.0 and p2.ravel()[c[4]] == 0.0
+
+
+def test_object_cells_rejected(tmp_path):
+    from clowns3d import decks
+    from clowns3d._lib import CLOWNS_EDECK, ClownsError
+    for sparse in (False, True):
+        sub = tmp_path / str(sparse)
+        sub.mkdir()
+        case = make_case(sub, 5, 4, 3, seed=9, sparse=sparse, object_cell=True)
+        with pytest.raises(ClownsError) as e:
+            decks.read_case(case["grid"], case["data"], case["fdata"])
+        assert e.value.code == CLOWNS_EDECK
+
+
+def test_truncated_and_missing_files(tmp_path):
+    from clowns3d import decks
+    from clowns3d._lib import CLOWNS_EDECK, ClownsError
+    case = make_case(tmp_path, 5, 4, 3, seed=10, sparse=False)
+    raw = case["data"].read_bytes()
+    cut = tmp_path / "cut.data"
+    cut.write_bytes(raw[:len(raw) // 2])
+    for args in ((case["grid"], cut, case["fdata"]), (tmp_path / "none.xyz", case["data"], None),
+                 (case["grid"], case["data"], tmp_path / "none.fdata0")):
+        with pytest.raises(ClownsError) as e:
+            decks.read_case(*args)
+        assert e.value.code == CLOWNS_EDECK
+    short = tmp_path / "short.xyz"
+    shutil.copy(case["grid"], short)
+    with open(short, "r+b") as f:
+        f.truncate(40)
+    with pytest.raises(ClownsError):
+        decks.read_case(short, case["data"], None)
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+def test_grid_through_files(tmp_path, sparse):
+    """A configs.Grid written as case files reads back as the same Grid:
+    every array the classification and the assembly take, with dx from
+    set_dx of the faces equal to the uniform cell sizes."""
+    from clowns3d import assembly
+    from _deckfiles import write_grid_case
+    from test_oracle_assembly import GRIDS
+    g0 = GRIDS["wavy32x24x16"]()
+    c = assembly.classify(g0)
+    p0 = np.random.default_rng(3).standard_normal(g0.ncell) * (g0.solid.ravel() == 0)
+    paths = write_grid_case(tmp_path, g0, c["nf"], c["nb"], p0, sparse=sparse, be=True)
+    g1, p1, d = decks_read(*paths)
+    for k in ("solid", "F", "a", "fb"):
+        assert np.array_equal(getattr(g1, k), getattr(g0, k)), k
+    # velocities live on numbered cells only; solid ones read back as zero
+    open_ = g0.solid == 0
+    assert np.array_equal(g1.un, np.where(open_, g0.un, 0.0))
+    for l, n in enumerate((g0.nx, g0.ny, g0.nz)):
+        assert np.array_equal(g1.xf[l, :n + 2], g0.xf[l, :n + 2])
+        assert np.array_equal(g1.dx[l, :n + 2], g0.dx[l, :n + 2])
+    assert g1.dt == g0.dt and g1.h == g0.h
+    assert np.array_equal(p1.ravel(), p0)
+    c1 = assembly.classify(g1)
+    for k in ("nf", "nb", "mno", "row2cell", "sp_row"):
+        assert np.array_equal(c1[k], c[k]), k
+
+
+def decks_read(grid, data, fdata):
+    from clowns3d import decks
+    return decks.read_case(grid, data, fdata)
+
+
+@pytest.mark.parametrize("be", [False, True])
+def test_grid_writer_reproduces_sparse_grid(tmp_path, be):
+    """Read a .xyzn grid, write it back: the same bytes."""
+    from clowns3d import decks
+    case = make_case(tmp_path, 9, 6, 5, seed=41, sparse=True, be=be)
+    g, p, d = ours(case)
+    fa = decks.faces(case["grid"], d)
+    for l, n in enumerate((9, 6, 5)):
+        assert np.array_equal(fa[l, :n + 4], case["coords"][l]) and not fa[l, n + 4:].any()
+    out = tmp_path / "out.xyzn"
+    decks.write_grid(out, d, g.solid, fa, big_endian=be)
+    assert out.read_bytes() == case["grid"].read_bytes()
+
+
+def test_grid_writer_matches_reference(tmp_path):
+    """A dense grid read at t0 = 0 makes the reference write its .xyzn form
+    (data_io.F90:137-165, output_xyzn's format); ours from the same case is
+    byte-identical, and reads back as the same box."""
+    from clowns3d import decks
+    case = make_case(tmp_path, 8, 7, 4, seed=42, sparse=False, t0=0.0)
+    ref_read(case)
+    theirs = tmp_path / "case.data.xyzn"
+    assert theirs.exists()
+    g, p, d = ours(case)
+    mine = tmp_path / "mine.xyzn"
+    decks.write_grid(mine, d, g.solid, decks.faces(case["grid"], d), big_endian=False)
+    assert mine.read_bytes() == theirs.read_bytes()
+    d2 = decks.DeckDesc()
+    from clowns3d import _lib as L
+    L.check(L.lib().clowns_deck_open(str(mine).encode(), str(case["data"]).encode(), ctypes.byref(d2)), "open")
+    assert d2.sparse == 1 and d2.inne == d.inne
