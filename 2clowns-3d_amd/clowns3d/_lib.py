@@ -19,6 +19,7 @@ CLOWNS_EHIP = 4004
 CLOWNS_ENOMEM = 4005
 CLOWNS_ESTRUCT = 4006
 CLOWNS_EDECK = 4007
+CLOWNS_EVOF = 4008
 
 CLOWNS_MIXED = 1
 CLOWNS_WARM_START = 2
@@ -32,6 +33,7 @@ ERRORS = {
     CLOWNS_ENOMEM: "device allocation failed",
     CLOWNS_ESTRUCT: "operator pattern is not structurally symmetric",
     CLOWNS_EDECK: "unreadable case file, or cells outside the box model",
+    CLOWNS_EVOF: "VOF update: F above F_lim on a surface cell, or dt below DT_lim (the reference STOPs)",
 }
 
 

@@ -34,6 +34,11 @@ class PCInfo(ctypes.Structure):
                 ("ms", ctypes.c_double)]
 
 
+class VofInfo(ctypes.Structure):
+    _fields_ = [("retries", ctypes.c_int32), ("modified", ctypes.c_int32), ("slid", ctypes.c_int32),
+                ("rounds", ctypes.c_int32), ("dt", ctypes.c_double), ("ms", ctypes.c_double)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 L.SIGNATURES.update({
@@ -45,6 +50,8 @@ L.SIGNATURES.update({
                                                          ctypes.c_double, _I, ctypes.POINTER(PCInfo)]),
     "clowns_solver_classify": (ctypes.c_int, [_P, ctypes.POINTER(GridDesc)] + [_P] * 11 +
                                [_I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
+    "clowns_solver_vof": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), _P, _I, _P, _P, _P, _P,
+                                         ctypes.POINTER(VofInfo)]),
 })
 
 
@@ -216,6 +223,39 @@ class DeviceGrid:
         solver.lda = (self.N + 63) // 64 * 64
         solver.version += 1
         return p, {k: getattr(info, k) for k, _ in PCInfo._fields_ if k != "reserved"}
+
+    def vof(self, solver, Fp, mode=0, qs=None, qs_mask=None, dt=None, out_F=None):
+        """The VOF update cal_vof on the device (clowns_solver_vof), without
+        its closing classification: face fluxes, surface stage (mode 1:
+        quvwf_kaidan; mode 0: the PLIC face fluxes qs where qs_mask != 0,
+        both (3, ncell), or none), quvwf_modify, cal_new_f, simple_slide(0).
+        Reads the flags, F, fb, a, un and dx held here; Fp is F of the
+        previous step.  Returns (F_new, qf (3, ncell), info dict) with
+        info["dt"] the step after cal_new_f's reductions."""
+        torch = self._torch
+        nc = self.g.ncell
+        dev = self.device
+        t = lambda a, dtype: (a if torch.is_tensor(a) else torch.from_numpy(np.ascontiguousarray(a))).to(
+            dev, dtype).contiguous()
+        Fp = t(Fp, torch.float64).view(-1)
+        if Fp.numel() != nc:
+            raise ValueError("Fp must hold one value per box cell")
+        qf = torch.empty(3 * nc, dtype=torch.float64, device=dev)
+        Fn = out_F if out_F is not None else torch.empty(nc, dtype=torch.float64, device=dev)
+        qp = mp_ = None
+        if qs is not None or qs_mask is not None:
+            qs, qs_mask = t(qs, torch.float64).view(-1), t(qs_mask, torch.int8).view(-1)
+            if qs.numel() != 3 * nc or qs_mask.numel() != 3 * nc:
+                raise ValueError("qs and qs_mask must be (3, ncell)")
+            qp, mp_ = ctypes.c_void_p(qs.data_ptr()), ctypes.c_void_p(qs_mask.data_ptr())
+        g = GridDev.from_buffer_copy(self.dev)
+        if dt is not None:
+            g.dt = float(dt)
+        info = VofInfo()
+        L.check(L.lib().clowns_solver_vof(solver._h, ctypes.byref(g), ctypes.c_void_p(Fp.data_ptr()), int(mode),
+                                          qp, mp_, ctypes.c_void_p(qf.data_ptr()), ctypes.c_void_p(Fn.data_ptr()),
+                                          ctypes.byref(info)), "clowns_solver_vof")
+        return Fn, qf.view(3, nc), {k: getattr(info, k) for k, _ in VofInfo._fields_}
 
     def assemble_into(self, solver, parm=5.0, keep_raw=False, b=None):
         """Fused assembly + scaling + sort into the solver's resident operator.

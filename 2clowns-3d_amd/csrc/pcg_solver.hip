@@ -22,6 +22,7 @@
 #include "pcg_kernels.hpp"
 #include "assembly_kernels.hpp"
 #include "classify_kernels.hpp"
+#include "vof_kernels.hpp"
 
 using namespace clowns;
 
@@ -119,6 +120,9 @@ struct clowns_solver {
     bool will_rev_valid = false;  // will_rev holds the readers of the resident LL0
     DevBuf<int> cls_scratch;  // classification: extents, change flag, block counts
     DevBuf<double> pc_b, pc_x, pc_part;  // pressure-correction loop: RHS, xe, DMAX partials
+    DevBuf<int> vof_state;     // VOF update: quvwf_modify state per cell
+    DevBuf<double> vof_term;   // VOF update: quvwf_modify correction per cell
+    DevBuf<int> vof_cnt;       // VOF update: flags and counters (VC_*)
     DevBuf<Scal> sc;       // [0] inner/FP64 run, [1] outer (mixed)
     Scal *h_sc = nullptr;  // pinned mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
@@ -160,18 +164,22 @@ inline Geo geo_of(int N, int R) {
     return {G, R};
 }
 
-int ensure_common(clowns_solver *h, int N) {
-    int rc;
-    Geo g = geo_of(N, h->R);
-    if ((rc = h->part.ensure((size_t)2 * g.G))) return rc;
-    if ((rc = h->sc.ensure(2))) return rc;
-    if (!h->h_sc) HIPCHK(hipHostMalloc((void **)&h->h_sc, 2 * sizeof(Scal), hipHostMallocDefault));
+int ensure_events(clowns_solver *h) {
     if (!h->ev0) {
         HIPCHK(hipEventCreate(&h->ev0));
         HIPCHK(hipEventCreate(&h->ev1));
         HIPCHK(hipEventCreate(&h->ev2));
     }
     return CLOWNS_OK;
+}
+
+int ensure_common(clowns_solver *h, int N) {
+    int rc;
+    Geo g = geo_of(N, h->R);
+    if ((rc = h->part.ensure((size_t)2 * g.G))) return rc;
+    if ((rc = h->sc.ensure(2))) return rc;
+    if (!h->h_sc) HIPCHK(hipHostMalloc((void **)&h->h_sc, 2 * sizeof(Scal), hipHostMallocDefault));
+    return ensure_events(h);
 }
 
 template <typename T> void fill(hipStream_t s, T *p, size_t n, T val) {
@@ -1524,6 +1532,105 @@ extern "C" int clowns_solver_pressure_correction(clowns_solver *h, const clowns_
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
     out.passes = ik;
+    out.ms = ms;
+    if (info) *info = out;
+    return CLOWNS_OK;
+}
+
+namespace {
+bool vof_grid_ok(const clowns_grid_dev *g) {
+    return g && g->nx >= 1 && g->ny >= 1 && g->nz >= 1 && g->nf && g->nb && g->F && g->fb && g->a && g->un &&
+           g->dx && g->dt > 0.0 && g->stride >= std::max(g->nx, std::max(g->ny, g->nz)) + 2;
+}
+__global__ void k_vof_cnt_zero(int *__restrict__ cnt, int i0, int n) {
+    if ((int)threadIdx.x < n) cnt[i0 + threadIdx.x] = 0;
+}
+} // namespace
+
+extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp, int32_t mode,
+                                 const double *d_qs, const int8_t *d_qs_mask, double *d_qf, double *d_Fnew,
+                                 clowns_vof_info *info) {
+    if (!h || !vof_grid_ok(g) || !d_Fp || !d_qf || !d_Fnew || (mode != 0 && mode != 1)) return CLOWNS_EBADARG;
+    if ((d_qs == nullptr) != (d_qs_mask == nullptr)) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    const long nc = (long)(g->nx + 2) * (g->ny + 2) * (g->nz + 2);
+    if ((rc = h->vof_state.ensure(nc)) || (rc = h->vof_term.ensure(nc)) || (rc = h->vof_cnt.ensure(VC_N)) ||
+        (rc = ensure_events(h)))
+        return rc;
+    VofDev v;
+    v.nx = g->nx;
+    v.ny = g->ny;
+    v.nz = g->nz;
+    v.stride = g->stride;
+    v.ncell = nc;
+    v.nf = g->nf;
+    v.nb = g->nb;
+    v.F = g->F;
+    v.Fp = d_Fp;
+    v.fb = g->fb;
+    v.a = g->a;
+    v.un = g->un;
+    v.dx = g->dx;
+    v.qf = d_qf;
+    v.Fn = d_Fnew;
+    int *cnt = h->vof_cnt.p;
+    double dt = g->dt;
+    const int G = cdiv(nc, 256);
+    clowns_vof_info out{};
+    HIPCHK(hipEventRecord(h->ev0, s));
+    k_vof_cnt_zero<<<1, 64, 0, s>>>(cnt, 0, VC_N);
+    k_vof_check<<<G, 256, 0, s>>>(v, cnt);
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, cnt + VC_BAD, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (bad) return CLOWNS_EBADARG;
+    // cal_quvwf with the surface stage
+    if (mode == 1)
+        k_vof_flux<1><<<G, 256, 0, s>>>(v, dt, nullptr, nullptr);
+    else
+        k_vof_flux<0><<<G, 256, 0, s>>>(v, dt, d_qs, d_qs_mask);
+    // quvwf_modify: rounds until every surface cell is decided (each round
+    // decides at least the first pending cell of the serial order)
+    k_vof_state_init<<<G, 256, 0, s>>>(nc, h->vof_state.p);
+    for (int round = 1;; ++round) {
+        k_vof_cnt_zero<<<1, 64, 0, s>>>(cnt, VC_PENDING, 1);
+        k_vof_mod_eval<<<G, 256, 0, s>>>(v, dt, round, h->vof_state.p, h->vof_term.p, cnt);
+        int pending = 0;
+        HIPCHK(hipMemcpyAsync(&pending, cnt + VC_PENDING, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        out.rounds = round;
+        if (pending == 0) break;
+        if (round >= (1 << 29)) return CLOWNS_EHIP;  // unreachable: every round decides a cell
+    }
+    k_vof_mod_apply<<<G, 256, 0, s>>>(v, h->vof_state.p, h->vof_term.p);
+    // cal_new_f, with its retry at dt/1.1 (1sub-vof.F90:256-268)
+    for (;;) {
+        k_vof_cnt_zero<<<1, 64, 0, s>>>(cnt, VC_RETRY, 2);
+        k_vof_newf<<<G, 256, 0, s>>>(v, dt, cnt);
+        int fl[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(fl, cnt + VC_RETRY, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (fl[1]) return CLOWNS_EVOF;  // the reference STOPs (F > F_lim on a surface cell)
+        if (!fl[0]) break;
+        dt = dt / 1.1;
+        ++out.retries;
+        if (dt < VOF_DTLIM) return CLOWNS_EVOF;
+    }
+    // simple_slide(0) (SLIDE0 builds)
+    k_vof_slide<<<cdiv((long)g->nx * g->ny, 256), 256, 0, s>>>(v, dt, cnt);
+    HIPCHK(hipGetLastError());
+    int c8[VC_N];
+    HIPCHK(hipMemcpyAsync(c8, cnt, sizeof(c8), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(h->ev1, s));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    if (c8[VC_SOLID]) return CLOWNS_ENONFINITE;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    out.modified = c8[VC_MODIFIED];
+    out.slid = c8[VC_SLID];
+    out.dt = dt;
     out.ms = ms;
     if (info) *info = out;
     return CLOWNS_OK;

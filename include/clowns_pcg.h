@@ -40,6 +40,8 @@ extern "C" {
 #define CLOWNS_ENOMEM 4005      /* device allocation failed                  */
 #define CLOWNS_ESTRUCT 4006     /* transpose: pattern not structurally symmetric */
 #define CLOWNS_EDECK 4007       /* unreadable deck file, or cells outside the box model */
+#define CLOWNS_EVOF 4008        /* VOF update: the reference would STOP (F > F_lim on a
+                                   surface cell, or dt below DT_lim) */
 
 /* Replaces PCGPME (src2LOWNS/pcgpme_para.F90:1, argument declarations
  * :35-46).  AL(NA,NL), LL(NA,NL) column-major, LL 1-based (0 = none);
@@ -316,6 +318,35 @@ int clowns_deck_write_data(const char *path, const clowns_deck_desc *d,
                            const double *un, const double *F,
                            const double *p, double t, double dt,
                            int32_t big_endian);
+
+/* The VOF update cal_vof (src2LOWNS/1sub-vof.F90:10-38) on the device, up to
+ * its closing set_nflag (run clowns_solver_classify for that):
+ * cal_quvwf/QFF face fluxes, the surface stage, quvwf_modify (in the
+ * reference's serial order), cal_new_f with its dt retry and
+ * simple_slide(0) (SLIDE0 builds).
+ *   g        the classified grid: nf, nb, F (current), fb, a, un (the new
+ *            face velocities), dx, dt (nrows, mno, row2cell, sp_row, xf unused)
+ *   d_Fp     F of the previous step (fp)
+ *   mode 1   quvwf_kaidan (builds without NORMAL);
+ *   mode 0   NORMAL builds: qf = d_qs where d_qs_mask != 0 ([3][ncell]), the
+ *            PLIC face fluxes cal_idou2/a of quvwf_normal (:119-154); both
+ *            NULL for none
+ *   d_qf     out [3][ncell]: the face fluxes qf after the update
+ *   d_Fnew   out [ncell]: the new F (may be g->F)
+ * info->dt is dt after cal_new_f's reductions (dt/1.1 per retry; the
+ * reference also moves t by the same amount).  CLOWNS_EVOF where the
+ * reference STOPs.  Synchronous on return. */
+typedef struct clowns_vof_info {
+    int32_t retries;   /* cal_new_f dt reductions */
+    int32_t modified;  /* surface cells corrected by quvwf_modify */
+    int32_t slid;      /* cells that moved volume down in simple_slide */
+    int32_t rounds;    /* device rounds of the serial quvwf_modify walk */
+    double dt, ms;
+} clowns_vof_info;
+int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g,
+                      const double *d_Fp, int32_t mode, const double *d_qs,
+                      const int8_t *d_qs_mask, double *d_qf, double *d_Fnew,
+                      clowns_vof_info *info);
 
 /* Width of the neighbour indices the resident operator's solve kernels read:
  * 16 when the sorted LL was coded in 16 bits (every offset fits; always for

@@ -1,6 +1,7 @@
 /* oracle.h -- TEST INFRASTRUCTURE ONLY: declarations of the C oracle. */
 #ifndef CLOWNS_ORACLE_H
 #define CLOWNS_ORACLE_H
+#include <stdint.h>
 void oracle_set_threads(int n);
 int oracle_get_threads(void);
 void oracle_scale_sort(double *AL, int NL, int NA, int *LL, double *D, int N,
@@ -19,4 +20,8 @@ int oracle_pcgpme_will(const double *AL, int NL, int NA, const int *LL,
                        double PARM, int ITER, int *IER, int *K_out,
                        double *X_raw, double *X_out);
 int oracle_set_dx(double *x, int m, double *dx);
+int oracle_vof(int nx, int ny, int nz, int stride, const int8_t *solid, const int8_t *nf, const int8_t *nb,
+               const double *F, const double *Fp, const double *fb, const double *a, const double *un,
+               const double *dx, double *dt, int mode, const double *qs, const int8_t *qs_mask, double *qf,
+               double *Fn, int *retries, int *counts);
 #endif
