@@ -52,6 +52,8 @@ L.SIGNATURES.update({
                                [_I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
     "clowns_solver_vof": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), _P, _I, _P, _P, _P, _P,
                                          ctypes.POINTER(VofInfo)]),
+    "clowns_solver_vof_time_kernel": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), _P, _I, _I, _P, _P,
+                                                     ctypes.POINTER(ctypes.c_float)]),
 })
 
 
@@ -256,6 +258,26 @@ class DeviceGrid:
                                           qp, mp_, ctypes.c_void_p(qf.data_ptr()), ctypes.c_void_p(Fn.data_ptr()),
                                           ctypes.byref(info)), "clowns_solver_vof")
         return Fn, qf.view(3, nc), {k: getattr(info, k) for k, _ in VofInfo._fields_}
+
+    def vof_time_kernel(self, solver, Fp, kernel, reps=20, dt=None):
+        """Average ms of one launch of a VOF-update kernel
+        (clowns_solver_vof_time_kernel): 0 face fluxes, 1 new F, 2 slide,
+        3 copy of F."""
+        torch = self._torch
+        nc = self.g.ncell
+        Fp = (Fp if torch.is_tensor(Fp) else torch.from_numpy(np.ascontiguousarray(Fp))).to(
+            self.device, torch.float64).contiguous()
+        qf = torch.empty(3 * nc, dtype=torch.float64, device=self.device)
+        Fn = torch.empty(nc, dtype=torch.float64, device=self.device)
+        g = GridDev.from_buffer_copy(self.dev)
+        if dt is not None:
+            g.dt = float(dt)
+        ms = ctypes.c_float(0.0)
+        L.check(L.lib().clowns_solver_vof_time_kernel(solver._h, ctypes.byref(g), ctypes.c_void_p(Fp.data_ptr()),
+                                                      int(kernel), int(reps), ctypes.c_void_p(qf.data_ptr()),
+                                                      ctypes.c_void_p(Fn.data_ptr()), ctypes.byref(ms)),
+                "clowns_solver_vof_time_kernel")
+        return ms.value
 
     def assemble_into(self, solver, parm=5.0, keep_raw=False, b=None):
         """Fused assembly + scaling + sort into the solver's resident operator.

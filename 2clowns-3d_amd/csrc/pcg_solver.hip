@@ -1545,13 +1545,13 @@ bool vof_grid_ok(const clowns_grid_dev *g) {
 __global__ void k_vof_cnt_zero(int *__restrict__ cnt, int i0, int n) {
     if ((int)threadIdx.x < n) cnt[i0 + threadIdx.x] = 0;
 }
-} // namespace
 
-extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp, int32_t mode,
-                                 const double *d_qs, const int8_t *d_qs_mask, double *d_qf, double *d_Fnew,
-                                 clowns_vof_info *info) {
-    if (!h || !vof_grid_ok(g) || !d_Fp || !d_qf || !d_Fnew || (mode != 0 && mode != 1)) return CLOWNS_EBADARG;
-    if ((d_qs == nullptr) != (d_qs_mask == nullptr)) return CLOWNS_EBADARG;
+// Buffers, the device view and the flag check shared by the VOF entries;
+// on success the counters are zeroed and every flag the kernels rely on
+// for their neighbour reads has been verified on the device.
+int vof_setup(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp, double *d_qf, double *d_Fnew,
+              VofDev &v) {
+    if (!h || !vof_grid_ok(g) || !d_Fp || !d_qf || !d_Fnew) return CLOWNS_EBADARG;
     int rc = set_device(h);
     if (rc) return rc;
     hipStream_t s = h->stream;
@@ -1559,7 +1559,6 @@ extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, con
     if ((rc = h->vof_state.ensure(nc)) || (rc = h->vof_term.ensure(nc)) || (rc = h->vof_cnt.ensure(VC_N)) ||
         (rc = ensure_events(h)))
         return rc;
-    VofDev v;
     v.nx = g->nx;
     v.ny = g->ny;
     v.nz = g->nz;
@@ -1575,22 +1574,37 @@ extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, con
     v.dx = g->dx;
     v.qf = d_qf;
     v.Fn = d_Fnew;
+    v.D = h->vof_term.p;  // quvwf_modify's terms are applied before k_vof_newf writes D
     int *cnt = h->vof_cnt.p;
-    double dt = g->dt;
-    const int G = cdiv(nc, 256);
-    clowns_vof_info out{};
-    HIPCHK(hipEventRecord(h->ev0, s));
     k_vof_cnt_zero<<<1, 64, 0, s>>>(cnt, 0, VC_N);
-    k_vof_check<<<G, 256, 0, s>>>(v, cnt);
+    k_vof_check<<<cdiv(nc, 256), 256, 0, s>>>(v, cnt);
     int bad = 0;
     HIPCHK(hipMemcpyAsync(&bad, cnt + VC_BAD, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (bad) return CLOWNS_EBADARG;
+    return bad ? CLOWNS_EBADARG : CLOWNS_OK;
+}
+} // namespace
+
+extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp, int32_t mode,
+                                 const double *d_qs, const int8_t *d_qs_mask, double *d_qf, double *d_Fnew,
+                                 clowns_vof_info *info) {
+    if (mode != 0 && mode != 1) return CLOWNS_EBADARG;
+    if ((d_qs == nullptr) != (d_qs_mask == nullptr)) return CLOWNS_EBADARG;
+    VofDev v;
+    int rc = vof_setup(h, g, d_Fp, d_qf, d_Fnew, v);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    const long nc = v.ncell;
+    int *cnt = h->vof_cnt.p;
+    double dt = g->dt;
+    const int G = cdiv(nc, 256), G8 = (G + 7) & ~7, Gc8 = (cdiv((long)g->nx * g->ny, 256) + 7) & ~7;
+    clowns_vof_info out{};
+    HIPCHK(hipEventRecord(h->ev0, s));
     // cal_quvwf with the surface stage
     if (mode == 1)
-        k_vof_flux<1><<<G, 256, 0, s>>>(v, dt, nullptr, nullptr);
+        k_vof_flux<1><<<G8, 256, 0, s>>>(v, dt, nullptr, nullptr);
     else
-        k_vof_flux<0><<<G, 256, 0, s>>>(v, dt, d_qs, d_qs_mask);
+        k_vof_flux<0><<<G8, 256, 0, s>>>(v, dt, d_qs, d_qs_mask);
     // quvwf_modify: rounds until every surface cell is decided (each round
     // decides at least the first pending cell of the serial order)
     k_vof_state_init<<<G, 256, 0, s>>>(nc, h->vof_state.p);
@@ -1608,7 +1622,7 @@ extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, con
     // cal_new_f, with its retry at dt/1.1 (1sub-vof.F90:256-268)
     for (;;) {
         k_vof_cnt_zero<<<1, 64, 0, s>>>(cnt, VC_RETRY, 2);
-        k_vof_newf<<<G, 256, 0, s>>>(v, dt, cnt);
+        k_vof_newf<<<G8, 256, 0, s>>>(v, dt, cnt);
         int fl[2] = {0, 0};
         HIPCHK(hipMemcpyAsync(fl, cnt + VC_RETRY, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -1619,7 +1633,7 @@ extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, con
         if (dt < VOF_DTLIM) return CLOWNS_EVOF;
     }
     // simple_slide(0) (SLIDE0 builds)
-    k_vof_slide<<<cdiv((long)g->nx * g->ny, 256), 256, 0, s>>>(v, dt, cnt);
+    k_vof_slide<<<Gc8, 256, 0, s>>>(v, dt, cnt);
     HIPCHK(hipGetLastError());
     int c8[VC_N];
     HIPCHK(hipMemcpyAsync(c8, cnt, sizeof(c8), hipMemcpyDeviceToHost, s));
@@ -1633,6 +1647,34 @@ extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, con
     out.dt = dt;
     out.ms = ms;
     if (info) *info = out;
+    return CLOWNS_OK;
+}
+
+extern "C" int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp,
+                                             int32_t kernel, int32_t reps, double *d_qf, double *d_Fnew,
+                                             float *ms_per_launch) {
+    if (kernel < 0 || kernel > 3 || reps < 1 || !ms_per_launch) return CLOWNS_EBADARG;
+    VofDev v;
+    int rc = vof_setup(h, g, d_Fp, d_qf, d_Fnew, v);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    const int G8 = (cdiv(v.ncell, 256) + 7) & ~7, Gc8 = (cdiv((long)v.nx * v.ny, 256) + 7) & ~7;
+    auto launch = [&]() {
+        switch (kernel) {
+        case 0: k_vof_flux<1><<<G8, 256, 0, s>>>(v, g->dt, nullptr, nullptr); break;
+        case 1: k_vof_newf<<<G8, 256, 0, s>>>(v, g->dt, h->vof_cnt.p); break;
+        case 2: k_vof_slide<<<Gc8, 256, 0, s>>>(v, g->dt, h->vof_cnt.p); break;
+        default: k_copy<double><<<cdiv(v.ncell, BS), BS, 0, s>>>((int)v.ncell, g->F, d_Fnew); break;
+        }
+    };
+    launch();  // warm-up
+    HIPCHK(hipEventRecord(h->ev0, s));
+    for (int r = 0; r < reps; ++r) launch();
+    HIPCHK(hipEventRecord(h->ev1, s));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    *ms_per_launch = ms / reps;
     return CLOWNS_OK;
 }
 

@@ -44,7 +44,17 @@ struct VofDev {
     const int8_t *nf, *nb;
     const double *F, *Fp, *fb, *a, *un, *dx;
     double *qf, *Fn;
+    double *D;  // df_vof of every open cell after quvwf_modify (k_vof_newf -> k_vof_slide)
 };
+
+// Launches of k_vof_flux / k_vof_newf / k_vof_slide have a multiple of 8
+// blocks; the hardware deals blocks to the 8 XCDs round robin, so block b
+// takes the logical index (b mod 8) * G/8 + b/8: each XCD streams one
+// contiguous range, and the +-y neighbours (one block away) sit in its own L2.
+__device__ __forceinline__ int vof_block() {
+    const int b = blockIdx.x;
+    return (b & 7) * (gridDim.x >> 3) + (b >> 3);
+}
 
 // counters: [0] retry (cal_new_f's icon = 30), [1] stop (F > F_lim on a
 // surface cell), [2] correction aimed at a solid cell, [3] pending cells of
@@ -95,7 +105,7 @@ __device__ inline double vof_sign(double a, double b) { return signbit(b) ? -fab
 template <int MODE>
 __global__ void __launch_bounds__(256)
 k_vof_flux(VofDev v, double dt, const double *__restrict__ qs, const int8_t *__restrict__ qmask) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long c = (long)vof_block() * blockDim.x + threadIdx.x;
     if (c >= v.ncell) return;
     const long st[3] = {1, (long)(v.nx + 2), (long)(v.nx + 2) * (v.ny + 2)};
     const size_t n = (size_t)v.ncell;
@@ -232,7 +242,7 @@ k_vof_mod_apply(VofDev v, const int *__restrict__ state, const double *__restric
 }
 
 __global__ void __launch_bounds__(256) k_vof_newf(VofDev v, double dt, int *__restrict__ cnt) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long c = (long)vof_block() * blockDim.x + threadIdx.x;
     if (c >= v.ncell) return;
     const int f = v.nf[c];
     if (f == -1) {
@@ -242,6 +252,7 @@ __global__ void __launch_bounds__(256) k_vof_newf(VofDev v, double dt, int *__re
     int p[3];
     vof_pos(v, c, p);
     double d = vof_df(v, c, p, dt);
+    v.D[c] = d;
     if (-d * dt * v.fb[c] > 0.4) {
         atomicOr(&cnt[VC_RETRY], 1);
     } else if (fabs(d * dt) < VOF_TINY && d != 0.0) {
@@ -251,37 +262,55 @@ __global__ void __launch_bounds__(256) k_vof_newf(VofDev v, double dt, int *__re
     v.Fn[c] = d;
 }
 
+// simple_slide(0): one thread per column, k downwards.  cal_fslide_m0
+// recomputes df_vof of the cell and of the one below before using them;
+// those equal D (k_vof_newf's values on the same fluxes) unless a move in
+// this column changed a face since: the cell's high face is changed by a
+// move one level up (then its df_vof is the value that move left for the
+// cell below), the lower cell's faces only by moves not yet made.
 __global__ void __launch_bounds__(256) k_vof_slide(VofDev v, double dt, int *__restrict__ cnt) {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = vof_block() * blockDim.x + threadIdx.x;
     if (col >= v.nx * v.ny) return;
     const int i = 1 + col % v.nx, j = 1 + col / v.nx;
     const long X = v.nx + 2, Y = v.ny + 2, sz = X * Y;
     const size_t S = (size_t)v.stride;
     const double dx0 = v.dx[i], dx1 = v.dx[S + j];
     int moved = 0;
+    bool changed_above = false;  // qf(2, c + e_z) moved by the previous level
+    double carried = 0.0;        // df_vof(c) after that move
     for (int k = v.nz; k >= 1; --k) {
         const long c = i + X * (j + Y * k), m = c - sz;
         const int f = v.nf[c];
         if (f == -1) break;
-        if (f == 1) continue;
-        if (v.nf[m] == -1) continue;
+        if (f == 1 || v.nf[m] == -1) {
+            changed_above = false;
+            continue;
+        }
         const double vmesh = dx0 * dx1 * v.dx[2 * S + k];
         const double dvc = vmesh * v.fb[c];
         const double dvm = dx0 * dx1 * v.dx[2 * S + k - 1] * v.fb[m];
-        if (dvc * dvm == 0.0) continue;
-        const int pc[3] = {i, j, k}, pm[3] = {i, j, k - 1};
-        const double fc = vof_df(v, c, pc, dt);
-        const double fm = vof_df(v, m, pm, dt);
+        if (dvc * dvm == 0.0) {
+            changed_above = false;
+            continue;
+        }
+        const double fc = changed_above ? carried : v.D[c];
+        const double fm = v.D[m];
         v.Fn[c] = fc;
         v.Fn[m] = fm;
         const double t1 = dvm * (1.0 - fm), t2 = dvc * fc;
         const double f2 = t1 < t2 ? t1 : t2;
-        if (fabs(f2) == 0.0) continue;
+        if (fabs(f2) == 0.0) {
+            changed_above = false;
+            continue;
+        }
         const size_t o = 2 * (size_t)v.ncell + c;
         v.qf[o] = v.qf[o] + -1.0 * f2 / vmesh / dt * v.dx[2 * S + k] / v.a[o];
+        const int pc[3] = {i, j, k}, pm[3] = {i, j, k - 1};
         const double ft = vof_df(v, c, pc, dt);
         v.Fn[c] = fabs(ft) < VOF_TINY ? 0.0 : ft;
-        v.Fn[m] = vof_df(v, m, pm, dt);
+        carried = vof_df(v, m, pm, dt);
+        v.Fn[m] = carried;
+        changed_above = true;
         ++moved;
     }
     if (moved) atomicAdd(&cnt[VC_SLID], moved);

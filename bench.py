@@ -22,6 +22,12 @@ The matrix is constant in time (disp_param builds it once), so it stays
 resident; a step is one solve_will on a new right-hand side already in HBM
 (solve + the reference's output pass).  The drop-in (host arrays in and out,
 matrix upload included) is timed beside it.
+
+`--vof 2|3` measures the VOF update cal_vof (src2LOWNS/1sub-vof.F90:10-38,
+up to its closing set_nflag) on configuration 2 or 3: face fluxes with the
+quvwf_kaidan surface stage, quvwf_modify, cal_new_f, simple_slide(0).  The
+classified grid, the face velocities and F of the previous step are
+resident; a step is one update (clowns_solver_vof).
 """
 import argparse
 import ctypes
@@ -66,6 +72,8 @@ def parse():
                     help="only assemble and run the timed kernel loops (for rocprofv3 --pmc)")
     ap.add_argument("--will", default=None, metavar="NXxNY",
                     help="measure the 2DH dispersion solve PCGPME_WILL on an NX x NY grid instead")
+    ap.add_argument("--vof", type=int, default=None, choices=[2, 3],
+                    help="measure the VOF update cal_vof on configuration 2 or 3 instead")
     return ap.parse_args()
 
 
@@ -207,6 +215,118 @@ def bench_will(args):
     print(json.dumps(res))
 
 
+# Algorithmic bytes per box cell of the two streaming VOF kernels (arrays read
+# once, written once; neighbour values come from cache).  Open cells:
+#   face fluxes  un 24 + F 8 + nf 1 + nb 1 + a_x,a_y 16 (kaidan) + qf 24 = 74
+#   new F        qf 24 + a 24 + fb 8 + Fp 8 + nf 1 + F_new 8 = 73
+# solid cells: flux nf 1 + qf 24 = 25; new F nf 1 + F 8 + F_new 8 = 17.
+VOF_BYTES = {"flux": (74, 25), "newf": (73, 17)}
+
+
+def vof_cpu_baseline(g, A, Fp, dt, threads):
+    """The reference's own VOF routines (flang build in oracle/_ref, after its
+    set_nflag at one thread, untimed) at `threads` OpenMP threads; the C
+    oracle (serial) if that was not built."""
+    import numpy as np
+    nc = g.ncell
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref_vof.so")
+    times = []
+    for _ in range(2):
+        qf, fo, fc = np.zeros(3 * nc), np.zeros(nc), np.zeros(nc)
+        nf, nb = np.zeros(nc, np.int8), np.zeros(nc, np.int8)
+        dtv, secs = ctypes.c_double(dt), ctypes.c_double(0.0)
+        if os.path.exists(ref):
+            lib = ctypes.CDLL(ref, mode=os.RTLD_LAZY)
+            kind = "reference"
+            qs, mask = np.zeros(3 * nc), np.zeros(3 * nc, np.int8)
+            with _quiet_stdout():
+                lib.ref_vof(g.nx, g.ny, g.nz, P(A["solid"]), P(A["F"]), P(Fp), P(A["fb"]), P(A["a"]), P(A["un"]),
+                            P(A["dx"]), P(A["xf"]), g.stride, ctypes.byref(dtv), 1, P(qs), P(mask), P(nf), P(nb),
+                            P(fc), P(qf), P(fo), threads, ctypes.byref(secs))
+            times.append(secs.value)
+        else:
+            from clowns3d import assembly
+            c = assembly.classify(g)
+            orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+            kind, threads = "port", 1
+            retries = ctypes.c_int(0)
+            t0 = time.perf_counter()
+            orc.oracle_vof(g.nx, g.ny, g.nz, g.stride, P(A["solid"]), P(c["nf"]), P(c["nb"]), P(c["F"]), P(Fp),
+                           P(A["fb"]), P(A["a"]), P(A["un"]), P(A["dx"]), ctypes.byref(dtv), 1, None, None, P(qf),
+                           P(fo), ctypes.byref(retries), None)
+            times.append(time.perf_counter() - t0)
+    return {"value": min(times), "unit": "s/step", "cores": threads, "kind": kind,
+            "sample": (f"the same grid ({nc} cells) on the host; "
+                       f"{'flang build of cal_quvwf, quvwf_kaidan, quvwf_modify, cal_new_f, simple_slide' if kind == 'reference' else 'C oracle of cal_vof'}"
+                       f" timed around the update at {threads} OpenMP threads, best of 2")}
+
+
+def bench_vof(args):
+    import numpy as np
+    import torch
+    from clowns3d import DeviceGrid, Solver, assembly, configs
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    g = configs.CONFIGS[args.vof]()
+    c = assembly.classify(g)
+    A = c["arrays"]
+    dg = DeviceGrid(g, classified=c)
+    sol = Solver(0)
+    Fp_h = c["F"].copy()
+    Fp = torch.from_numpy(Fp_h).to(dev)
+    out = torch.empty(g.ncell, dtype=torch.float64, device=dev)
+    nc = g.ncell
+    n_open = int((A["solid"] == 0).sum())
+    if args.profile_kernels:
+        dg.vof(sol, Fp, mode=1, out_F=out)
+        res = {k: dg.vof_time_kernel(sol, Fp, i, 10) * 1e3 for k, i in
+               (("k_vof_flux<1>", 0), ("k_vof_newf", 1), ("k_vof_slide", 2), ("k_copy<double>", 3))}
+        print(json.dumps({"cells": nc, "avg_us": res}))
+        return
+    for _ in range(args.warmup):
+        dg.vof(sol, Fp, mode=1, out_F=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = [dg.vof(sol, Fp, mode=1, out_F=out)[2] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    dt_step = (time.perf_counter() - t0) / args.steps
+    info = infos[-1]
+    kt = {k: dg.vof_time_kernel(sol, Fp, i, 50) for k, i in (("flux", 0), ("newf", 1), ("slide", 2), ("copy", 3))}
+    kname, key = ("k_vof_flux<1>", "flux") if kt["flux"] >= kt["newf"] else ("k_vof_newf", "newf")
+    bo, bs = VOF_BYTES[key]
+    kbytes = n_open * bo + (nc - n_open) * bs
+    achieved = kbytes / (kt[key] * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"traffic_vof_config{args.vof}.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get(kname)
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = vof_cpu_baseline(g, A, Fp_h, g.dt, threads)
+    res = {
+        "metric": "VOF update (cal_vof) s/step; HBM GB/s",
+        "value": dt_step, "unit": "s/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt_step * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"configuration {args.vof}: {g.name}, {DESCR[args.vof]}; VOF update with the "
+                               "quvwf_kaidan surface stage, classified grid resident",
+                   "cells": nc, "open_cells": n_open, "modified": info["modified"], "slid": info["slid"],
+                   "rounds": info["rounds"], "retries": info["retries"], "update_ms_hip_events": info["ms"],
+                   "kernel_us": {k: v * 1e3 for k, v in kt.items()}},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "kernel": kname,
+                     "alg_bytes_per_launch": kbytes, "avg_us": kt[key] * 1e3,
+                     "copy_GBps": 16 * nc / (kt["copy"] * 1e-3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        res["speedup_vs_cpu"] = cpu["value"] / dt_step
+    print(json.dumps(res))
+
+
 class _quiet_stdout:
     """Route file descriptor 1 to /dev/null around a call into the flang
     build of the reference, which writes progress lines to unit 6 (flushed
@@ -321,6 +441,9 @@ def main():
         sys.exit(2)
     if args.will:
         bench_will(args)
+        return
+    if args.vof:
+        bench_vof(args)
         return
     import numpy as np
     import torch

@@ -348,6 +348,16 @@ int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g,
                       const int8_t *d_qs_mask, double *d_qf, double *d_Fnew,
                       clowns_vof_info *info);
 
+/* Average duration of one launch of a VOF-update kernel over `reps`
+ * launches (HIP events on the solver's stream), on the inputs of
+ * clowns_solver_vof: kernel 0 the face fluxes (kaidan surface stage),
+ * 1 cal_new_f's pass, 2 simple_slide, 3 a plain copy of F into d_Fnew
+ * (the copy rate of the same arrays).  Writes d_qf / d_Fnew. */
+int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid_dev *g,
+                                  const double *d_Fp, int32_t kernel,
+                                  int32_t reps, double *d_qf, double *d_Fnew,
+                                  float *ms_per_launch);
+
 /* Width of the neighbour indices the resident operator's solve kernels read:
  * 16 when the sorted LL was coded in 16 bits (every offset fits; always for
  * box grids of up to 16383 cells a line), else 32; negative CLOWNS_E* on

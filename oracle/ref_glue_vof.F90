@@ -9,22 +9,23 @@
 !   of quvwf_normal (mode 0: qf(l,nn) = qs where qs_mask /= 0),
 !   quvwf_modify, cal_new_f, simple_slide(0).
 ! Outputs: the flags and F after set_nflag (the VOF update's inputs), then
-! qf, F and dt after the update, in the box layout.
+! qf, F and dt after the update, in the box layout, and the wall time of
+! the update (cal_quvwf .. simple_slide) run at nthreads OpenMP threads.
 subroutine ref_vof(nx, ny, nz, solid, fvv, fpv, fbv, av, unv, dxv, xfv, stride, dt_io, mode, qs, qs_mask, &
-                   nf_out, nb_out, f_cls, qf_out, f_out) bind(C, name="ref_vof")
+                   nf_out, nb_out, f_cls, qf_out, f_out, nthreads, secs) bind(C, name="ref_vof")
     use iso_c_binding
-    use omp_lib, only: omp_set_num_threads
+    use omp_lib, only: omp_set_num_threads, omp_get_wtime
     use variables, only: is, js, ks, ie, je, ke, inns, inne, inne2, dt, NTHR, icon, inn2d, n, t, m_alloc
     use arrays, only: ls, le, lf, dim3d, dx, x, mn, nff, in, jn, kn, u, un, f, fp, p, ox, sfn, &
                       fln, a, fb, a0, fb0, sp, spn, nor, objno, qf, in2d, alloc_uvwfp, alloc_afb, &
                       alloc_a0fb0, alloc_norsp, alloc_quvwf
     implicit none
-    integer(c_int), value :: nx, ny, nz, stride, mode
+    integer(c_int), value :: nx, ny, nz, stride, mode, nthreads
     integer(c_int8_t), intent(in) :: solid(0:*), qs_mask(0:*)
     real(c_double), intent(in) :: fvv(0:*), fpv(0:*), fbv(0:*), av(0:*), unv(0:*), dxv(0:*), xfv(0:*), qs(0:*)
     real(c_double), intent(inout) :: dt_io
     integer(c_int8_t), intent(out) :: nf_out(0:*), nb_out(0:*)
-    real(c_double), intent(out) :: f_cls(0:*), qf_out(0:*), f_out(0:*)
+    real(c_double), intent(out) :: f_cls(0:*), qf_out(0:*), f_out(0:*), secs
     interface
         subroutine set_nflag()
         end subroutine
@@ -41,6 +42,7 @@ subroutine ref_vof(nx, ny, nz, solid, fvv, fpv, fbv, av, unv, dxv, xfv, stride, 
         end subroutine
     end interface
     integer :: XX, YY, ncell, i, j, k, c, nn, l, mx2
+    real(c_double) :: t0
 
     XX = nx + 2 ; YY = ny + 2 ; ncell = XX * YY * (nz + 2)
     NTHR = 1
@@ -132,6 +134,9 @@ subroutine ref_vof(nx, ny, nz, solid, fvv, fpv, fbv, av, unv, dxv, xfv, stride, 
     dt = dt_io
     call alloc_quvwf(inne2)
     qf = 0d0
+    ! the update itself at nthreads OpenMP threads (set_nflag above ran at one)
+    call omp_set_num_threads(max(1, nthreads))
+    t0 = omp_get_wtime()
     call cal_quvwf
     if (mode == 1) then
         call quvwf_kaidan
@@ -146,6 +151,8 @@ subroutine ref_vof(nx, ny, nz, solid, fvv, fpv, fbv, av, unv, dxv, xfv, stride, 
     call quvwf_modify
     call cal_new_f
     call simple_slide(0)
+    secs = omp_get_wtime() - t0
+    call omp_set_num_threads(1)
 
     dt_io = dt
     qf_out(0:3*ncell-1) = 0d0

@@ -84,11 +84,12 @@ def _ref_child(args):
     qf = np.zeros(3 * nc)
     fo = np.zeros(nc)
     dtv = ctypes.c_double(dt)
+    secs = ctypes.c_double(0.0)
     qs = np.zeros(3 * nc) if qs is None else np.ascontiguousarray(qs).ravel()
     mask = np.zeros(3 * nc, np.int8) if mask is None else np.ascontiguousarray(mask).ravel()
     lib.ref_vof(nx, ny, nz, ptr(A["solid"]), ptr(F), ptr(Fp), ptr(A["fb"]), ptr(A["a"]), ptr(A["un"]),
                 ptr(A["dx"]), ptr(A["xf"]), stride, ctypes.byref(dtv), mode, ptr(qs), ptr(mask), ptr(nf), ptr(nb),
-                ptr(fcls), ptr(qf), ptr(fo))
+                ptr(fcls), ptr(qf), ptr(fo), 1, ctypes.byref(secs))
     return dict(nf=nf, nb=nb, F_cls=fcls, qf=qf.reshape(3, nc), F=fo, dt=dtv.value)
 
 
