@@ -328,6 +328,42 @@ __device__ __forceinline__ typename V2<T>::t ld2(const T *p) {
     return *reinterpret_cast<const typename V2<T>::t *>(p);
 }
 
+template <typename T> struct NV2;
+template <> struct NV2<float> { typedef float t __attribute__((ext_vector_type(2))); };
+template <> struct NV2<double> { typedef double t __attribute__((ext_vector_type(2))); };
+template <> struct NV2<int> { typedef int t __attribute__((ext_vector_type(2))); };
+template <> struct NV2<short> { typedef short t __attribute__((ext_vector_type(2))); };
+// Two consecutive elements of an array the kernel streams once (operator
+// values, index codes, the dot vector, outputs).  ST: non-temporal loads and
+// stores, which keep those streams from displacing the gathered vector in
+// L2 -- faster once the operator no longer fits the 256 MB last-level cache,
+// slower while it does (the non-temporal accesses are not kept there
+// either); the solver times both per operator (tune_stream).
+template <bool ST, typename T>
+__device__ __forceinline__ typename NV2<T>::t ldn2(const T *p) {
+    const typename NV2<T>::t *q = reinterpret_cast<const typename NV2<T>::t *>(p);
+    if (ST) return __builtin_nontemporal_load(q);
+    return *q;
+}
+template <bool ST, typename T>
+__device__ __forceinline__ typename V2<T>::t lds2(const T *p) {
+    const typename NV2<T>::t v = ldn2<ST>(p);
+    typename V2<T>::t o;
+    o.x = v.x;
+    o.y = v.y;
+    return o;
+}
+template <bool ST, typename T>
+__device__ __forceinline__ void sts2(T *p, typename V2<T>::t o) {
+    typename NV2<T>::t v;
+    v.x = o.x;
+    v.y = o.y;
+    if (ST)
+        __builtin_nontemporal_store(v, reinterpret_cast<typename NV2<T>::t *>(p));
+    else
+        *reinterpret_cast<typename NV2<T>::t *>(p) = v;
+}
+
 // Neighbour index sources for the two-row kernels, 0-based, ghosts (the
 // reference's N+1+mod(I,64), pcgpme_para.F90:80) at the zero rows >= N.
 // Ix32: the sorted LL itself.  Ix16: the same indices coded in 16 bits per
@@ -341,8 +377,8 @@ __device__ __forceinline__ typename V2<T>::t ld2(const T *p) {
 struct Ix32 {
     const int *L;
     __device__ __forceinline__ Ix32 col(size_t off) const { return {L + off}; }
-    __device__ __forceinline__ int2 pair(int i) const {
-        const int2 j = *reinterpret_cast<const int2 *>(L + i);
+    template <bool ST = false> __device__ __forceinline__ int2 pair(int i) const {
+        const NV2<int>::t j = ldn2<ST>(L + i);
         return make_int2(j.x - 1, j.y - 1);
     }
     __device__ __forceinline__ int one(int i) const { return L[i] - 1; }
@@ -370,9 +406,9 @@ struct Ix16 {
         if (v == LL16_WIDE) j = L32[i] - 1;
         return min(max(j, 0), N);
     }
-    __device__ __forceinline__ int2 pair(int i) const {
+    template <bool ST = false> __device__ __forceinline__ int2 pair(int i) const {
         const int2 b = base[i / LL16_ROWS];
-        const short2 v = *reinterpret_cast<const short2 *>(L + i);
+        const NV2<short>::t v = ldn2<ST>(L + i);
         return make_int2(dec(v.x, i, b), dec(v.y, i + 1, b));
     }
     __device__ __forceinline__ int one(int i) const { return dec(L[i], i, base[i / LL16_ROWS]); }
@@ -445,7 +481,7 @@ k_ll16_build(int N, int lda, int NL, const int *__restrict__ LL, short *__restri
     if (threadIdx.x == 0) base[blockIdx.x] = make_int2(bp, bn);
 }
 
-template <typename T, int NT, int DOT, typename IX>
+template <typename T, int NT, int DOT, typename IX, bool ST>
 __global__ void __launch_bounds__(BS)
 k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
         const T *__restrict__ a1, IX l1, const T *__restrict__ in,
@@ -455,23 +491,23 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
     const int i = (lb * BS + threadIdx.x) * 2;
     double s0 = 0.0, s1 = 0.0;
     if (i + 1 < N) {
-        const auto c0 = ld2(a0 + i);
-        const int2 j0 = l0.pair(i);
+        const auto c0 = lds2<ST>(a0 + i);
+        const int2 j0 = l0.template pair<ST>(i);
         const auto xi = ld2(in + i);
         T o0 = xi.x - c0.x * in[j0.x];
         T o1 = xi.y - c0.y * in[j0.y];
         if (NT == 2) {
-            const auto c1 = ld2(a1 + i);
-            const int2 j1 = l1.pair(i);
+            const auto c1 = lds2<ST>(a1 + i);
+            const int2 j1 = l1.template pair<ST>(i);
             o0 = o0 - c1.x * in[j1.x];
             o1 = o1 - c1.y * in[j1.y];
         }
         typename V2<T>::t o;
         o.x = o0;
         o.y = o1;
-        *reinterpret_cast<typename V2<T>::t *>(out + i) = o;
+        sts2<ST>(out + i, o);
         if (DOT == 1 || DOT == 2) {
-            const auto wi = ld2(w + i);
+            const auto wi = lds2<ST>(w + i);
             s0 += (double)wi.x * (double)o0;
             s0 += (double)wi.y * (double)o1;
         }
@@ -502,7 +538,7 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
     }
 }
 
-template <typename T, typename IX>
+template <typename T, typename IX, bool ST>
 __global__ void __launch_bounds__(BS)
 k_pm2_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
          const T *__restrict__ a1, IX l1, const T *__restrict__ in, T *__restrict__ out) {
@@ -516,22 +552,22 @@ k_pm2_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
         return o1i - c1 * o1m;
     };
     if (i + 1 < N) {
-        const auto c0 = ld2(a0 + i);
-        const int2 j0 = l0.pair(i);
-        const auto c1 = ld2(a1 + i);
-        const int2 j1 = l1.pair(i);
+        const auto c0 = lds2<ST>(a0 + i);
+        const int2 j0 = l0.template pair<ST>(i);
+        const auto c1 = lds2<ST>(a1 + i);
+        const int2 j1 = l1.template pair<ST>(i);
         const auto xi = ld2(in + i);
         typename V2<T>::t o;
         o.x = one(c0.x, j0.x, c1.x, j1.x, xi.x);
         o.y = one(c0.y, j0.y, c1.y, j1.y, xi.y);
-        *reinterpret_cast<typename V2<T>::t *>(out + i) = o;
+        sts2<ST>(out + i, o);
     } else if (i < N) {
         out[i] = one(a0[i], l0.one(i), a1[i], l1.one(i), in[i]);
     }
 }
 
 // SpMV with two consecutive rows per thread (same per-row arithmetic).
-template <typename T, int NL, int MODE, typename IX>
+template <typename T, int NL, int MODE, typename IX, bool ST>
 __global__ void __launch_bounds__(BS)
 k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL, IX LL,
           const T *__restrict__ D, const T *__restrict__ v0, const T *__restrict__ v1,
@@ -555,10 +591,10 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
         int2 l[NL];
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
-            l[k] = LL.col((size_t)k * lda).pair(i);
-            a[k] = ld2(AL + (size_t)k * lda + i);
+            l[k] = LL.col((size_t)k * lda).template pair<ST>(i);
+            a[k] = lds2<ST>(AL + (size_t)k * lda + i);
         }
-        const auto d = ld2(D + i);
+        const auto d = lds2<ST>(D + i);
         T x0, x1;
         if (MODE == 2) {
             const auto r = ld2(v1 + i), q = ld2(v2 + i);
@@ -593,7 +629,7 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
             o.x = acc0;
             o.y = acc1;
         }
-        *reinterpret_cast<typename V2<T>::t *>(y + i) = o;
+        sts2<ST>(y + i, o);
     } else if (i < N) {
         const T xi = X(i);
         T acc = xi * D[i];

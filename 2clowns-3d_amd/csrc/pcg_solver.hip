@@ -146,6 +146,8 @@ struct clowns_solver {
     int will_max_rounds = 4096;  // PCGPME_WILL output pass: device rounds before the host pass
     int fuse64 = -1, fuse32 = -1;      // pairwise PM fusion chosen by tune_fuse ...
     int fuse64_N = 0, fuse32_N = 0;    // ... for operators of this many rows
+    int st64 = -1, st32 = -1;          // non-temporal streams chosen by tune_stream (bit 0 PM, bit 1 SpMV) ...
+    int st64_N = 0, st32_N = 0;        // ... for operators of this many rows
 };
 
 namespace {
@@ -194,6 +196,7 @@ template <typename T> struct Op {
     const int *LL;
     const T *D;
     bool fuse = false;  // PM stages fused pairwise (k_pm2) in apply_pm
+    bool st_pm = false, st_spmv = false;  // non-temporal streams (tune_stream)
     const short *LL16 = nullptr;  // 16-bit coded LL (two-row kernels), or null
     const int2 *base16 = nullptr;
 };
@@ -212,6 +215,13 @@ template <typename T> Ix16 ix16(const Op<T> &A, size_t off) { return Ix16{A.LL16
 // the first time it solves at a given size (tune_fuse) and keeps the faster.
 // CLOWNS_FUSE_PM=0/1 forces both precisions off/on.
 int h_fuse_pm = -1;
+// Non-temporal loads/stores of the streamed arrays in the PM stages and the
+// SpMV (the ST kernels).  Measured on MI355X, at 13.1M rows FP64 SpMV 224 ->
+// 196 us and PM stage 56 -> 51 us, but FP32 PM stage 28 -> 35 us; at 2.45M
+// rows (operator inside the 256 MB last-level cache) every kernel slower.
+// tune_stream times both per operator and precision; CLOWNS_STREAM = 0/1
+// forces them off/on for solvers created from then on (-1: timed again).
+int h_stream = -1;
 // CLOWNS_VERBOSE = 1: report per-operator tuning decisions on stderr.
 bool h_verbose = false;
 // Two rows per thread in the PM stages (k_pm_v2, k_pm2_v2): 8/16-byte
@@ -234,13 +244,20 @@ bool h_ll16 = true;
 template <typename T>
 void launch_pm2(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j, const T *in, T *out) {
     const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
-    if (use_v2<T>() && has16(A))
-        k_pm2_v2<T, Ix16><<<geo_of(A.N, 2).G, BS, 0, s>>>(sc, A.N, A.AL + o0, ix16(A, o0), A.AL + o1,
-                                                          ix16(A, o1), in, out);
-    else if (use_v2<T>())
-        k_pm2_v2<T, Ix32><<<geo_of(A.N, 2).G, BS, 0, s>>>(sc, A.N, A.AL + o0, ix32(A, o0), A.AL + o1,
-                                                          ix32(A, o1), in, out);
-    else
+    const int G2 = geo_of(A.N, 2).G;
+    if (use_v2<T>() && has16(A)) {
+        if (A.st_pm)
+            k_pm2_v2<T, Ix16, true><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix16(A, o0), A.AL + o1, ix16(A, o1), in, out);
+        else
+            k_pm2_v2<T, Ix16, false><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix16(A, o0), A.AL + o1, ix16(A, o1), in,
+                                                       out);
+    } else if (use_v2<T>()) {
+        if (A.st_pm)
+            k_pm2_v2<T, Ix32, true><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix32(A, o0), A.AL + o1, ix32(A, o1), in, out);
+        else
+            k_pm2_v2<T, Ix32, false><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix32(A, o0), A.AL + o1, ix32(A, o1), in,
+                                                       out);
+    } else
         k_pm2<T><<<g.G, BS, 0, s>>>(sc, A.N, g.R, A.AL + o0, A.LL + o0, A.AL + o1, A.LL + o1, in, out);
 }
 
@@ -263,6 +280,17 @@ void launch_spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
     }
 }
 
+template <typename T, int NT, typename IX, bool ST>
+void launch_pm_v2(hipStream_t s, int G2, const Scal *sc, int N, const T *a0, IX x0, const T *a1, IX x1,
+                  const T *in, T *out, int dot, const T *w, double *part) {
+    switch (dot) {
+    case 0: k_pm_v2<T, NT, 0, IX, ST><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    case 1: k_pm_v2<T, NT, 1, IX, ST><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    case 2: k_pm_v2<T, NT, 2, IX, ST><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    default: k_pm_v2<T, NT, 3, IX, ST><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    }
+}
+
 template <typename T, int NT>
 int launch_pm_stage(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j,
                     const T *in, T *out, int dot, const T *w, double *part) {
@@ -273,21 +301,17 @@ int launch_pm_stage(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j,
         const int G2 = geo_of(A.N, 2).G;
         if (has16(A)) {
             const Ix16 x0 = ix16(A, o0), x1 = NT == 2 ? ix16(A, o1) : x0;
-            switch (dot) {
-            case 0: k_pm_v2<T, NT, 0, Ix16><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out, w, part); break;
-            case 1: k_pm_v2<T, NT, 1, Ix16><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out, w, part); break;
-            case 2: k_pm_v2<T, NT, 2, Ix16><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out, w, part); break;
-            default: k_pm_v2<T, NT, 3, Ix16><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out, w, part); break;
-            }
+            if (A.st_pm)
+                launch_pm_v2<T, NT, Ix16, true>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
+            else
+                launch_pm_v2<T, NT, Ix16, false>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
             return G2;
         }
         const Ix32 x0 = ix32(A, o0), x1 = NT == 2 ? ix32(A, o1) : x0;
-        switch (dot) {
-        case 0: k_pm_v2<T, NT, 0, Ix32><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out, w, part); break;
-        case 1: k_pm_v2<T, NT, 1, Ix32><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out, w, part); break;
-        case 2: k_pm_v2<T, NT, 2, Ix32><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out, w, part); break;
-        default: k_pm_v2<T, NT, 3, Ix32><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out, w, part); break;
-        }
+        if (A.st_pm)
+            launch_pm_v2<T, NT, Ix32, true>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
+        else
+            launch_pm_v2<T, NT, Ix32, false>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
         return G2;
     }
     switch (dot) {
@@ -380,16 +404,25 @@ template <typename T> bool tune_fuse(clowns_solver *h, const Op<T> &A, VecSet<T>
     return dec == 1;
 }
 
+template <typename T, int NL, typename IX, bool ST>
+void launch_spmv_v2_st(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int mode, const T *v0,
+                       const T *v1, const T *v2, T *vout, T *y) {
+    const int G2 = geo_of(A.N, 2).G;
+    switch (mode) {
+    case 0: k_spmv_v2<T, NL, 0, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y); break;
+    case 1: k_spmv_v2<T, NL, 1, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y); break;
+    case 2: k_spmv_v2<T, NL, 2, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y); break;
+    default: k_spmv_v2<T, NL, 3, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y); break;
+    }
+}
+
 template <typename T, int NL, typename IX>
 void launch_spmv_v2(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int mode, const T *v0,
                     const T *v1, const T *v2, T *vout, T *y) {
-    const int G2 = geo_of(A.N, 2).G;
-    switch (mode) {
-    case 0: k_spmv_v2<T, NL, 0, IX><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y); break;
-    case 1: k_spmv_v2<T, NL, 1, IX><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y); break;
-    case 2: k_spmv_v2<T, NL, 2, IX><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y); break;
-    default: k_spmv_v2<T, NL, 3, IX><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y); break;
-    }
+    if (A.st_spmv)
+        launch_spmv_v2_st<T, NL, IX, true>(s, sc, A, ix, mode, v0, v1, v2, vout, y);
+    else
+        launch_spmv_v2_st<T, NL, IX, false>(s, sc, A, ix, mode, v0, v1, v2, vout, y);
 }
 
 template <typename T, int NL>
@@ -415,6 +448,92 @@ void spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
         launch_spmv<T, 6>(s, g, sc, A, mode, v0, v1, v2, vout, y);
     else
         launch_spmv<T, 4>(s, g, sc, A, mode, v0, v1, v2, vout, y);
+}
+
+// Chooses the non-temporal (ST) or cached form of the PM application and of
+// the SpMV for this operator, once per size and precision (A.fuse already
+// decided).  What the forms leave in the last-level cache decides between
+// them, so each combination runs a steady-state block of its own: two
+// untimed, then three timed sequences of two iterations' kernels, the four
+// combinations in turn, the whole twice; the fastest mean wins.
+// (Interleaving single applications of different forms favours the
+// streamed form, which finds data the cached form left behind.)
+template <typename T> void tune_stream(clowns_solver *h, Op<T> &A, VecSet<T> &V) {
+    if (h_stream >= 0) {
+        A.st_pm = A.st_spmv = h_stream == 1;
+        return;
+    }
+    int &dec = sizeof(T) == 4 ? h->st32 : h->st64;
+    int &decN = sizeof(T) == 4 ? h->st32_N : h->st64_N;
+    if (dec < 0 || decN != A.N) {
+        hipStream_t s = h->stream;
+        Scal *sc = h->sc.p;
+        const Geo g = geo_of(A.N, h->R);
+        k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1, 0);
+        // two Bi-CGSTAB iterations' kernels in run_bicgstab's order, without
+        // the one-block finalizations (the vectors are scratch here: the
+        // solve initializes every one of them after tuning)
+        const int G2 = geo_of(A.N, 2).G;
+        double *part = h->part.p;
+        auto seq = [&](int form) {
+            Op<T> B = A;
+            B.st_pm = (form & 1) != 0;
+            B.st_spmv = (form & 2) != 0;
+            for (int it = 0; it < 2; ++it) {
+                if (use_v2_vec<T>()) k_pupd_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.p0.p, V.r.p, V.q.p);
+                spmv<T>(s, g, sc, B, 0, V.p0.p, nullptr, nullptr, nullptr, V.y.p);
+                apply_pm<T>(s, g, sc, B, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
+                if (use_v2_vec<T>()) k_eupd_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.r.p, V.q.p, V.e.p);
+                spmv<T>(s, g, sc, B, 0, V.e.p, nullptr, nullptr, nullptr, V.y.p);
+                apply_pm<T>(s, g, sc, B, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
+                if (use_v2_vec<T>())
+                    k_update_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
+            }
+        };
+        float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+        bool ok = true;
+        for (int pass = 0; pass < 2 && ok; ++pass)
+            for (int f = 0; f < 4 && ok; ++f) {
+                seq(f);
+                seq(f);
+                float ms = 0.f;
+                ok = hipEventRecord(h->ev0, s) == hipSuccess;
+                if (!ok) break;
+                for (int r = 0; r < 3; ++r) seq(f);
+                ok = hipEventRecord(h->ev1, s) == hipSuccess && hipEventSynchronize(h->ev1) == hipSuccess &&
+                     hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess;
+                best[f] = std::min(best[f], ms / 3.f);
+            }
+        if (!ok) return;
+        int f_best = 0;
+        for (int f = 1; f < 4; ++f)
+            if (best[f] < best[f_best]) f_best = f;
+        dec = f_best;
+        decN = A.N;
+        if (h_verbose)
+            fprintf(stderr,
+                    "clowns_pcg: N=%d %s two iterations: cached %.1f, PM streamed %.1f, SpMV streamed "
+                    "%.1f, both streamed %.1f us -> PM %s, SpMV %s\n",
+                    A.N, sizeof(T) == 4 ? "FP32" : "FP64", 1e3f * best[0], 1e3f * best[1], 1e3f * best[2],
+                    1e3f * best[3], (dec & 1) ? "streamed" : "cached", (dec & 2) ? "streamed" : "cached");
+    }
+    A.st_pm = (dec & 1) != 0;
+    A.st_spmv = (dec & 2) != 0;
+}
+
+// The decisions already taken for this operator (time_kernel reports the
+// forms the solver runs).
+template <typename T> void known_stream(const clowns_solver *h, Op<T> &A) {
+    if (h_stream >= 0) {
+        A.st_pm = A.st_spmv = h_stream == 1;
+        return;
+    }
+    const int dec = sizeof(T) == 4 ? h->st32 : h->st64;
+    const int decN = sizeof(T) == 4 ? h->st32_N : h->st64_N;
+    if (dec >= 0 && decN == A.N) {
+        A.st_pm = (dec & 1) != 0;
+        A.st_spmv = (dec & 2) != 0;
+    }
 }
 
 // The PCGPME iteration (pcgpme_para.F90:148-247) on already initialised
@@ -620,6 +739,7 @@ int solve_fp64(clowns_solver *h, const Op<double> &A_in, const double *Wsc, cons
     if ((rc = ensure_vecs(h, V, A_in.N)) || (rc = h->bs.ensure(n_pad(A_in.N)))) return rc;
     Op<double> A = A_in;
     A.fuse = tune_fuse(h, A, V);
+    tune_stream(h, A, V);
     const int N = A.N;
     const Geo g = geo_of(N, h->R);
     Scal *sc = h->sc.p;
@@ -679,6 +799,8 @@ int solve_mixed(clowns_solver *h, const Op<double> &A_in, const Op<float> &A32_i
     Op<float> A32 = A32_in;
     A.fuse = tune_fuse(h, A, V);
     A32.fuse = tune_fuse(h, A32, F);
+    tune_stream(h, A, V);
+    tune_stream(h, A32, F);
     const int N = A.N;
     const Geo g = geo_of(N, h->R);
     Scal *sc = h->sc.p, *outer = h->sc.p + 1;
@@ -809,6 +931,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_CLASSIFY_HOST")) h->cls_force_host = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_WILL_MAX_ROUNDS")) h->will_max_rounds = std::max(1, atoi(e));
     if (const char *e = getenv("CLOWNS_FUSE_PM")) h_fuse_pm = atoi(e) != 0 ? 1 : 0;
+    if (const char *e = getenv("CLOWNS_STREAM")) h_stream = atoi(e) < 0 ? -1 : (atoi(e) != 0 ? 1 : 0);
     if (const char *e = getenv("CLOWNS_V2_32")) h_v2_32 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2VEC")) h_v2_vec = atoi(e) != 0;
@@ -1186,6 +1309,8 @@ extern "C" int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32
     const Geo g = geo_of(N, h->R);
     Op<double> A = op64(h);
     Op<float> A32 = op32(h);
+    known_stream(h, A);
+    known_stream(h, A32);
     auto one = [&]() {
         switch (kernel) {
         case 0: spmv<double>(s, g, sc, A, 0, h->v64.p0.p, nullptr, nullptr, nullptr, h->v64.y.p); break;
@@ -1675,6 +1800,16 @@ extern "C" int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     *ms_per_launch = ms / reps;
+    return CLOWNS_OK;
+}
+
+extern "C" int clowns_solver_tuning(clowns_solver *h, int32_t out[4]) {
+    if (!h || !out || h->N < 1) return CLOWNS_EBADARG;
+    const int N = h->N;
+    out[0] = h_fuse_pm >= 0 ? h_fuse_pm : (h->fuse64_N == N ? h->fuse64 : -1);
+    out[1] = h_fuse_pm >= 0 ? h_fuse_pm : (h->fuse32_N == N ? h->fuse32 : -1);
+    out[2] = h_stream >= 0 ? 3 * h_stream : (h->st64_N == N ? h->st64 : -1);
+    out[3] = h_stream >= 0 ? 3 * h_stream : (h->st32_N == N ? h->st32 : -1);
     return CLOWNS_OK;
 }
 

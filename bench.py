@@ -150,9 +150,12 @@ def bench_will(args):
     sol = Solver(0)
     sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
     if args.profile_kernels:
+        sol.solve_will(t(s["B"]), eps=args.eps, iters=1000)
         ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
+        st = sol.tuning()["stream64"]
+        sp, sm = ("true" if st > 0 and st & 2 else "false"), ("true" if st > 0 and st & 1 else "false")
         res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
-               ((f"k_spmv_v2<double,4,0,{ix}>", 0), (f"k_pm_v2<double,1,0,{ix}>", 1), ("k_copy<double>", 4))}
+               ((f"k_spmv_v2<double,4,0,{ix},{sp}>", 0), (f"k_pm_v2<double,1,0,{ix},{sm}>", 1), ("k_copy<double>", 4))}
         print(json.dumps({"rows": N, "avg_us": res}))
         return
     rng = np.random.default_rng(7)
@@ -178,10 +181,12 @@ def bench_will(args):
     t_pm = sol.time_kernel(1, 50)
     t_copy = sol.time_kernel(4, 50)
     ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
+    st = sol.tuning()["stream64"]
+    sp, sm = ("true" if st > 0 and st & 2 else "false"), ("true" if st > 0 and st & 1 else "false")
     if 2 * t_spmv >= 8 * t_pm:
-        kname, kms, kbytes = f"k_spmv_v2<double,4,0,{ix}>", t_spmv, 56 * N
+        kname, kms, kbytes = f"k_spmv_v2<double,4,0,{ix},{sp}>", t_spmv, 56 * N
     else:
-        kname, kms, kbytes = f"k_pm_v2<double,1,0,{ix}>", t_pm, ALG_BYTES["pm_stage"] * N
+        kname, kms, kbytes = f"k_pm_v2<double,1,0,{ix},{sm}>", t_pm, ALG_BYTES["pm_stage"] * N
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = None
     pmc = os.path.join(ROOT, "profiles", f"traffic_will{nx}x{ny}.json")
@@ -460,13 +465,20 @@ def main():
     x = torch.zeros(N, dtype=torch.float64, device=dev)
 
     if args.profile_kernels:
+        # the kernels in the forms the solver picks for this operator
         dg.assemble_into(sol, b=b)
+        sol.solve(b, x, eps=args.eps)
+        sol.solve(b, x, eps=args.eps, mixed=True)
         ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
+        tune = sol.tuning()
+        st = {p: ("true" if tune["stream" + p] > 0 and tune["stream" + p] & 1 else "false",
+                  "true" if tune["stream" + p] > 0 and tune["stream" + p] & 2 else "false") for p in ("64", "32")}
         res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
-               ((f"k_spmv_v2<double,6,0,{ix}>", 0), (f"k_pm_v2<double,1,0,{ix}>", 1), ("k_copy<double>", 4),
-                (f"k_pm2_v2<double,{ix}>", 5), (f"k_spmv_v2<float,6,0,{ix}>", 2), (f"k_pm_v2<float,1,0,{ix}>", 3),
-                (f"k_pm2_v2<float,{ix}>", 6))}
-        print(json.dumps({"rows": N, "avg_us": res}))
+               ((f"k_spmv_v2<double,6,0,{ix},{st['64'][1]}>", 0), (f"k_pm_v2<double,1,0,{ix},{st['64'][0]}>", 1),
+                ("k_copy<double>", 4), (f"k_pm2_v2<double,{ix},{st['64'][0]}>", 5),
+                (f"k_spmv_v2<float,6,0,{ix},{st['32'][1]}>", 2), (f"k_pm_v2<float,1,0,{ix},{st['32'][0]}>", 3),
+                (f"k_pm2_v2<float,{ix},{st['32'][0]}>", 6))}
+        print(json.dumps({"rows": N, "avg_us": res, "tuning": tune}))
         return
 
     def step(mixed):
@@ -521,11 +533,14 @@ def main():
     t_pair = sol.time_kernel(5, 50)
     t_copy = sol.time_kernel(4, 50)
     ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
-    cands = [(2 * t_spmv, f"k_spmv_v2<double,6,0,{ix}>", t_spmv, ALG_BYTES["spmv"] * N)]
-    if t_pair < 2 * t_pm:
-        cands.append((4 * t_pair, f"k_pm2_v2<double,{ix}>", t_pair, ALG_BYTES["pm_pair"] * N))
+    tune = sol.tuning()
+    st_pm = "true" if tune["stream64"] > 0 and tune["stream64"] & 1 else "false"
+    st_sp = "true" if tune["stream64"] > 0 and tune["stream64"] & 2 else "false"
+    cands = [(2 * t_spmv, f"k_spmv_v2<double,6,0,{ix},{st_sp}>", t_spmv, ALG_BYTES["spmv"] * N)]
+    if tune["fuse64"] == 1:
+        cands.append((4 * t_pair, f"k_pm2_v2<double,{ix},{st_pm}>", t_pair, ALG_BYTES["pm_pair"] * N))
     else:
-        cands.append((8 * t_pm, f"k_pm_v2<double,1,0,{ix}>", t_pm, ALG_BYTES["pm_stage"] * N))
+        cands.append((8 * t_pm, f"k_pm_v2<double,1,0,{ix},{st_pm}>", t_pm, ALG_BYTES["pm_stage"] * N))
     _, kname, kms, kbytes = max(cands)
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = None

@@ -358,6 +358,13 @@ int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid_dev *g,
                                   int32_t reps, double *d_qf, double *d_Fnew,
                                   float *ms_per_launch);
 
+/* The per-operator kernel forms the solver chose for the resident operator
+ * when it first solved at this size: out[0], out[1] pairwise-fused PM
+ * stages (FP64, FP32: 1 fused, 0 single); out[2], out[3] streamed
+ * (non-temporal) arrays (FP64, FP32: bit 0 the PM stages, bit 1 the SpMV).
+ * -1 where not yet decided.  No counterpart in the reference (reporting). */
+int clowns_solver_tuning(clowns_solver *h, int32_t out[4]);
+
 /* Width of the neighbour indices the resident operator's solve kernels read:
  * 16 when the sorted LL was coded in 16 bits (every offset fits; always for
  * box grids of up to 16383 cells a line), else 32; negative CLOWNS_E* on

@@ -1,0 +1,52 @@
+"""GPU: the non-temporal (streamed) forms of the PM stages and the SpMV
+compute exactly what the cached forms do -- FP64, mixed and transposed
+solves bit-identical with the forms forced on and off -- and the solver
+reports the forms it chose."""
+import os
+
+import numpy as np
+import pytest
+
+from _systems import box_system
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(stream):
+    from clowns3d import Solver
+    os.environ["CLOWNS_STREAM"] = str(stream)
+    try:
+        return Solver(0)  # reads CLOWNS_STREAM at creation
+    finally:
+        del os.environ["CLOWNS_STREAM"]
+
+
+def _solve_all(s, stream):
+    import torch
+    sol = _solver(stream)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
+    b = t(s["B"])
+    x64, i64 = sol.solve(b, eps=1e-10)
+    x32, i32 = sol.solve(b, eps=1e-10, mixed=True)
+    xt, it = sol.solve_transpose(b, eps=1e-10)
+    tune = sol.tuning()
+    out = [x.cpu().numpy() for x in (x64, x32, xt)], (i64, i32, it), tune
+    sol.close()
+    return out
+
+
+@pytest.mark.parametrize("shape", [(160, 128, 6), (33, 17, 9)])
+def test_streamed_forms_bit_identical(shape):
+    s = box_system(*shape, seed=23)
+    on, info_on, tune_on = _solve_all(s, 1)
+    off, info_off, tune_off = _solve_all(s, 0)
+    auto, _, tune_auto = _solve_all(s, -1)  # back to timing per operator
+    assert tune_on["stream64"] == 3 and tune_on["stream32"] == 3
+    assert tune_off["stream64"] == 0 and tune_off["stream32"] == 0
+    assert tune_auto["stream64"] in (0, 1, 2, 3) and tune_auto["fuse64"] in (0, 1)
+    for a, b, c in zip(on, off, auto):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    for a, b in zip(info_on, info_off):
+        assert a["iters"] == b["iters"] and a["ier"] == b["ier"] == 0
