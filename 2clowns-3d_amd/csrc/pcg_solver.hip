@@ -145,9 +145,9 @@ struct clowns_solver {
     int poll = 8;          // iterations between host polls of `done`
     int will_max_rounds = 4096;  // PCGPME_WILL output pass: device rounds before the host pass
     int fuse64 = -1, fuse32 = -1;      // pairwise PM fusion chosen by tune_fuse ...
-    int fuse64_N = 0, fuse32_N = 0;    // ... for operators of this many rows
+    int fuse64_N = 0, fuse32_N = 0;    // ... for operators of about this many rows (tuned_size)
     int st64 = -1, st32 = -1;          // non-temporal streams chosen by tune_stream (bit 0 PM, bit 1 SpMV) ...
-    int st64_N = 0, st32_N = 0;        // ... for operators of this many rows
+    int st64_N = 0, st32_N = 0;        // ... for operators of about this many rows (tuned_size)
 };
 
 namespace {
@@ -359,11 +359,16 @@ int apply_pm(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, const T *in,
 // for operators of this size.  NL = 4 operators time stages 1-2 only (their
 // apply fuses one pair).  V's y, t, e are scratch here; the run's scalars are
 // reset (kernels skip work once `done` is set).
+// A tuning decision holds for operators within 1/16 of the size it was
+// timed at: a time-stepped run whose row count moves by a few rows per step
+// (the free surface) does not time again.
+inline bool tuned_size(int N, int decN) { return decN > 0 && std::abs(N - decN) <= decN / 16; }
+
 template <typename T> bool tune_fuse(clowns_solver *h, const Op<T> &A, VecSet<T> &V) {
     if (h_fuse_pm >= 0) return h_fuse_pm == 1;
     int &dec = sizeof(T) == 4 ? h->fuse32 : h->fuse64;
     int &decN = sizeof(T) == 4 ? h->fuse32_N : h->fuse64_N;
-    if (dec >= 0 && decN == A.N) return dec == 1;
+    if (dec >= 0 && tuned_size(A.N, decN)) return dec == 1;
     hipStream_t s = h->stream;
     Scal *sc = h->sc.p;
     const Geo g = geo_of(A.N, h->R);
@@ -465,7 +470,7 @@ template <typename T> void tune_stream(clowns_solver *h, Op<T> &A, VecSet<T> &V)
     }
     int &dec = sizeof(T) == 4 ? h->st32 : h->st64;
     int &decN = sizeof(T) == 4 ? h->st32_N : h->st64_N;
-    if (dec < 0 || decN != A.N) {
+    if (dec < 0 || !tuned_size(A.N, decN)) {
         hipStream_t s = h->stream;
         Scal *sc = h->sc.p;
         const Geo g = geo_of(A.N, h->R);
@@ -530,7 +535,7 @@ template <typename T> void known_stream(const clowns_solver *h, Op<T> &A) {
     }
     const int dec = sizeof(T) == 4 ? h->st32 : h->st64;
     const int decN = sizeof(T) == 4 ? h->st32_N : h->st64_N;
-    if (dec >= 0 && decN == A.N) {
+    if (dec >= 0 && tuned_size(A.N, decN)) {
         A.st_pm = (dec & 1) != 0;
         A.st_spmv = (dec & 2) != 0;
     }
@@ -1806,10 +1811,10 @@ extern "C" int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid
 extern "C" int clowns_solver_tuning(clowns_solver *h, int32_t out[4]) {
     if (!h || !out || h->N < 1) return CLOWNS_EBADARG;
     const int N = h->N;
-    out[0] = h_fuse_pm >= 0 ? h_fuse_pm : (h->fuse64_N == N ? h->fuse64 : -1);
-    out[1] = h_fuse_pm >= 0 ? h_fuse_pm : (h->fuse32_N == N ? h->fuse32 : -1);
-    out[2] = h_stream >= 0 ? 3 * h_stream : (h->st64_N == N ? h->st64 : -1);
-    out[3] = h_stream >= 0 ? 3 * h_stream : (h->st32_N == N ? h->st32 : -1);
+    out[0] = h_fuse_pm >= 0 ? h_fuse_pm : (tuned_size(N, h->fuse64_N) ? h->fuse64 : -1);
+    out[1] = h_fuse_pm >= 0 ? h_fuse_pm : (tuned_size(N, h->fuse32_N) ? h->fuse32 : -1);
+    out[2] = h_stream >= 0 ? 3 * h_stream : (tuned_size(N, h->st64_N) ? h->st64 : -1);
+    out[3] = h_stream >= 0 ? 3 * h_stream : (tuned_size(N, h->st32_N) ? h->st32 : -1);
     return CLOWNS_OK;
 }
 
