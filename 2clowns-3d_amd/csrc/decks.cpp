@@ -27,6 +27,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -44,7 +45,13 @@ uint64_t bswap64(uint64_t v) {
 // One Fortran unformatted sequential file, read record by record.
 class RecordReader {
   public:
-    explicit RecordReader(const char *path) : f_(path ? std::fopen(path, "rb") : nullptr) {}
+    explicit RecordReader(const char *path) : f_(path ? std::fopen(path, "rb") : nullptr) {
+        if (f_) {
+            std::fseek(f_, 0, SEEK_END);
+            size_ = std::ftell(f_);
+            std::fseek(f_, 0, SEEK_SET);
+        }
+    }
     ~RecordReader() {
         if (f_) std::fclose(f_);
     }
@@ -53,17 +60,15 @@ class RecordReader {
     // length marker (the one of the two readings that fits the file).
     bool next() {
         uint32_t m;
+        const long pos = std::ftell(f_);
         if (std::fread(&m, 4, 1, f_) != 1) return false;
         if (!order_set_) {
-            long pos = std::ftell(f_);
-            std::fseek(f_, 0, SEEK_END);
-            const long size = std::ftell(f_);
-            std::fseek(f_, pos, SEEK_SET);
             const uint32_t sw = bswap32(m);
-            swap_ = !((long)m + 8 <= size) && ((long)sw + 8 <= size);
+            swap_ = !((long)m + 8 <= size_) && ((long)sw + 8 <= size_);
             order_set_ = true;
         }
         const uint32_t n = swap_ ? bswap32(m) : m;
+        if (pos < 0 || (long)n + 8 > size_ - pos) return false;  // a length the file cannot hold
         rec_.resize(n);
         if (n && std::fread(rec_.data(), 1, n, f_) != n) return false;
         uint32_t t;
@@ -95,10 +100,17 @@ class RecordReader {
 
   private:
     FILE *f_;
+    long size_ = 0;
     std::vector<char> rec_;
     size_t pos_ = 0;
     bool swap_ = false, order_set_ = false;
 };
+
+// The descriptor a read or write relies on for the caller's array sizes.
+bool desc_ok(const clowns_deck_desc *d) {
+    return d->nx >= 1 && d->ny >= 1 && d->nz >= 1 && d->ie - d->is == d->nx && d->je - d->js == d->ny &&
+           d->ke - d->ks == d->nz && d->stride >= std::max(d->nx, std::max(d->ny, d->nz)) + 2;
+}
 
 class RecordWriter {
   public:
@@ -207,7 +219,7 @@ int read_flags(const clowns_deck_desc *d, RecordReader &grid, RecordReader &data
 
 } // namespace
 
-extern "C" int clowns_deck_open(const char *grid_path, const char *data_path, clowns_deck_desc *d) {
+static int clowns_deck_open_impl(const char *grid_path, const char *data_path, clowns_deck_desc *d) {
     if (!grid_path || !data_path || !d) return CLOWNS_EBADARG;
     std::memset(d, 0, sizeof(*d));
     RecordReader grid(grid_path);
@@ -243,11 +255,12 @@ extern "C" int clowns_deck_open(const char *grid_path, const char *data_path, cl
     return CLOWNS_OK;
 }
 
-extern "C" int clowns_deck_read(const char *grid_path, const char *data_path, const char *fdata_path,
+static int clowns_deck_read_impl(const char *grid_path, const char *data_path, const char *fdata_path,
                                 clowns_deck_desc *d, int8_t *solid, double *F, double *un, double *p,
                                 double *a, double *fb, double *x, double *dx) {
     if (!grid_path || !data_path || !d || !solid || !F || !un || !p || !a || !fb || !x || !dx)
         return CLOWNS_EBADARG;
+    if (!desc_ok(d)) return CLOWNS_EBADARG;
     const long X = d->nx + 2, Y = d->ny + 2, Z = d->nz + 2, nc = X * Y * Z;
     const int S = d->stride;
     RecordReader grid(grid_path), data(data_path);
@@ -353,8 +366,9 @@ extern "C" int clowns_deck_read(const char *grid_path, const char *data_path, co
     return CLOWNS_OK;
 }
 
-extern "C" int clowns_deck_faces(const char *grid_path, const clowns_deck_desc *d, double *faces) {
+static int clowns_deck_faces_impl(const char *grid_path, const clowns_deck_desc *d, double *faces) {
     if (!grid_path || !d || !faces) return CLOWNS_EBADARG;
+    if (!desc_ok(d)) return CLOWNS_EBADARG;
     const int S2 = d->stride + 2;
     RecordReader grid(grid_path);
     if (!grid.ok() || !grid.next()) return CLOWNS_EDECK;
@@ -370,9 +384,10 @@ extern "C" int clowns_deck_faces(const char *grid_path, const clowns_deck_desc *
     return CLOWNS_OK;
 }
 
-extern "C" int clowns_deck_write_grid(const char *path, const clowns_deck_desc *d, const int8_t *solid,
+static int clowns_deck_write_grid_impl(const char *path, const clowns_deck_desc *d, const int8_t *solid,
                                       const double *faces, int32_t big_endian) {
     if (!path || !d || !solid || !faces) return CLOWNS_EBADARG;
+    if (!desc_ok(d)) return CLOWNS_EBADARG;
     const long X = d->nx + 2, Y = d->ny + 2, Z = d->nz + 2;
     const int S2 = d->stride + 2;
     RecordWriter w(path, big_endian != 0);
@@ -405,10 +420,11 @@ extern "C" int clowns_deck_write_grid(const char *path, const clowns_deck_desc *
     return w.ok() ? CLOWNS_OK : CLOWNS_EDECK;
 }
 
-extern "C" int clowns_deck_write_data(const char *path, const clowns_deck_desc *d, const int8_t *nf,
+static int clowns_deck_write_data_impl(const char *path, const clowns_deck_desc *d, const int8_t *nf,
                                       const int8_t *nb, const double *un, const double *F, const double *p,
                                       double t, double dt, int32_t big_endian) {
     if (!path || !d || !nf || !nb || !un || !F || !p) return CLOWNS_EBADARG;
+    if (!desc_ok(d)) return CLOWNS_EBADARG;
     const long X = d->nx + 2, Y = d->ny + 2, Z = d->nz + 2, nc = X * Y * Z;
     std::vector<int32_t> nf32(nc);
     for (long c = 0; c < nc; ++c) nf32[c] = nf[c];
@@ -468,4 +484,43 @@ extern "C" int clowns_deck_write_data(const char *path, const clowns_deck_desc *
     w.f64(dt);
     w.end();
     return w.ok() ? CLOWNS_OK : CLOWNS_EDECK;
+}
+
+// The C-ABI: the readers and writers allocate host vectors; an allocation
+// failure is reported as CLOWNS_ENOMEM rather than crossing the boundary.
+template <typename F> static int guarded(F f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return CLOWNS_ENOMEM;
+    } catch (...) {
+        return CLOWNS_EDECK;
+    }
+}
+
+extern "C" int clowns_deck_open(const char *grid_path, const char *data_path, clowns_deck_desc *d) {
+    return guarded([&] { return clowns_deck_open_impl(grid_path, data_path, d); });
+}
+
+extern "C" int clowns_deck_read(const char *grid_path, const char *data_path, const char *fdata_path,
+                                clowns_deck_desc *d, int8_t *solid, double *F, double *un, double *p,
+                                double *a, double *fb, double *x, double *dx) {
+    return guarded([&] {
+        return clowns_deck_read_impl(grid_path, data_path, fdata_path, d, solid, F, un, p, a, fb, x, dx);
+    });
+}
+
+extern "C" int clowns_deck_faces(const char *grid_path, const clowns_deck_desc *d, double *faces) {
+    return guarded([&] { return clowns_deck_faces_impl(grid_path, d, faces); });
+}
+
+extern "C" int clowns_deck_write_grid(const char *path, const clowns_deck_desc *d, const int8_t *solid,
+                                      const double *faces, int32_t big_endian) {
+    return guarded([&] { return clowns_deck_write_grid_impl(path, d, solid, faces, big_endian); });
+}
+
+extern "C" int clowns_deck_write_data(const char *path, const clowns_deck_desc *d, const int8_t *nf,
+                                      const int8_t *nb, const double *un, const double *F, const double *p,
+                                      double t, double dt, int32_t big_endian) {
+    return guarded([&] { return clowns_deck_write_data_impl(path, d, nf, nb, un, F, p, t, dt, big_endian); });
 }

@@ -325,3 +325,25 @@ def test_grid_writer_matches_reference(tmp_path):
     from clowns3d import _lib as L
     L.check(L.lib().clowns_deck_open(str(mine).encode(), str(case["data"]).encode(), ctypes.byref(d2)), "open")
     assert d2.sparse == 1 and d2.inne == d.inne
+
+
+def test_corrupt_record_length_and_bad_descriptor(tmp_path):
+    """A record length larger than the file is an unreadable file, not an
+    allocation; a descriptor whose extents disagree with its sizes is refused
+    before any array is touched."""
+    import struct
+    from clowns3d import decks
+    from clowns3d._lib import CLOWNS_EBADARG, CLOWNS_EDECK, ClownsError
+    case = make_case(tmp_path, 5, 4, 3, seed=12, sparse=False)
+    bad = tmp_path / "bad.data"
+    raw = bytearray(case["data"].read_bytes())
+    raw[32:36] = struct.pack("<i", 0x7FFFFFF0)  # the nf record's length
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(ClownsError) as e:
+        decks.read_case(case["grid"], bad, None)
+    assert e.value.code == CLOWNS_EDECK
+    g, p, d = ours(case)
+    d.stride = 3  # too small for nx = 5
+    with pytest.raises(ClownsError) as e:
+        decks.write_data(tmp_path / "x.data", d, case["nf"], case["nb"], g.un.reshape(3, -1), g.F, p, 1.0, 0.1)
+    assert e.value.code == CLOWNS_EBADARG
