@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <new>
 #include <vector>
 
 #include "clowns_pcg.h"
@@ -246,11 +247,10 @@ class Classifier {
 
 } // namespace
 
-extern "C" int clowns_classify(const clowns_grid_desc *g, const int8_t *solid, double *F,
-                               const double *fb, const double *a, const double *dx,
-                               const double *xf, int8_t *nf, int8_t *nb, int32_t *mno,
-                               int32_t *row2cell, double *sp_row, int32_t cap, int32_t *nrows,
-                               double *mp) {
+static int classify_impl(const clowns_grid_desc *g, const int8_t *solid, double *F, const double *fb,
+                         const double *a, const double *dx, const double *xf, int8_t *nf, int8_t *nb,
+                         int32_t *mno, int32_t *row2cell, double *sp_row, int32_t cap, int32_t *nrows,
+                         double *mp) {
     if (!g || !solid || !F || !fb || !a || !dx || !xf || !nf || !nb || !mno || !row2cell ||
         !nrows || !mp)
         return CLOWNS_EBADARG;
@@ -313,6 +313,20 @@ extern "C" int clowns_classify(const clowns_grid_desc *g, const int8_t *solid, d
     }
     *nrows = N;
     return CLOWNS_OK;
+}
+
+// The classifier keeps host lists (fln, sfn): an allocation failure is
+// reported as CLOWNS_ENOMEM rather than thrown across the C-ABI.
+extern "C" int clowns_classify(const clowns_grid_desc *g, const int8_t *solid, double *F,
+                               const double *fb, const double *a, const double *dx,
+                               const double *xf, int8_t *nf, int8_t *nb, int32_t *mno,
+                               int32_t *row2cell, double *sp_row, int32_t cap, int32_t *nrows,
+                               double *mp) {
+    try {
+        return classify_impl(g, solid, F, fb, a, dx, xf, nf, nb, mno, row2cell, sp_row, cap, nrows, mp);
+    } catch (const std::bad_alloc &) {
+        return CLOWNS_ENOMEM;
+    }
 }
 
 namespace {

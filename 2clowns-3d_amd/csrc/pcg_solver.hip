@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "clowns_pcg.h"
@@ -1181,18 +1182,22 @@ int will_filter(clowns_solver *h, const int *d_LL, int NL, int lda, int N, const
         if (rounds) *rounds = r0 + 1;
         return CLOWNS_OK;
     }
-    std::vector<int> LL((size_t)lda * NL);
-    std::vector<double> X(N), B(N);
-    HIPCHK(hipMemcpy2DAsync(LL.data(), (size_t)lda * sizeof(int), d_LL, (size_t)lda * sizeof(int),
-                            (size_t)N * sizeof(int), NL, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(X.data(), d_X, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(B.data(), d_R, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    will_host(N, lda, NL, LL, X, B);
-    if (d_Xout)
-        HIPCHK(hipMemcpyAsync(d_Xout, X.data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d_B, B.data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
+    try {  // host vectors: an allocation failure is reported, not thrown across the C-ABI
+        std::vector<int> LL((size_t)lda * NL);
+        std::vector<double> X(N), B(N);
+        HIPCHK(hipMemcpy2DAsync(LL.data(), (size_t)lda * sizeof(int), d_LL, (size_t)lda * sizeof(int),
+                                (size_t)N * sizeof(int), NL, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(X.data(), d_X, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(B.data(), d_R, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        will_host(N, lda, NL, LL, X, B);
+        if (d_Xout)
+            HIPCHK(hipMemcpyAsync(d_Xout, X.data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_B, B.data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    } catch (const std::bad_alloc &) {
+        return CLOWNS_ENOMEM;
+    }
     if (rounds) *rounds = -1;
     return CLOWNS_OK;
 }
@@ -1560,6 +1565,7 @@ extern "C" int clowns_solver_classify(clowns_solver *h, const clowns_grid_desc *
         fallback = hv[0] || hv[1] || h->cls_force_host;
         break;
     }
+    try {  // host vectors: an allocation failure is reported, not thrown across the C-ABI
     // dx/xf are tiny: mp = (smallest cell size)^2 on the host
     std::vector<double> hdx;
     if ((rc = to_host(hdx, d_dx, (size_t)3 * g->stride, s))) return rc;
@@ -1596,6 +1602,9 @@ extern "C" int clowns_solver_classify(clowns_solver *h, const clowns_grid_desc *
         HIPCHK(hipStreamSynchronize(s));
         *nrows = N;
         return CLOWNS_OK;
+    }
+    } catch (const std::bad_alloc &) {
+        return CLOWNS_ENOMEM;
     }
     k_cls_count<<<nscan, SCAN_BS, 0, s>>>(G, imax, kmax, bcount);
     k_cls_scan_blocks<<<1, 1024, 0, s>>>(nscan, bcount);
