@@ -54,6 +54,8 @@ L.SIGNATURES.update({
                                          ctypes.POINTER(VofInfo)]),
     "clowns_solver_vof_time_kernel": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), _P, _I, _I, _P, _P,
                                                      ctypes.POINTER(ctypes.c_float)]),
+    "clowns_solver_vof_plic": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), _P, _I, _P, _P, _P, _P,
+                                              ctypes.POINTER(VofInfo)]),
 })
 
 
@@ -226,14 +228,15 @@ class DeviceGrid:
         solver.version += 1
         return p, {k: getattr(info, k) for k, _ in PCInfo._fields_ if k != "reserved"}
 
-    def vof(self, solver, Fp, mode=0, qs=None, qs_mask=None, dt=None, out_F=None):
+    def vof(self, solver, Fp, mode=0, qs=None, qs_mask=None, dt=None, out_F=None, plic=None):
         """The VOF update cal_vof on the device (clowns_solver_vof), without
         its closing classification: face fluxes, surface stage (mode 1:
         quvwf_kaidan; mode 0: the PLIC face fluxes qs where qs_mask != 0,
-        both (3, ncell), or none), quvwf_modify, cal_new_f, simple_slide(0).
-        Reads the flags, F, fb, a, un and dx held here; Fp is F of the
-        previous step.  Returns (F_new, qf (3, ncell), info dict) with
-        info["dt"] the step after cal_new_f's reductions."""
+        both (3, ncell), or none; plic = (cells, volumes (n, 6)): quvwf_normal
+        from PLIC segment volumes, clowns_solver_vof_plic), quvwf_modify,
+        cal_new_f, simple_slide(0).  Reads the flags, F, fb, a, un and dx held
+        here; Fp is F of the previous step.  Returns (F_new, qf (3, ncell),
+        info dict) with info["dt"] the step after cal_new_f's reductions."""
         torch = self._torch
         nc = self.g.ncell
         dev = self.device
@@ -254,6 +257,17 @@ class DeviceGrid:
         if dt is not None:
             g.dt = float(dt)
         info = VofInfo()
+        if plic is not None:
+            cells = t(plic[0], torch.int32).view(-1)
+            vol = t(plic[1], torch.float64).view(-1)
+            if vol.numel() != 6 * cells.numel():
+                raise ValueError("plic volumes must be (n, 6)")
+            L.check(L.lib().clowns_solver_vof_plic(solver._h, ctypes.byref(g), ctypes.c_void_p(Fp.data_ptr()),
+                                                   int(cells.numel()), ctypes.c_void_p(cells.data_ptr()),
+                                                   ctypes.c_void_p(vol.data_ptr()), ctypes.c_void_p(qf.data_ptr()),
+                                                   ctypes.c_void_p(Fn.data_ptr()), ctypes.byref(info)),
+                    "clowns_solver_vof_plic")
+            return Fn, qf.view(3, nc), {k: getattr(info, k) for k, _ in VofInfo._fields_}
         L.check(L.lib().clowns_solver_vof(solver._h, ctypes.byref(g), ctypes.c_void_p(Fp.data_ptr()), int(mode),
                                           qp, mp_, ctypes.c_void_p(qf.data_ptr()), ctypes.c_void_p(Fn.data_ptr()),
                                           ctypes.byref(info)), "clowns_solver_vof")

@@ -1689,7 +1689,7 @@ __global__ void k_vof_cnt_zero(int *__restrict__ cnt, int i0, int n) {
 // on success the counters are zeroed and every flag the kernels rely on
 // for their neighbour reads has been verified on the device.
 int vof_setup(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp, double *d_qf, double *d_Fnew,
-              VofDev &v) {
+              VofDev &v, int n_plic = 0, const int32_t *d_plic_cells = nullptr) {
     if (!h || !vof_grid_ok(g) || !d_Fp || !d_qf || !d_Fnew) return CLOWNS_EBADARG;
     int rc = set_device(h);
     if (rc) return rc;
@@ -1717,20 +1717,20 @@ int vof_setup(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp, do
     int *cnt = h->vof_cnt.p;
     k_vof_cnt_zero<<<1, 64, 0, s>>>(cnt, 0, VC_N);
     k_vof_check<<<cdiv(nc, 256), 256, 0, s>>>(v, cnt);
+    if (n_plic > 0) k_vof_check_plic<<<cdiv(n_plic, 256), 256, 0, s>>>(v, n_plic, d_plic_cells, cnt);
     int bad = 0;
     HIPCHK(hipMemcpyAsync(&bad, cnt + VC_BAD, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return bad ? CLOWNS_EBADARG : CLOWNS_OK;
 }
-} // namespace
 
-extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp, int32_t mode,
-                                 const double *d_qs, const int8_t *d_qs_mask, double *d_qf, double *d_Fnew,
-                                 clowns_vof_info *info) {
-    if (mode != 0 && mode != 1) return CLOWNS_EBADARG;
-    if ((d_qs == nullptr) != (d_qs_mask == nullptr)) return CLOWNS_EBADARG;
+// One cal_vof: mode 0 (PLIC face fluxes given per face, or the PLIC
+// volumes of n_plic listed cells) or 1 (quvwf_kaidan).
+int vof_run(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp, int32_t mode, const double *d_qs,
+            const int8_t *d_qs_mask, int n_plic, const int32_t *d_plic_cells, const double *d_plic_vol,
+            double *d_qf, double *d_Fnew, clowns_vof_info *info) {
     VofDev v;
-    int rc = vof_setup(h, g, d_Fp, d_qf, d_Fnew, v);
+    int rc = vof_setup(h, g, d_Fp, d_qf, d_Fnew, v, n_plic, d_plic_cells);
     if (rc) return rc;
     hipStream_t s = h->stream;
     const long nc = v.ncell;
@@ -1744,6 +1744,7 @@ extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, con
         k_vof_flux<1><<<G8, 256, 0, s>>>(v, dt, nullptr, nullptr);
     else
         k_vof_flux<0><<<G8, 256, 0, s>>>(v, dt, d_qs, d_qs_mask);
+    if (n_plic > 0) k_vof_plic<<<cdiv(n_plic, 256), 256, 0, s>>>(v, dt, n_plic, d_plic_cells, d_plic_vol);
     // quvwf_modify: rounds until every surface cell is decided (each round
     // decides at least the first pending cell of the serial order)
     k_vof_state_init<<<G, 256, 0, s>>>(nc, h->vof_state.p);
@@ -1787,6 +1788,22 @@ extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, con
     out.ms = ms;
     if (info) *info = out;
     return CLOWNS_OK;
+}
+} // namespace
+
+extern "C" int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp, int32_t mode,
+                                 const double *d_qs, const int8_t *d_qs_mask, double *d_qf, double *d_Fnew,
+                                 clowns_vof_info *info) {
+    if (mode != 0 && mode != 1) return CLOWNS_EBADARG;
+    if ((d_qs == nullptr) != (d_qs_mask == nullptr)) return CLOWNS_EBADARG;
+    return vof_run(h, g, d_Fp, mode, d_qs, d_qs_mask, 0, nullptr, nullptr, d_qf, d_Fnew, info);
+}
+
+extern "C" int clowns_solver_vof_plic(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp,
+                                      int32_t n_plic, const int32_t *d_plic_cells, const double *d_plic_vol,
+                                      double *d_qf, double *d_Fnew, clowns_vof_info *info) {
+    if (n_plic < 0 || (n_plic > 0 && (!d_plic_cells || !d_plic_vol))) return CLOWNS_EBADARG;
+    return vof_run(h, g, d_Fp, 0, nullptr, nullptr, n_plic, d_plic_cells, d_plic_vol, d_qf, d_Fnew, info);
 }
 
 extern "C" int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid_dev *g, const double *d_Fp,

@@ -159,6 +159,51 @@ k_vof_flux(VofDev v, double dt, const double *__restrict__ qs, const int8_t *__r
     }
 }
 
+// quvwf_normal (1sub-vof.F90:119-154) on the listed surface cells, with
+// cal_idou2's flux (1sub-normal.F90:29-99) from the caller's PLIC segment
+// volumes vol[6 m + 2 l + ll] (face l, ll = 0 low / 1 high of cells[m]):
+// on each face the cell drains through (un (1 - 2 ll) < 0, a != 0),
+// qf = sign(u) V / area / dt / a, 0 for |u| dt < SMALL or V = 0.  Runs after
+// k_vof_flux<0>.  Two listed cells never write one face: the shared face's
+// velocity drains only one of them.
+__global__ void __launch_bounds__(256)
+k_vof_plic(VofDev v, double dt, int n, const int *__restrict__ cells, const double *__restrict__ vol) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const long c = cells[m];
+    const long st[3] = {1, (long)(v.nx + 2), (long)(v.nx + 2) * (v.ny + 2)};
+    int p[3];
+    vof_pos(v, c, p);
+    const size_t S = (size_t)v.stride, nc = (size_t)v.ncell;
+    const double dxx = v.dx[p[0]], dyy = v.dx[S + p[1]], dzz = v.dx[2 * S + p[2]];
+    for (int l = 0; l < 3; ++l)
+        for (int ll = 0; ll < 2; ++ll) {
+            const long nn = c + ll * st[l];
+            const bool solid = v.nf[nn] == -1;
+            const double u = solid ? 0.0 : v.un[l * nc + nn];
+            const double a = solid ? 0.0 : v.a[l * nc + nn];
+            const double lsn = 1.0 - 2.0 * ll;
+            if (u * lsn >= 0.0 || a == 0.0) continue;
+            double idou = 0.0;
+            if (!(fabs(u) * dt < 1e-9)) {  // SMALL, 0mod-variables.F90:39
+                const double area = l == 2 ? dxx * dyy : (l == 1 ? dxx * dzz : dzz * dyy);
+                const double vs = vol[6 * (size_t)m + 2 * l + ll];
+                idou = vs == 0.0 ? 0.0 : vof_sign(1.0, u) * vs / area / dt;
+            }
+            v.qf[l * nc + nn] = idou / a;
+        }
+}
+
+// Each listed PLIC cell must be a flow cell (interior: its neighbour reads
+// stay in the box, the ring being solid).
+__global__ void __launch_bounds__(256)
+k_vof_check_plic(VofDev v, int n, const int *__restrict__ cells, int *__restrict__ cnt) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const long c = cells[m];
+    if (c < 0 || c >= v.ncell || v.nf[c] == -1) atomicOr(&cnt[VC_BAD], 1);
+}
+
 // The flags every other kernel relies on for its neighbour reads: the box
 // ring is solid (nf = -1), nf is -1..2, and a surface cell has 1 <= |nb| <= 3.
 __global__ void __launch_bounds__(256) k_vof_check(VofDev v, int *__restrict__ cnt) {

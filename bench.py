@@ -258,8 +258,8 @@ def vof_cpu_baseline(g, A, Fp, dt, threads):
             retries = ctypes.c_int(0)
             t0 = time.perf_counter()
             orc.oracle_vof(g.nx, g.ny, g.nz, g.stride, P(A["solid"]), P(c["nf"]), P(c["nb"]), P(c["F"]), P(Fp),
-                           P(A["fb"]), P(A["a"]), P(A["un"]), P(A["dx"]), ctypes.byref(dtv), 1, None, None, P(qf),
-                           P(fo), ctypes.byref(retries), None)
+                           P(A["fb"]), P(A["a"]), P(A["un"]), P(A["dx"]), ctypes.byref(dtv), 1, None, None, 0, None,
+                           None, P(qf), P(fo), ctypes.byref(retries), None)
             times.append(time.perf_counter() - t0)
     return {"value": min(times), "unit": "s/step", "cores": threads, "kind": kind,
             "sample": (f"the same grid ({nc} cells) on the host; "

@@ -348,6 +348,21 @@ int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g,
                       const int8_t *d_qs_mask, double *d_qf, double *d_Fnew,
                       clowns_vof_info *info);
 
+/* clowns_solver_vof for NORMAL builds with quvwf_normal itself on the device
+ * (src2LOWNS/1sub-vof.F90:119-154): d_plic_cells lists the n_plic surface
+ * cells of the PLIC reconstruction (the reference's sfnoi), d_plic_vol
+ * holds per listed cell the six segment volumes cal_idou2 takes from its
+ * polyhedron clipping (src2LOWNS/1sub-normal.F90:29-99), face l, low
+ * (ll = 0) or high (ll = 1) at [6 m + 2 l + ll].  On each face the cell
+ * drains through, qf = sign(u) V / area / dt / a (0 where |u| dt < 1e-9 or
+ * V = 0); the rest as clowns_solver_vof.  The listed cells must be flow
+ * cells (CLOWNS_EBADARG otherwise). */
+int clowns_solver_vof_plic(clowns_solver *h, const clowns_grid_dev *g,
+                           const double *d_Fp, int32_t n_plic,
+                           const int32_t *d_plic_cells,
+                           const double *d_plic_vol, double *d_qf,
+                           double *d_Fnew, clowns_vof_info *info);
+
 /* Average duration of one launch of a VOF-update kernel over `reps`
  * launches (HIP events on the solver's stream), on the inputs of
  * clowns_solver_vof: kernel 0 the face fluxes (kaidan surface stage),

@@ -22,6 +22,7 @@ int oracle_pcgpme_will(const double *AL, int NL, int NA, const int *LL,
 int oracle_set_dx(double *x, int m, double *dx);
 int oracle_vof(int nx, int ny, int nz, int stride, const int8_t *solid, const int8_t *nf, const int8_t *nb,
                const double *F, const double *Fp, const double *fb, const double *a, const double *un,
-               const double *dx, double *dt, int mode, const double *qs, const int8_t *qs_mask, double *qf,
-               double *Fn, int *retries, int *counts);
+               const double *dx, double *dt, int mode, const double *qs, const int8_t *qs_mask, int n_plic,
+               const int32_t *plic_cells, const double *plic_vol, double *qf, double *Fn, int *retries,
+               int *counts);
 #endif

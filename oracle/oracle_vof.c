@@ -127,6 +127,36 @@ static void kaidan(V *v) {
     }
 }
 
+/* quvwf_normal (1sub-vof.F90:119-154) with cal_idou2's flux
+ * (1sub-normal.F90:29-99) from given PLIC segment volumes: for each listed
+ * surface cell and each of its faces (l, ll = 0 low / 1 high) through which
+ * it drains (un * (1 - 2 ll) < 0, a != 0), qf = sign(u) * V / area / dt / a,
+ * 0 where |u| dt < SMALL or V = 0.  vol[6*m + 2*l + ll] belongs to cells[m]. */
+static void plic_flux(V *v, int n, const int32_t *cells, const double *vol) {
+    const double SMALL = 1e-9;
+    for (int m = 0; m < n; ++m) {
+        const long c = cells[m];
+        const double dxx = dxc(v, 0, c), dyy = dxc(v, 1, c), dzz = dxc(v, 2, c);
+        for (int l = 0; l < 3; ++l)
+            for (int ll = 0; ll < 2; ++ll) {
+                const long nn = c + ll * v->st[l];
+                const double lsn = 1.0 - 2.0 * ll;
+                const double u = v->solid[nn] ? ZERO : v->un[l * v->nc + nn];
+                const double a = A_(v, l, nn);
+                if (u * lsn >= ZERO || a == ZERO) continue;
+                double idou;
+                if (fabs(u) * v->dt < SMALL) {
+                    idou = ZERO;
+                } else {
+                    const double area = l == 2 ? dxx * dyy : (l == 1 ? dxx * dzz : dzz * dyy);
+                    const double vs = vol[6 * m + 2 * l + ll];
+                    idou = vs == ZERO ? ZERO : fsign(ONE, u) * vs / area / v->dt;
+                }
+                v->qf[l * v->nc + nn] = idou / a;
+            }
+    }
+}
+
 /* quvwf_modify (1sub-vof.F90:186-213), sfn order = ascending c */
 static int modify(V *v) {
     for (long c = 0; c < v->nc; ++c) {
@@ -200,7 +230,8 @@ static void slide(V *v) {
 
 /* One cal_vof without its closing set_nflag.  mode 0: NORMAL build, with the
  * PLIC face fluxes qs where qs_mask != 0 (either may be NULL: none);
- * mode 1: quvwf_kaidan.  F: current F (after set_nflag), Fp: F of the
+ * mode 1: quvwf_kaidan; mode 2: NORMAL build, quvwf_normal on the n_plic
+ * listed cells with their PLIC segment volumes (plic_flux above).  F: current F (after set_nflag), Fp: F of the
  * previous step, nf/nb: the flags.  Out: qf (3*ncell), Fn (ncell; solid
  * cells keep F), dt (reduced by cal_new_f's retries), *retries, and
  * counts[0..1] (may be NULL): surface cells corrected by quvwf_modify, cells
@@ -208,8 +239,9 @@ static void slide(V *v) {
  * Returns 0, VOF_EVOF or VOF_ENONFINITE. */
 int oracle_vof(int nx, int ny, int nz, int stride, const int8_t *solid, const int8_t *nf, const int8_t *nb,
                const double *F, const double *Fp, const double *fb, const double *a, const double *un,
-               const double *dx, double *dt, int mode, const double *qs, const int8_t *qs_mask, double *qf,
-               double *Fn, int *retries, int *counts) {
+               const double *dx, double *dt, int mode, const double *qs, const int8_t *qs_mask, int n_plic,
+               const int32_t *plic_cells, const double *plic_vol, double *qf, double *Fn, int *retries,
+               int *counts) {
     V v;
     v.n_modified = v.n_slid = 0;
     v.X = nx + 2;
@@ -236,6 +268,8 @@ int oracle_vof(int nx, int ny, int nz, int stride, const int8_t *solid, const in
     face_flux(&v);
     if (mode == 1) {
         kaidan(&v);
+    } else if (mode == 2) {
+        plic_flux(&v, n_plic, plic_cells, plic_vol);
     } else if (qs && qs_mask) {
         for (long c = 0; c < v.nc; ++c)
             for (int l = 0; l < 3; ++l)

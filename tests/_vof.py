@@ -57,7 +57,20 @@ def plic_faces(g, A, nf, seed):
     return qs, mask
 
 
-def call_oracle_vof(g, A, nf, nb, F, Fp, dt, mode, qs=None, mask=None):
+def plic_volumes(g, A, nf, seed, frac=1.0):
+    """Stand-in PLIC segment volumes for NORMAL builds' quvwf_normal: for the
+    listed surface cells (a seeded fraction of the nf = 2 cells) six
+    volumes each, faces (l, ll) in the order 2 l + ll, up to the slab
+    |u| dt * area; some zero.  Returns (cells int32, vol (n, 6))."""
+    rng = np.random.default_rng(seed)
+    cells = np.nonzero(nf == 2)[0]
+    cells = cells[rng.random(len(cells)) < frac].astype(np.int32)
+    vol = rng.uniform(0.0, 1.0, (len(cells), 6)) * 0.02
+    vol[rng.random(vol.shape) < 0.1] = 0.0
+    return cells, vol
+
+
+def call_oracle_vof(g, A, nf, nb, F, Fp, dt, mode, qs=None, mask=None, plic=None):
     nc = g.ncell
     qf = np.zeros(3 * nc)
     Fn = np.zeros(nc)
@@ -66,10 +79,15 @@ def call_oracle_vof(g, A, nf, nb, F, Fp, dt, mode, qs=None, mask=None):
     counts = np.zeros(2, np.int32)
     o = oracle()
     o.oracle_vof.restype = ctypes.c_int
+    n_plic, pc, pv = 0, None, None
+    if plic is not None:
+        cells = np.ascontiguousarray(plic[0], dtype=np.int32)
+        vol = np.ascontiguousarray(plic[1], dtype=np.float64)
+        n_plic, pc, pv = len(cells), ptr(cells), ptr(vol)
     rc = o.oracle_vof(g.nx, g.ny, g.nz, g.stride, ptr(A["solid"]), ptr(nf), ptr(nb), ptr(F), ptr(Fp), ptr(A["fb"]),
                       ptr(A["a"]), ptr(A["un"]), ptr(A["dx"]), ctypes.byref(dtv), mode,
-                      ptr(qs) if qs is not None else None, ptr(mask) if mask is not None else None, ptr(qf),
-                      ptr(Fn), ctypes.byref(retries), ptr(counts))
+                      ptr(qs) if qs is not None else None, ptr(mask) if mask is not None else None, n_plic, pc, pv,
+                      ptr(qf), ptr(Fn), ctypes.byref(retries), ptr(counts))
     return dict(rc=rc, qf=qf.reshape(3, nc), F=Fn, dt=dtv.value, retries=retries.value, modified=int(counts[0]),
                 slid=int(counts[1]))
 

@@ -3,8 +3,8 @@
 cal_vof routines): face fluxes, new F and the reduced dt bit for bit, with
 the same counts of corrected and slid cells.
 
-Cases: the synthetic configurations in both surface-stage modes (kaidan, and
-PLIC face fluxes given), a stressed field (corrections and slides), an
+Cases: the synthetic configurations in the surface-stage modes (kaidan, PLIC
+face fluxes given, quvwf_normal from PLIC segment volumes), a stressed field (corrections and slides), an
 outflow that makes cal_new_f shrink dt, a chain of surface cells whose
 corrections feed each other (the serial order of quvwf_modify), and the
 256x256x64 configuration (volume conservation to rounding and the oracle)."""
@@ -112,5 +112,41 @@ def test_vof_rejects_open_ring():
     dg = DeviceGrid(g, classified=c)
     with pytest.raises(ClownsError) as e:
         dg.vof(sol, Fp, mode=1, dt=dt)
+    assert e.value.code == CLOWNS_EBADARG
+    sol.close()
+
+
+@pytest.mark.parametrize("name,scenario", [("bathy28x24x16", "stress"), ("wavy32x24x16", "step"),
+                                           ("solitary40x8x12", "stress")])
+def test_vof_plic_volumes_match_oracle(name, scenario):
+    """quvwf_normal on the device from PLIC segment volumes
+    (clowns_solver_vof_plic) equals the oracle's bit for bit."""
+    from clowns3d import DeviceGrid, Solver, assembly
+    from _vof import plic_volumes
+    g, A, Fp, dt = vof_case(name, scenario, seed=9)
+    c = assembly.classify(g)
+    cells, vol = plic_volumes(g, A, c["nf"], seed=4, frac=0.8)
+    o = call_oracle_vof(g, A, c["nf"], c["nb"], c["F"], Fp, dt, 2, plic=(cells, vol))
+    assert o["rc"] == VOF_OK
+    sol = Solver(0)
+    dg = DeviceGrid(g, classified=c)
+    Fn, qf, info = dg.vof(sol, Fp, dt=dt, plic=(cells, vol))
+    print("oracle", {k: o[k] for k in ("retries", "modified", "slid", "dt")}, "gpu", info, flush=True)
+    assert np.array_equal(qf.cpu().numpy(), o["qf"]) and np.array_equal(Fn.cpu().numpy(), o["F"])
+    assert info["modified"] == o["modified"] and info["slid"] == o["slid"] and info["dt"] == o["dt"]
+    sol.close()
+
+
+def test_vof_plic_rejects_solid_cell():
+    from clowns3d import DeviceGrid, Solver, assembly
+    from clowns3d._lib import CLOWNS_EBADARG, ClownsError
+    g, A, Fp, dt = vof_case("wavy20x16x12", "step", seed=1)
+    c = assembly.classify(g)
+    cells = np.array([np.nonzero(c["nf"] == 2)[0][0], 0], np.int32)  # cell 0: the solid ring
+    vol = np.zeros((2, 6))
+    sol = Solver(0)
+    dg = DeviceGrid(g, classified=c)
+    with pytest.raises(ClownsError) as e:
+        dg.vof(sol, Fp, dt=dt, plic=(cells, vol))
     assert e.value.code == CLOWNS_EBADARG
     sol.close()

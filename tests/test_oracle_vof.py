@@ -122,3 +122,40 @@ def test_vof_plug_flow_moves_exact_volume():
     assert nf[c] == 1
     assert o["F"][c] == pytest.approx(F[c] - dF, abs=1e-15)
     assert o["F"][c + 1] == pytest.approx(F[c + 1] + dF, abs=1e-15)
+
+
+def test_vof_plic_volumes_formula():
+    """quvwf_normal from PLIC segment volumes (mode 2) equals mode 0 with the
+    face fluxes cal_idou2/a written out by hand from the same volumes: the
+    drained faces only, sign of u, V / area / dt, area from the surface
+    cell's sizes, nothing below |u| dt < 1e-9 or for V = 0."""
+    from _vof import plic_volumes
+    g, A, Fp, dt = vof_case("bathy28x24x16", "stress", seed=2)
+    nf, nb, F = _classified(g)
+    cells, vol = plic_volumes(g, A, nf, seed=3)
+    nc = g.ncell
+    st = (1, g.X, g.X * g.Y)
+    X, Y = g.X, g.Y
+    qs = np.zeros((3, nc))
+    mask = np.zeros((3, nc), np.int8)
+    un, a = A["un"], A["a"]
+    for m, c in enumerate(cells):
+        i, j, k = c % X, (c // X) % Y, c // (X * Y)
+        dxx, dyy, dzz = g.dx[0, i], g.dx[1, j], g.dx[2, k]
+        for l in range(3):
+            for ll in (0, 1):
+                n = c + ll * st[l]
+                u = 0.0 if A["solid"][n] else un[l, n]
+                lsn = 1 - 2 * ll
+                if u * lsn >= 0 or a[l, n] == 0:
+                    continue
+                area = dxx * dyy if l == 2 else (dxx * dzz if l == 1 else dzz * dyy)
+                v = vol[m, 2 * l + ll]
+                idou = 0.0 if abs(u) * dt < 1e-9 or v == 0 else np.copysign(1.0, u) * v / area / dt
+                qs[l, n] = idou / a[l, n]
+                mask[l, n] = 1
+    assert mask.sum() > 100
+    o2 = call_oracle_vof(g, A, nf, nb, F, Fp, dt, 2, plic=(cells, vol))
+    o0 = call_oracle_vof(g, A, nf, nb, F, Fp, dt, 0, qs, mask)
+    assert o2["rc"] == o0["rc"] == VOF_OK
+    assert np.array_equal(o2["qf"], o0["qf"]) and np.array_equal(o2["F"], o0["F"])
