@@ -25,6 +25,9 @@
  *
  * Pinning (tests/test_oracle_vof.py): bit-exact against the flang build of
  * those routines (oracle/_ref/libref_vof.so) on synthetic grids.
+ * plic_flux (quvwf_normal from PLIC segment volumes): parity unpinned -- the
+ * reference's loop needs its PLIC state, which is out of scope; the test
+ * only restates the same formula.
  */
 #include <math.h>
 #include <stdint.h>
