@@ -441,8 +441,8 @@ def cpu_baseline(g, iters_full, n_iter_hi, threads):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU z-slab design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(a multi-GPU run would shard the rows into z-slabs, one per GPU)", file=sys.stderr)
         sys.exit(2)
     if args.will:
         bench_will(args)
