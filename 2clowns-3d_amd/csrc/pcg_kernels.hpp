@@ -784,24 +784,41 @@ k_init(const Scal *__restrict__ sc, int N, int R, T *__restrict__ wk1,
 }
 
 // ------------------------------------------------------------ finalize --
-// Loads are issued eight at a time (independent) before the fixed-order adds:
-// the partials are few and L2-resident, so latency, not bandwidth, bounds this.
-__device__ __forceinline__ double fin_sum(const double *__restrict__ p, int G) {
+// Each thread sums its partials k, k + FIN_BS, k + 2 FIN_BS, ... in that
+// order, then the block tree adds the threads in fixed order.  The partials
+// are few (one per 512 rows) and latency, not bandwidth, bounds this: every
+// thread issues all its loads (up to eight per array, predicated) before the
+// first add, and the two-array form loads both arrays at once.
+template <int NA>
+__device__ __forceinline__ void fin_sums(const double *__restrict__ p, int G, double *out) {
     __shared__ double lds[FIN_BS / 64];
-    double s = 0.0;
-    int k = threadIdx.x;
-    for (; k + 7 * FIN_BS < G; k += 8 * FIN_BS) {
-        double v[8];
+    double s[NA];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = p[k + u * FIN_BS];
+    for (int a = 0; a < NA; ++a) s[a] = 0.0;
+    for (int k0 = 0; k0 < G; k0 += 8 * FIN_BS) {
+        double v[NA][8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s += v[u];
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + (int)threadIdx.x + u * FIN_BS;
+                v[a][u] = k < G ? p[(size_t)a * G + k] : 0.0;
+            }
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k0 + (int)threadIdx.x + u * FIN_BS < G) s[a] += v[a][u];
     }
-    for (; k < G; k += FIN_BS) s += p[k];
-    return block_sum<FIN_BS>(s, lds);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) out[a] = block_sum<FIN_BS>(s[a], lds);
+}
+__device__ __forceinline__ double fin_sum(const double *__restrict__ p, int G) {
+    double r;
+    fin_sums<1>(p, G, &r);
+    return r;
 }
 
-// Reset the run's scalars (also makes a stale `done` from an earlier run go).
 __global__ void k_reset(Scal *sc, double eps, int itermax, int beta_prod) {
     if (threadIdx.x != 0) return;
     Scal s = {};
@@ -816,8 +833,9 @@ __global__ void k_reset(Scal *sc, double eps, int itermax, int beta_prod) {
 __global__ void __launch_bounds__(FIN_BS)
 k_fin_init(Scal *sc, const double *__restrict__ part, int G) {
     if (sc->done) return;
-    const double bn = fin_sum(part, G);
-    const double c1 = fin_sum(part + G, G);
+    double r[2];
+    fin_sums<2>(part, G, r);
+    const double bn = r[0], c1 = r[1];
     if (threadIdx.x == 0) {
         sc->BN = bn;
         sc->C1 = c1;
@@ -832,14 +850,16 @@ k_fin_init(Scal *sc, const double *__restrict__ part, int G) {
     }
 }
 
-// ALP = C1/RQ (pcgpme_para.F90:167).
+// ALP = C1/RQ (pcgpme_para.F90:167).  The scalars are read before the sums
+// (their loads overlap the partials').
 __global__ void __launch_bounds__(FIN_BS)
 k_fin_alpha(Scal *sc, const double *__restrict__ part, int G) {
     if (sc->done) return;
+    const double C1 = sc->C1;
     const double rq = fin_sum(part, G);
     if (threadIdx.x == 0) {
         sc->RQ = rq;
-        sc->ALP = sc->C1 / rq;
+        sc->ALP = C1 / rq;
     }
 }
 
@@ -847,11 +867,11 @@ k_fin_alpha(Scal *sc, const double *__restrict__ part, int G) {
 __global__ void __launch_bounds__(FIN_BS)
 k_fin_omega(Scal *sc, const double *__restrict__ part, int G) {
     if (sc->done) return;
-    const double amu = fin_sum(part, G);
-    const double ss = fin_sum(part + G, G);
+    double r[2];
+    fin_sums<2>(part, G, r);
     if (threadIdx.x == 0) {
-        sc->SS = ss;
-        sc->AMU = amu / ss;
+        sc->SS = r[1];
+        sc->AMU = r[0] / r[1];
     }
 }
 
@@ -859,31 +879,34 @@ k_fin_omega(Scal *sc, const double *__restrict__ part, int G) {
 __global__ void __launch_bounds__(FIN_BS)
 k_fin_beta(Scal *sc, const double *__restrict__ part, int G, int K) {
     if (sc->done) return;
-    const double rn = fin_sum(part, G);
-    const double c1 = fin_sum(part + G, G);
+    const double BN = sc->BN, ERRP = sc->ERRP, EPS = sc->EPS, AMU = sc->AMU, RQ = sc->RQ;
+    const int beta_prod = sc->beta_prod, itermax = sc->itermax;
+    double r[2];
+    fin_sums<2>(part, G, r);
+    const double rn = r[0], c1 = r[1];
     if (threadIdx.x != 0) return;
     sc->RN = rn;
     sc->C1 = c1;
     sc->K = K;
-    const double err = sqrt(rn / sc->BN);
+    const double err = sqrt(rn / BN);
     sc->ERR = err;
     int stop = 0;
     if (!(err == err) || err > 1.7976931348623157e308) {
         sc->ier = 4001; // CLOWNS_ENONFINITE
         stop = 1;
-    } else if (K >= 10 && fabs(err - sc->ERRP) <= err * 1.0e-6) {
+    } else if (K >= 10 && fabs(err - ERRP) <= err * 1.0e-6) {
         stop = 1;
-    } else if (err <= sc->EPS) {
+    } else if (err <= EPS) {
         stop = 1;
     } else {
         sc->ERRP = err;
-        if (sc->AMU == 0.0 || c1 == 0.0) {
+        if (AMU == 0.0 || c1 == 0.0) {
             stop = 1;
         } else {
             // Same quantity, two roundings: pcgpme_para.F90:241 divides
             // twice, pcgpme_2DH.f90:241 divides by the product.
-            sc->BETA = sc->beta_prod ? c1 / (sc->AMU * sc->RQ) : c1 / sc->AMU / sc->RQ;
-            if (K >= sc->itermax) {
+            sc->BETA = beta_prod ? c1 / (AMU * RQ) : c1 / AMU / RQ;
+            if (K >= itermax) {
                 sc->ier = 3000;
                 sc->kout = K + 1;
                 sc->done = 1;
