@@ -130,17 +130,18 @@ class Solver:
         return x, info.as_dict()
 
     def index_bits(self):
-        """16 if the resident operator's kernels read 16-bit coded neighbour
-        indices, else 32."""
+        """8 or 16 if the resident operator's kernels read 8- or 16-bit coded
+        neighbour indices, else 32."""
         return L.lib().clowns_solver_index_bits(self._h)
 
     def tuning(self):
         """Kernel forms chosen for the resident operator (clowns_solver_tuning):
         dict fuse64, fuse32 (1 fused PM pairs), stream64, stream32 (bit 0
-        streamed PM stages, bit 1 streamed SpMV); -1 where undecided."""
-        out = np.zeros(4, np.int32)
+        streamed PM stages, bit 1 streamed SpMV), index64, index32 (bits of
+        the neighbour-index codes: 8, 16 or 32); -1 where undecided."""
+        out = np.zeros(6, np.int32)
         L.check(L.lib().clowns_solver_tuning(self._h, ctypes.c_void_p(out.ctypes.data)), "clowns_solver_tuning")
-        return dict(zip(("fuse64", "fuse32", "stream64", "stream32"), (int(v) for v in out)))
+        return dict(zip(("fuse64", "fuse32", "stream64", "stream32", "index64", "index32"), (int(v) for v in out)))
 
     def solve_will(self, b, out=None, eps=1e-10, iters=1000):
         """PCGPME_WILL on the resident operator (set_matrix): zero start, EPS

@@ -98,6 +98,10 @@ struct clowns_solver {
     DevBuf<int2> base16;
     bool ll16_ok = false;
     int ll16_wide = 0;     // slots read from the 32-bit LL (LL16_WIDE)
+    DevBuf<uint8_t> LL8;   // LL coded in 8 bits (k_ll8_build), valid if ll8_ok
+    DevBuf<int4> base8;
+    bool ll8_ok = false;
+    int ll8_wide = 0;      // slots read from the 32-bit LL (LL8_WIDE)
     DevBuf<float> AL32, D32;
     bool have32 = false;
     // Transposed operator (adjoint), scaled + sorted the same way.
@@ -149,6 +153,9 @@ struct clowns_solver {
     int fuse64_N = 0, fuse32_N = 0;    // ... for operators of about this many rows (tuned_size)
     int st64 = -1, st32 = -1;          // non-temporal streams chosen by tune_stream (bit 0 PM, bit 1 SpMV) ...
     int st64_N = 0, st32_N = 0;        // ... for operators of about this many rows (tuned_size)
+    int st64_f = -1, st32_f = -1;      // ... under this forcing (force_key)
+    int ix64 = -1, ix32 = -1;          // 8-bit (1) or 16-bit (0) index codes chosen by tune_stream ...
+    int ix64_N = 0, ix32_N = 0;        // ... for operators of about this many rows
 };
 
 namespace {
@@ -200,12 +207,17 @@ template <typename T> struct Op {
     bool st_pm = false, st_spmv = false;  // non-temporal streams (tune_stream)
     const short *LL16 = nullptr;  // 16-bit coded LL (two-row kernels), or null
     const int2 *base16 = nullptr;
+    const uint8_t *LL8 = nullptr; // 8-bit coded LL (two-row kernels), or null
+    const int4 *base8 = nullptr;
+    bool ix8 = false;             // the kernels read LL8 (tune_stream), else LL16
 };
 
 // The index source of column j for the two-row kernels.
 template <typename T> bool has16(const Op<T> &A) { return A.LL16 != nullptr; }
 template <typename T> Ix32 ix32(const Op<T> &A, size_t off) { return Ix32{A.LL + off}; }
 template <typename T> Ix16 ix16(const Op<T> &A, size_t off) { return Ix16{A.LL16 + off, A.LL + off, A.base16, A.N}; }
+template <typename T> bool has8(const Op<T> &A) { return A.LL8 != nullptr && A.ix8; }
+template <typename T> Ix8 ix8(const Op<T> &A, size_t off) { return Ix8{A.LL8 + off, A.LL + off, A.base8, A.N}; }
 
 // Pairwise fusion of single-term PM stages (k_pm2) saves one pass over the
 // intermediate vector but recomputes the first stage at every gather point
@@ -241,12 +253,24 @@ template <typename T> bool use_v2_vec() { return h_v2_vec; }
 // the index stream, set the pace.  CLOWNS_LL16 = 0 switches it off.  (Four
 // rows per thread was measured too: no faster than two in either precision.)
 bool h_ll16 = true;
+// 8-bit coded neighbour indices (Ix8) in the two-row kernels in place of the
+// 16-bit codes.  Half the index bytes, but more decoding and a scattered
+// 32-bit read per wide slot: measured on MI355X, FP64 solve 2.6% faster at
+// 13.1M rows and 2% slower at 2.45M rows (operator inside the last-level
+// cache), so tune_stream decides per operator.  CLOWNS_LL8 = 0 off, 1 on,
+// -1 (default) timed.
+int h_ll8 = -1;
 
 template <typename T>
 void launch_pm2(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j, const T *in, T *out) {
     const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
     const int G2 = geo_of(A.N, 2).G;
-    if (use_v2<T>() && has16(A)) {
+    if (use_v2<T>() && has8(A)) {
+        if (A.st_pm)
+            k_pm2_v2<T, Ix8, true><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix8(A, o0), A.AL + o1, ix8(A, o1), in, out);
+        else
+            k_pm2_v2<T, Ix8, false><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix8(A, o0), A.AL + o1, ix8(A, o1), in, out);
+    } else if (use_v2<T>() && has16(A)) {
         if (A.st_pm)
             k_pm2_v2<T, Ix16, true><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix16(A, o0), A.AL + o1, ix16(A, o1), in, out);
         else
@@ -300,6 +324,14 @@ int launch_pm_stage(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j,
     const int *l0 = A.LL + o0, *l1 = NT == 2 ? A.LL + o1 : nullptr;
     if (use_v2<T>()) {
         const int G2 = geo_of(A.N, 2).G;
+        if (has8(A)) {
+            const Ix8 x0 = ix8(A, o0), x1 = NT == 2 ? ix8(A, o1) : x0;
+            if (A.st_pm)
+                launch_pm_v2<T, NT, Ix8, true>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
+            else
+                launch_pm_v2<T, NT, Ix8, false>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
+            return G2;
+        }
         if (has16(A)) {
             const Ix16 x0 = ix16(A, o0), x1 = NT == 2 ? ix16(A, o1) : x0;
             if (A.st_pm)
@@ -434,7 +466,9 @@ void launch_spmv_v2(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int mo
 template <typename T, int NL>
 void launch_spmv_v2(hipStream_t s, const Scal *sc, const Op<T> &A, int mode, const T *v0, const T *v1,
                     const T *v2, T *vout, T *y) {
-    if (has16(A))
+    if (has8(A))
+        launch_spmv_v2<T, NL, Ix8>(s, sc, A, ix8(A, 0), mode, v0, v1, v2, vout, y);
+    else if (has16(A))
         launch_spmv_v2<T, NL, Ix16>(s, sc, A, ix16(A, 0), mode, v0, v1, v2, vout, y);
     else
         launch_spmv_v2<T, NL, Ix32>(s, sc, A, ix32(A, 0), mode, v0, v1, v2, vout, y);
@@ -456,90 +490,134 @@ void spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
         launch_spmv<T, 4>(s, g, sc, A, mode, v0, v1, v2, vout, y);
 }
 
-// Chooses the non-temporal (ST) or cached form of the PM application and of
-// the SpMV for this operator, once per size and precision (A.fuse already
-// decided).  What the forms leave in the last-level cache decides between
-// them, so each combination runs a steady-state block of its own: two
-// untimed, then three timed sequences of two iterations' kernels, the four
-// combinations in turn, the whole twice; the fastest mean wins.
-// (Interleaving single applications of different forms favours the
-// streamed form, which finds data the cached form left behind.)
+// Kernel forms as bits: bit 0 streamed PM stages, bit 1 streamed SpMV, bit 2
+// 8-bit index codes (only where the operator has them).
+template <typename T> bool can_ix8(const Op<T> &A) { return A.LL8 != nullptr; }
+template <typename T> void set_forms(Op<T> &A, int f) {
+    A.st_pm = (f & 1) != 0;
+    A.st_spmv = (f & 2) != 0;
+    A.ix8 = (f & 4) != 0 && can_ix8(A);
+}
+// The environment's forcing, which the stored decisions are keyed on.
+inline int force_key() { return (h_stream + 1) | ((h_ll8 + 1) << 2); }
+// Bits fixed for operator A (mask) and their values.
+template <typename T> void fixed_forms(const Op<T> &A, int &mask, int &val) {
+    mask = val = 0;
+    if (h_stream >= 0) mask |= 3, val |= 3 * h_stream;
+    if (!can_ix8(A) || h_ll8 >= 0) mask |= 4, val |= (can_ix8(A) && h_ll8 == 1) ? 4 : 0;
+}
+// The stored decisions, if they hold for A: forms in *f.  The index-code
+// decision is kept apart from the streamed forms', so an operator without
+// 8-bit codes (the adjoint's) reuses the streamed forms of its forward
+// operator.
+template <typename T> bool stored_forms(const clowns_solver *h, const Op<T> &A, int *f) {
+    int mask, val;
+    fixed_forms(A, mask, val);
+    const bool f32 = sizeof(T) == 4;
+    const int st = f32 ? h->st32 : h->st64, stN = f32 ? h->st32_N : h->st64_N;
+    const int stF = f32 ? h->st32_f : h->st64_f;
+    const int ix = f32 ? h->ix32 : h->ix64, ixN = f32 ? h->ix32_N : h->ix64_N;
+    const bool key = stF == force_key();
+    const bool st_ok = (mask & 3) == 3 || (st >= 0 && tuned_size(A.N, stN) && key);
+    const bool ix_ok = (mask & 4) != 0 || (ix >= 0 && tuned_size(A.N, ixN) && key);
+    if (!st_ok || !ix_ok) return false;
+    *f = ((mask & 3) == 3 ? (val & 3) : st) | ((mask & 4) ? (val & 4) : 4 * ix);
+    return true;
+}
+
+// Chooses, once per size and precision, the non-temporal (ST) or cached form
+// of the PM stages and of the SpMV, and the 8- or 16-bit index codes, for
+// this operator (A.fuse already decided).  What the forms leave in the
+// last-level cache decides between them, so each combination runs a
+// steady-state block of its own: two untimed, then three timed sequences of
+// two iterations' kernels, the combinations in turn, the whole twice; the
+// fastest mean wins.  (Interleaving single applications of different forms
+// favours the streamed form, which finds data the cached form left behind.)
 template <typename T> void tune_stream(clowns_solver *h, Op<T> &A, VecSet<T> &V) {
-    if (h_stream >= 0) {
-        A.st_pm = A.st_spmv = h_stream == 1;
+    int mask, fval, have;
+    fixed_forms(A, mask, fval);
+    if (stored_forms(h, A, &have)) {
+        set_forms(A, have);
         return;
     }
-    int &dec = sizeof(T) == 4 ? h->st32 : h->st64;
-    int &decN = sizeof(T) == 4 ? h->st32_N : h->st64_N;
-    if (dec < 0 || !tuned_size(A.N, decN)) {
-        hipStream_t s = h->stream;
-        Scal *sc = h->sc.p;
-        const Geo g = geo_of(A.N, h->R);
-        k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1, 0);
-        // two Bi-CGSTAB iterations' kernels in run_bicgstab's order, without
-        // the one-block finalizations (the vectors are scratch here: the
-        // solve initializes every one of them after tuning)
-        const int G2 = geo_of(A.N, 2).G;
-        double *part = h->part.p;
-        auto seq = [&](int form) {
-            Op<T> B = A;
-            B.st_pm = (form & 1) != 0;
-            B.st_spmv = (form & 2) != 0;
-            for (int it = 0; it < 2; ++it) {
-                if (use_v2_vec<T>()) k_pupd_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.p0.p, V.r.p, V.q.p);
-                spmv<T>(s, g, sc, B, 0, V.p0.p, nullptr, nullptr, nullptr, V.y.p);
-                apply_pm<T>(s, g, sc, B, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
-                if (use_v2_vec<T>()) k_eupd_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.r.p, V.q.p, V.e.p);
-                spmv<T>(s, g, sc, B, 0, V.e.p, nullptr, nullptr, nullptr, V.y.p);
-                apply_pm<T>(s, g, sc, B, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
-                if (use_v2_vec<T>())
-                    k_update_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
-            }
-        };
-        float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-        bool ok = true;
-        for (int pass = 0; pass < 2 && ok; ++pass)
-            for (int f = 0; f < 4 && ok; ++f) {
-                seq(f);
-                seq(f);
-                float ms = 0.f;
-                ok = hipEventRecord(h->ev0, s) == hipSuccess;
-                if (!ok) break;
-                for (int r = 0; r < 3; ++r) seq(f);
-                ok = hipEventRecord(h->ev1, s) == hipSuccess && hipEventSynchronize(h->ev1) == hipSuccess &&
-                     hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess;
-                best[f] = std::min(best[f], ms / 3.f);
-            }
-        if (!ok) return;
-        int f_best = 0;
-        for (int f = 1; f < 4; ++f)
-            if (best[f] < best[f_best]) f_best = f;
-        dec = f_best;
-        decN = A.N;
-        if (h_verbose)
-            fprintf(stderr,
-                    "clowns_pcg: N=%d %s two iterations: cached %.1f, PM streamed %.1f, SpMV streamed "
-                    "%.1f, both streamed %.1f us -> PM %s, SpMV %s\n",
-                    A.N, sizeof(T) == 4 ? "FP32" : "FP64", 1e3f * best[0], 1e3f * best[1], 1e3f * best[2],
-                    1e3f * best[3], (dec & 1) ? "streamed" : "cached", (dec & 2) ? "streamed" : "cached");
+    const bool f32 = sizeof(T) == 4;
+    int &st = f32 ? h->st32 : h->st64;
+    int &stN = f32 ? h->st32_N : h->st64_N;
+    int &stF = f32 ? h->st32_f : h->st64_f;
+    int &ix = f32 ? h->ix32 : h->ix64;
+    int &ixN = f32 ? h->ix32_N : h->ix64_N;
+    hipStream_t s = h->stream;
+    Scal *sc = h->sc.p;
+    const Geo g = geo_of(A.N, h->R);
+    k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1, 0);
+    // two Bi-CGSTAB iterations' kernels in run_bicgstab's order, without the
+    // one-block finalizations (the vectors are scratch here: the solve
+    // initializes every one of them after tuning)
+    const int G2 = geo_of(A.N, 2).G;
+    double *part = h->part.p;
+    auto seq = [&](int form) {
+        Op<T> B = A;
+        set_forms(B, form);
+        for (int it = 0; it < 2; ++it) {
+            if (use_v2_vec<T>()) k_pupd_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.p0.p, V.r.p, V.q.p);
+            spmv<T>(s, g, sc, B, 0, V.p0.p, nullptr, nullptr, nullptr, V.y.p);
+            apply_pm<T>(s, g, sc, B, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
+            if (use_v2_vec<T>()) k_eupd_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.r.p, V.q.p, V.e.p);
+            spmv<T>(s, g, sc, B, 0, V.e.p, nullptr, nullptr, nullptr, V.y.p);
+            apply_pm<T>(s, g, sc, B, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
+            if (use_v2_vec<T>())
+                k_update_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
+        }
+    };
+    float best[8];
+    for (float &b : best) b = 1e30f;
+    bool ok = true;
+    for (int pass = 0; pass < 2 && ok; ++pass)
+        for (int f = 0; f < 8 && ok; ++f) {
+            if ((f & mask) != fval) continue;
+            seq(f);
+            seq(f);
+            float ms = 0.f;
+            ok = hipEventRecord(h->ev0, s) == hipSuccess;
+            if (!ok) break;
+            for (int r = 0; r < 3; ++r) seq(f);
+            ok = hipEventRecord(h->ev1, s) == hipSuccess && hipEventSynchronize(h->ev1) == hipSuccess &&
+                 hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess;
+            best[f] = std::min(best[f], ms / 3.f);
+        }
+    if (!ok) {
+        set_forms(A, fval);
+        return;
     }
-    A.st_pm = (dec & 1) != 0;
-    A.st_spmv = (dec & 2) != 0;
+    int f_best = fval;
+    for (int f = 0; f < 8; ++f)
+        if (best[f] < best[f_best]) f_best = f;
+    if ((mask & 3) != 3) {
+        st = f_best & 3;
+        stN = A.N;
+    }
+    if (!(mask & 4)) {
+        ix = (f_best >> 2) & 1;
+        ixN = A.N;
+    }
+    stF = force_key();
+    if (h_verbose) {
+        fprintf(stderr, "clowns_pcg: N=%d %s two iterations (us):", A.N, f32 ? "FP32" : "FP64");
+        for (int f = 0; f < 8; ++f)
+            if (best[f] < 1e29f)
+                fprintf(stderr, " [%s%s%s] %.1f", (f & 4) ? "Ix8 " : "", (f & 1) ? "PM-st " : "PM ",
+                        (f & 2) ? "SpMV-st" : "SpMV", 1e3f * best[f]);
+        fprintf(stderr, " -> %s, PM %s, SpMV %s\n", (f_best & 4) ? "8-bit" : "16-bit",
+                (f_best & 1) ? "streamed" : "cached", (f_best & 2) ? "streamed" : "cached");
+    }
+    set_forms(A, f_best);
 }
 
 // The decisions already taken for this operator (time_kernel reports the
 // forms the solver runs).
 template <typename T> void known_stream(const clowns_solver *h, Op<T> &A) {
-    if (h_stream >= 0) {
-        A.st_pm = A.st_spmv = h_stream == 1;
-        return;
-    }
-    const int dec = sizeof(T) == 4 ? h->st32 : h->st64;
-    const int decN = sizeof(T) == 4 ? h->st32_N : h->st64_N;
-    if (dec >= 0 && tuned_size(A.N, decN)) {
-        A.st_pm = (dec & 1) != 0;
-        A.st_spmv = (dec & 2) != 0;
-    }
+    int f;
+    if (stored_forms(h, A, &f)) set_forms(A, f);
 }
 
 // The PCGPME iteration (pcgpme_para.F90:148-247) on already initialised
@@ -607,6 +685,7 @@ int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc,
 int check_nl(int NL) { return (NL == 6 || NL == 4) ? CLOWNS_OK : CLOWNS_EBADNL; }
 
 int build_ll16(clowns_solver *h);
+int build_ll8(clowns_solver *h);
 
 // Load AL/LL/D (reference layout, stride lda) from `kind` memory and scale +
 // sort in place on the device.
@@ -699,6 +778,30 @@ int build_ll16(clowns_solver *h) {
     HIPCHK(hipStreamSynchronize(s));
     h->ll16_ok = res[0] == 0;
     h->ll16_wide = res[1];
+    return build_ll8(h);
+}
+
+// The 8-bit form (Ix8), built beside the 16-bit one (which stays the
+// fallback: CLOWNS_LL8 = 0).
+int build_ll8(clowns_solver *h) {
+    h->ll8_ok = false;
+    h->ll8_wide = 0;
+    if (h_ll8 == 0 || !h->ll16_ok) return CLOWNS_OK;
+    const int N = h->N, nblk = cdiv(N, LL8_ROWS);
+    int rc;
+    if ((rc = h->LL8.ensure((size_t)h->lda * h->NL)) || (rc = h->base8.ensure(nblk)) || (rc = h->flag.ensure(2)))
+        return rc;
+    hipStream_t s = h->stream;
+    fill<int>(s, h->flag.p, 2, 0);
+    k_ll8_build<<<nblk, BS, 0, s>>>(N, h->lda, h->NL, h->LL.p, h->LL8.p, h->base8.p, h->flag.p, h->flag.p + 1);
+    HIPCHK(hipGetLastError());
+    int res[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(res, h->flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    h->ll8_ok = res[0] == 0;
+    h->ll8_wide = res[1];
+    if (h_verbose)
+        fprintf(stderr, "clowns_pcg: N=%d 8-bit neighbour codes: %d of %d slots wide\n", N, res[1], N * h->NL);
     return CLOWNS_OK;
 }
 
@@ -709,6 +812,10 @@ Op<double> op64(const clowns_solver *h) {
         A.LL16 = h->LL16.p;
         A.base16 = h->base16.p;
     }
+    if (h->ll8_ok) {
+        A.LL8 = h->LL8.p;
+        A.base8 = h->base8.p;
+    }
     return A;
 }
 Op<float> op32(const clowns_solver *h) {
@@ -716,6 +823,10 @@ Op<float> op32(const clowns_solver *h) {
     if (h->ll16_ok) {
         A.LL16 = h->LL16.p;
         A.base16 = h->base16.p;
+    }
+    if (h->ll8_ok) {
+        A.LL8 = h->LL8.p;
+        A.base8 = h->base8.p;
     }
     return A;
 }
@@ -942,6 +1053,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2VEC")) h_v2_vec = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_LL16")) h_ll16 = atoi(e) != 0;
+    if (const char *e = getenv("CLOWNS_LL8")) h_ll8 = atoi(e) < 0 ? -1 : (atoi(e) != 0 ? 1 : 0);
     if (const char *e = getenv("CLOWNS_VERBOSE")) h_verbose = atoi(e) != 0;
     *out = h;
     return CLOWNS_OK;
@@ -951,7 +1063,7 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     if (!h) return CLOWNS_OK;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
-    h->AL.release(); h->D.release(); h->W.release(); h->LL.release(); h->LL16.release(); h->base16.release();
+    h->AL.release(); h->D.release(); h->W.release(); h->LL.release(); h->LL16.release(); h->base16.release(); h->LL8.release(); h->base8.release();
     h->AL32.release(); h->D32.release();
     h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
     h->AL0.release(); h->D0.release(); h->LL0.release();
@@ -1834,18 +1946,25 @@ extern "C" int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid
     return CLOWNS_OK;
 }
 
-extern "C" int clowns_solver_tuning(clowns_solver *h, int32_t out[4]) {
+extern "C" int clowns_solver_tuning(clowns_solver *h, int32_t out[6]) {
     if (!h || !out || h->N < 1) return CLOWNS_EBADARG;
     const int N = h->N;
     out[0] = h_fuse_pm >= 0 ? h_fuse_pm : (tuned_size(N, h->fuse64_N) ? h->fuse64 : -1);
     out[1] = h_fuse_pm >= 0 ? h_fuse_pm : (tuned_size(N, h->fuse32_N) ? h->fuse32 : -1);
-    out[2] = h_stream >= 0 ? 3 * h_stream : (tuned_size(N, h->st64_N) ? h->st64 : -1);
-    out[3] = h_stream >= 0 ? 3 * h_stream : (tuned_size(N, h->st32_N) ? h->st32 : -1);
+    int f[2] = {-1, -1};
+    if (!stored_forms(h, op64(h), &f[0])) f[0] = -1;
+    if (!stored_forms(h, op32(h), &f[1])) f[1] = -1;
+    out[2] = f[0] < 0 ? -1 : (f[0] & 3);
+    out[3] = f[1] < 0 ? -1 : (f[1] & 3);
+    out[4] = f[0] < 0 ? -1 : (f[0] & 4 ? 8 : (h->ll16_ok ? 16 : 32));
+    out[5] = f[1] < 0 ? -1 : (f[1] & 4 ? 8 : (h->ll16_ok ? 16 : 32));
     return CLOWNS_OK;
 }
 
 extern "C" int clowns_solver_index_bits(clowns_solver *h) {
     if (!h || h->N < 1) return -CLOWNS_EBADARG;
+    int f;
+    if (h->ll8_ok && (h_ll8 == 1 || (stored_forms(h, op64(h), &f) && (f & 4)))) return 8;
     return h->ll16_ok ? 16 : 32;
 }
 

@@ -151,7 +151,7 @@ def bench_will(args):
     sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
     if args.profile_kernels:
         sol.solve_will(t(s["B"]), eps=args.eps, iters=1000)
-        ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
+        ix = "Ix%d" % sol.index_bits()
         st = sol.tuning()["stream64"]
         sp, sm = ("true" if st > 0 and st & 2 else "false"), ("true" if st > 0 and st & 1 else "false")
         res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
@@ -180,7 +180,7 @@ def bench_will(args):
     t_spmv = sol.time_kernel(0, 50)
     t_pm = sol.time_kernel(1, 50)
     t_copy = sol.time_kernel(4, 50)
-    ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
+    ix = "Ix%d" % sol.index_bits()
     st = sol.tuning()["stream64"]
     sp, sm = ("true" if st > 0 and st & 2 else "false"), ("true" if st > 0 and st & 1 else "false")
     if 2 * t_spmv >= 8 * t_pm:
@@ -469,15 +469,15 @@ def main():
         dg.assemble_into(sol, b=b)
         sol.solve(b, x, eps=args.eps)
         sol.solve(b, x, eps=args.eps, mixed=True)
-        ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
         tune = sol.tuning()
+        ix, ix32 = "Ix%d" % tune["index64"], "Ix%d" % tune["index32"]
         st = {p: ("true" if tune["stream" + p] > 0 and tune["stream" + p] & 1 else "false",
                   "true" if tune["stream" + p] > 0 and tune["stream" + p] & 2 else "false") for p in ("64", "32")}
         res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
                ((f"k_spmv_v2<double,6,0,{ix},{st['64'][1]}>", 0), (f"k_pm_v2<double,1,0,{ix},{st['64'][0]}>", 1),
                 ("k_copy<double>", 4), (f"k_pm2_v2<double,{ix},{st['64'][0]}>", 5),
-                (f"k_spmv_v2<float,6,0,{ix},{st['32'][1]}>", 2), (f"k_pm_v2<float,1,0,{ix},{st['32'][0]}>", 3),
-                (f"k_pm2_v2<float,{ix},{st['32'][0]}>", 6))}
+                (f"k_spmv_v2<float,6,0,{ix32},{st['32'][1]}>", 2), (f"k_pm_v2<float,1,0,{ix32},{st['32'][0]}>", 3),
+                (f"k_pm2_v2<float,{ix32},{st['32'][0]}>", 6))}
         print(json.dumps({"rows": N, "avg_us": res, "tuning": tune}))
         return
 
@@ -532,7 +532,7 @@ def main():
     t_pm = sol.time_kernel(1, 50)
     t_pair = sol.time_kernel(5, 50)
     t_copy = sol.time_kernel(4, 50)
-    ix = "Ix16" if sol.index_bits() == 16 else "Ix32"
+    ix = "Ix%d" % sol.index_bits()
     tune = sol.tuning()
     st_pm = "true" if tune["stream64"] > 0 and tune["stream64"] & 1 else "false"
     st_sp = "true" if tune["stream64"] > 0 and tune["stream64"] & 2 else "false"

@@ -376,14 +376,19 @@ int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid_dev *g,
 /* The per-operator kernel forms the solver chose for the resident operator
  * when it first solved at this size: out[0], out[1] pairwise-fused PM
  * stages (FP64, FP32: 1 fused, 0 single); out[2], out[3] streamed
- * (non-temporal) arrays (FP64, FP32: bit 0 the PM stages, bit 1 the SpMV).
- * -1 where not yet decided.  No counterpart in the reference (reporting). */
-int clowns_solver_tuning(clowns_solver *h, int32_t out[4]);
+ * (non-temporal) arrays (FP64, FP32: bit 0 the PM stages, bit 1 the SpMV);
+ * out[4], out[5] the width of the neighbour-index codes (FP64, FP32: 8, 16
+ * or 32).  -1 where not yet decided.  No counterpart in the reference
+ * (reporting). */
+int clowns_solver_tuning(clowns_solver *h, int32_t out[6]);
 
-/* Width of the neighbour indices the resident operator's solve kernels read:
- * 16 when the sorted LL was coded in 16 bits (every offset fits; always for
- * box grids of up to 16383 cells a line), else 32; negative CLOWNS_E* on
- * error.  No counterpart in the reference (diagnostics). */
+/* Width of the neighbour indices the resident operator's FP64 solve kernels
+ * read: 8 when they use the 8-bit codes (per 512-row block, four offset
+ * classes with 6-bit steps; the few slots that do not fit read the 32-bit
+ * LL; chosen per operator when it is first solved, or forced), 16 for the
+ * 16-bit codes (every offset fits; always for box grids of up to 16383
+ * cells a line), else 32; negative CLOWNS_E* on error.  No counterpart in
+ * the reference (diagnostics). */
 int clowns_solver_index_bits(clowns_solver *h);
 
 /* Library build string, e.g. "clowns_pcg gfx950 ...". */
