@@ -26,6 +26,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace clowns {
 
@@ -152,6 +153,15 @@ k_to_f32(int N, int lda, int NL, const double *__restrict__ AL,
     for (int k = 0; k < NL; ++k)
         AL32[(size_t)k * lda + i] = (float)AL[(size_t)k * lda + i];
     D32[i] = (float)D[i];
+}
+
+// Half-precision copy of the scaled, sorted PM coefficients (the mixed
+// solve's inner preconditioner, ldc2).
+__global__ void __launch_bounds__(BS)
+k_to_f16(int N, int lda, int NL, const double *__restrict__ AL, _Float16 *__restrict__ ALh) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= N) return;
+    for (int k = 0; k < NL; ++k) ALh[(size_t)k * lda + i] = (_Float16)(float)AL[(size_t)k * lda + i];
 }
 
 // b_s = W o b (pcgpme_para.F90:92); x0 into the padded solve vector.
@@ -362,6 +372,30 @@ __device__ __forceinline__ void sts2(T *p, typename V2<T>::t o) {
         __builtin_nontemporal_store(v, reinterpret_cast<typename NV2<T>::t *>(p));
     else
         *reinterpret_cast<typename NV2<T>::t *>(p) = v;
+}
+
+// Two consecutive PM coefficients as T.  C = T, or C = _Float16: the
+// half-precision copy of the preconditioner's coefficients that the FP32
+// inner solve of the mixed form may use (the preconditioner there need only
+// approximate the FP64 one: the outer FP64 loop measures the true residual
+// with the exact one).
+template <typename T, bool ST, typename C>
+__device__ __forceinline__ typename V2<T>::t ldc2(const C *p) {
+    if constexpr (std::is_same<C, T>::value) {
+        return lds2<ST>(p);
+    } else {
+        typedef C c2 __attribute__((ext_vector_type(2)));
+        const c2 *q = reinterpret_cast<const c2 *>(p);
+        c2 v;
+        if (ST)
+            v = __builtin_nontemporal_load(q);
+        else
+            v = *q;
+        typename V2<T>::t o;
+        o.x = (T)v.x;
+        o.y = (T)v.y;
+        return o;
+    }
 }
 
 // Neighbour index sources for the two-row kernels, 0-based, ghosts (the
@@ -616,23 +650,23 @@ k_ll8_build(int N, int lda, int NL, const int *__restrict__ LL, uint8_t *__restr
     if (threadIdx.x == 0) base[blockIdx.x] = b;
 }
 
-template <typename T, int NT, int DOT, typename IX, bool ST>
+template <typename T, int NT, int DOT, typename IX, bool ST, typename C = T>
 __global__ void __launch_bounds__(BS)
-k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
-        const T *__restrict__ a1, IX l1, const T *__restrict__ in,
+k_pm_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
+        const C *__restrict__ a1, IX l1, const T *__restrict__ in,
         T *__restrict__ out, const T *__restrict__ w, double *__restrict__ part) {
     if (sc->done) return;
     const int lb = logical_block();
     const int i = (lb * BS + threadIdx.x) * 2;
     double s0 = 0.0, s1 = 0.0;
     if (i + 1 < N) {
-        const auto c0 = lds2<ST>(a0 + i);
+        const auto c0 = ldc2<T, ST>(a0 + i);
         const int2 j0 = l0.template pair<ST>(i);
         const auto xi = ld2(in + i);
         T o0 = xi.x - c0.x * in[j0.x];
         T o1 = xi.y - c0.y * in[j0.y];
         if (NT == 2) {
-            const auto c1 = lds2<ST>(a1 + i);
+            const auto c1 = ldc2<T, ST>(a1 + i);
             const int2 j1 = l1.template pair<ST>(i);
             o0 = o0 - c1.x * in[j1.x];
             o1 = o1 - c1.y * in[j1.y];
@@ -655,8 +689,8 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
             s0 += (double)o1 * (double)o1;
         }
     } else if (i < N) {
-        T o = in[i] - a0[i] * in[l0.one(i)];
-        if (NT == 2) o = o - a1[i] * in[l1.one(i)];
+        T o = in[i] - (T)a0[i] * in[l0.one(i)];
+        if (NT == 2) o = o - (T)a1[i] * in[l1.one(i)];
         out[i] = o;
         if (DOT == 1 || DOT == 2) s0 += (double)w[i] * (double)o;
         if (DOT == 2) s1 += (double)o * (double)o;
@@ -673,23 +707,23 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
     }
 }
 
-template <typename T, typename IX, bool ST>
+template <typename T, typename IX, bool ST, typename C = T>
 __global__ void __launch_bounds__(BS)
-k_pm2_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
-         const T *__restrict__ a1, IX l1, const T *__restrict__ in, T *__restrict__ out) {
+k_pm2_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
+         const C *__restrict__ a1, IX l1, const T *__restrict__ in, T *__restrict__ out) {
     if (sc->done) return;
     const int i = (logical_block() * BS + threadIdx.x) * 2;
     // stage 1 at row r, and at its stage-2 neighbour m (a ghost m reads 0)
     auto one = [&](T c0, int jj0, T c1, int m, T xr) -> T {
         const T o1i = xr - c0 * in[jj0];
         T o1m = T(0);
-        if (m < N) o1m = in[m] - a0[m] * in[l0.one(m)];
+        if (m < N) o1m = in[m] - (T)a0[m] * in[l0.one(m)];
         return o1i - c1 * o1m;
     };
     if (i + 1 < N) {
-        const auto c0 = lds2<ST>(a0 + i);
+        const auto c0 = ldc2<T, ST>(a0 + i);
         const int2 j0 = l0.template pair<ST>(i);
-        const auto c1 = lds2<ST>(a1 + i);
+        const auto c1 = ldc2<T, ST>(a1 + i);
         const int2 j1 = l1.template pair<ST>(i);
         const auto xi = ld2(in + i);
         typename V2<T>::t o;
@@ -697,7 +731,7 @@ k_pm2_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ a0, IX l0,
         o.y = one(c0.y, j0.y, c1.y, j1.y, xi.y);
         sts2<ST>(out + i, o);
     } else if (i < N) {
-        out[i] = one(a0[i], l0.one(i), a1[i], l1.one(i), in[i]);
+        out[i] = one((T)a0[i], l0.one(i), (T)a1[i], l1.one(i), in[i]);
     }
 }
 

@@ -103,7 +103,9 @@ struct clowns_solver {
     bool ll8_ok = false;
     int ll8_wide = 0;      // slots read from the 32-bit LL (LL8_WIDE)
     DevBuf<float> AL32, D32;
+    DevBuf<_Float16> ALh;  // half-precision PM coefficients of the FP32 inner solve (h_pm_half)
     bool have32 = false;
+    bool have_h = false;   // ALh built with the FP32 copy
     // Transposed operator (adjoint), scaled + sorted the same way.
     DevBuf<double> ALt, Dt, Wt;
     DevBuf<int> LLt;
@@ -210,6 +212,7 @@ template <typename T> struct Op {
     const uint8_t *LL8 = nullptr; // 8-bit coded LL (two-row kernels), or null
     const int4 *base8 = nullptr;
     bool ix8 = false;             // the kernels read LL8 (tune_stream), else LL16
+    const _Float16 *ALh = nullptr;  // FP32 inner solve: half-precision PM coefficients, or null
 };
 
 // The index source of column j for the two-row kernels.
@@ -260,30 +263,51 @@ bool h_ll16 = true;
 // cache), so tune_stream decides per operator.  CLOWNS_LL8 = 0 off, 1 on,
 // -1 (default) timed.
 int h_ll8 = -1;
+// The FP32 inner solve of the mixed form applies the PM preconditioner with
+// half-precision coefficients (k_to_f16): the outer FP64 loop measures the
+// true residual with the exact FP64 preconditioner, so this changes only how
+// much each refinement gains (by about |M~ - M| |M~^-1|, some 1e-3 against
+// the inner tolerance 1e-2), not the accuracy of the result; the inner
+// operator keeps FP32.  CLOWNS_PM_HALF = 0 keeps FP32 coefficients.
+bool h_pm_half = true;
+
+// Two-row PM launches for coefficient type C (T, or _Float16: the FP32
+// inner solve's half-precision preconditioner) and the operator's index form.
+template <typename T, typename C, typename IX>
+void launch_pm2_ix(hipStream_t s, int G2, const Scal *sc, const Op<T> &A, const C *a0, IX x0, const C *a1, IX x1,
+                   const T *in, T *out) {
+    if (A.st_pm)
+        k_pm2_v2<T, IX, true, C><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out);
+    else
+        k_pm2_v2<T, IX, false, C><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out);
+}
+
+template <typename T, typename C>
+void launch_pm2_c(hipStream_t s, const Scal *sc, const Op<T> &A, const C *AL, int j, const T *in, T *out) {
+    const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
+    const int G2 = geo_of(A.N, 2).G;
+    if (has8(A))
+        launch_pm2_ix<T, C>(s, G2, sc, A, AL + o0, ix8(A, o0), AL + o1, ix8(A, o1), in, out);
+    else if (has16(A))
+        launch_pm2_ix<T, C>(s, G2, sc, A, AL + o0, ix16(A, o0), AL + o1, ix16(A, o1), in, out);
+    else
+        launch_pm2_ix<T, C>(s, G2, sc, A, AL + o0, ix32(A, o0), AL + o1, ix32(A, o1), in, out);
+}
 
 template <typename T>
 void launch_pm2(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j, const T *in, T *out) {
-    const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
-    const int G2 = geo_of(A.N, 2).G;
-    if (use_v2<T>() && has8(A)) {
-        if (A.st_pm)
-            k_pm2_v2<T, Ix8, true><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix8(A, o0), A.AL + o1, ix8(A, o1), in, out);
-        else
-            k_pm2_v2<T, Ix8, false><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix8(A, o0), A.AL + o1, ix8(A, o1), in, out);
-    } else if (use_v2<T>() && has16(A)) {
-        if (A.st_pm)
-            k_pm2_v2<T, Ix16, true><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix16(A, o0), A.AL + o1, ix16(A, o1), in, out);
-        else
-            k_pm2_v2<T, Ix16, false><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix16(A, o0), A.AL + o1, ix16(A, o1), in,
-                                                       out);
-    } else if (use_v2<T>()) {
-        if (A.st_pm)
-            k_pm2_v2<T, Ix32, true><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix32(A, o0), A.AL + o1, ix32(A, o1), in, out);
-        else
-            k_pm2_v2<T, Ix32, false><<<G2, BS, 0, s>>>(sc, A.N, A.AL + o0, ix32(A, o0), A.AL + o1, ix32(A, o1), in,
-                                                       out);
-    } else
+    if (!use_v2<T>()) {
+        const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
         k_pm2<T><<<g.G, BS, 0, s>>>(sc, A.N, g.R, A.AL + o0, A.LL + o0, A.AL + o1, A.LL + o1, in, out);
+        return;
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (A.ALh) {
+            launch_pm2_c<T, _Float16>(s, sc, A, A.ALh, j, in, out);
+            return;
+        }
+    }
+    launch_pm2_c<T, T>(s, sc, A, A.AL, j, in, out);
 }
 
 template <typename T, int NL>
@@ -305,48 +329,57 @@ void launch_spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
     }
 }
 
-template <typename T, int NT, typename IX, bool ST>
-void launch_pm_v2(hipStream_t s, int G2, const Scal *sc, int N, const T *a0, IX x0, const T *a1, IX x1,
+template <typename T, int NT, typename IX, bool ST, typename C>
+void launch_pm_v2(hipStream_t s, int G2, const Scal *sc, int N, const C *a0, IX x0, const C *a1, IX x1,
                   const T *in, T *out, int dot, const T *w, double *part) {
     switch (dot) {
-    case 0: k_pm_v2<T, NT, 0, IX, ST><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
-    case 1: k_pm_v2<T, NT, 1, IX, ST><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
-    case 2: k_pm_v2<T, NT, 2, IX, ST><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
-    default: k_pm_v2<T, NT, 3, IX, ST><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    case 0: k_pm_v2<T, NT, 0, IX, ST, C><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    case 1: k_pm_v2<T, NT, 1, IX, ST, C><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    case 2: k_pm_v2<T, NT, 2, IX, ST, C><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    default: k_pm_v2<T, NT, 3, IX, ST, C><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
     }
+}
+
+template <typename T, int NT, typename C, typename IX>
+void launch_pm_stage_ix(hipStream_t s, int G2, const Scal *sc, const Op<T> &A, const C *a0, IX x0, const C *a1,
+                        IX x1, const T *in, T *out, int dot, const T *w, double *part) {
+    if (A.st_pm)
+        launch_pm_v2<T, NT, IX, true, C>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
+    else
+        launch_pm_v2<T, NT, IX, false, C>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
+}
+
+template <typename T, int NT, typename C>
+int launch_pm_stage_c(hipStream_t s, const Scal *sc, const Op<T> &A, const C *AL, int j, const T *in, T *out,
+                      int dot, const T *w, double *part) {
+    const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
+    const C *a0 = AL + o0, *a1 = NT == 2 ? AL + o1 : nullptr;
+    const int G2 = geo_of(A.N, 2).G;
+    if (has8(A)) {
+        const Ix8 x0 = ix8(A, o0), x1 = NT == 2 ? ix8(A, o1) : x0;
+        launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
+    } else if (has16(A)) {
+        const Ix16 x0 = ix16(A, o0), x1 = NT == 2 ? ix16(A, o1) : x0;
+        launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
+    } else {
+        const Ix32 x0 = ix32(A, o0), x1 = NT == 2 ? ix32(A, o1) : x0;
+        launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
+    }
+    return G2;
 }
 
 template <typename T, int NT>
 int launch_pm_stage(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j,
                     const T *in, T *out, int dot, const T *w, double *part) {
+    if (use_v2<T>()) {
+        if constexpr (sizeof(T) == 4) {
+            if (A.ALh) return launch_pm_stage_c<T, NT, _Float16>(s, sc, A, A.ALh, j, in, out, dot, w, part);
+        }
+        return launch_pm_stage_c<T, NT, T>(s, sc, A, A.AL, j, in, out, dot, w, part);
+    }
     const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
     const T *a0 = A.AL + o0, *a1 = NT == 2 ? A.AL + o1 : nullptr;
     const int *l0 = A.LL + o0, *l1 = NT == 2 ? A.LL + o1 : nullptr;
-    if (use_v2<T>()) {
-        const int G2 = geo_of(A.N, 2).G;
-        if (has8(A)) {
-            const Ix8 x0 = ix8(A, o0), x1 = NT == 2 ? ix8(A, o1) : x0;
-            if (A.st_pm)
-                launch_pm_v2<T, NT, Ix8, true>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
-            else
-                launch_pm_v2<T, NT, Ix8, false>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
-            return G2;
-        }
-        if (has16(A)) {
-            const Ix16 x0 = ix16(A, o0), x1 = NT == 2 ? ix16(A, o1) : x0;
-            if (A.st_pm)
-                launch_pm_v2<T, NT, Ix16, true>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
-            else
-                launch_pm_v2<T, NT, Ix16, false>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
-            return G2;
-        }
-        const Ix32 x0 = ix32(A, o0), x1 = NT == 2 ? ix32(A, o1) : x0;
-        if (A.st_pm)
-            launch_pm_v2<T, NT, Ix32, true>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
-        else
-            launch_pm_v2<T, NT, Ix32, false>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
-        return G2;
-    }
     switch (dot) {
     case 0: k_pm<T, NT, 0><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
     case 1: k_pm<T, NT, 1><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
@@ -753,6 +786,11 @@ int ensure_f32(clowns_solver *h) {
     if ((rc = h->AL32.ensure(nm)) || (rc = h->D32.ensure(h->N))) return rc;
     k_to_f32<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->NL, h->AL.p, h->D.p,
                                                    h->AL32.p, h->D32.p);
+    if (h_pm_half) {
+        if ((rc = h->ALh.ensure(nm))) return rc;
+        k_to_f16<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->NL, h->AL.p, h->ALh.p);
+    }
+    h->have_h = h_pm_half;
     HIPCHK(hipGetLastError());
     h->have32 = true;
     return CLOWNS_OK;
@@ -820,6 +858,7 @@ Op<double> op64(const clowns_solver *h) {
 }
 Op<float> op32(const clowns_solver *h) {
     Op<float> A{h->N, h->NL, h->lda, h->AL32.p, h->LL.p, h->D32.p};
+    if (h->have_h) A.ALh = h->ALh.p;
     if (h->ll16_ok) {
         A.LL16 = h->LL16.p;
         A.base16 = h->base16.p;
@@ -1053,6 +1092,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2VEC")) h_v2_vec = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_LL16")) h_ll16 = atoi(e) != 0;
+    if (const char *e = getenv("CLOWNS_PM_HALF")) h_pm_half = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_LL8")) h_ll8 = atoi(e) < 0 ? -1 : (atoi(e) != 0 ? 1 : 0);
     if (const char *e = getenv("CLOWNS_VERBOSE")) h_verbose = atoi(e) != 0;
     *out = h;
@@ -1064,7 +1104,7 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     h->AL.release(); h->D.release(); h->W.release(); h->LL.release(); h->LL16.release(); h->base16.release(); h->LL8.release(); h->base8.release();
-    h->AL32.release(); h->D32.release();
+    h->AL32.release(); h->D32.release(); h->ALh.release();
     h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
     h->AL0.release(); h->D0.release(); h->LL0.release();
     h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->will_flags.release(); h->will_cand.release(); h->will_rev.release(); h->cls_scratch.release(); h->pc_b.release(); h->pc_x.release(); h->pc_part.release(); h->sc.release();
