@@ -79,6 +79,7 @@ SIGNATURES = {
     "clowns_will_filter": (ctypes.c_int, [_P, _P, _I, _I, _I, _P, _P, _P, _P, ctypes.POINTER(_I)]),
     "clowns_solver_index_bits": (ctypes.c_int, [_P]),
     "clowns_solver_tuning": (ctypes.c_int, [_P, _P]),
+    "clowns_solver_set_block_order": (ctypes.c_int, [_P, _P, _I]),
     "clowns_set_dx": (ctypes.c_int, [_P, _I, _P]),
     "clowns_deck_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P]),
     "clowns_deck_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _P] + [_P] * 8),

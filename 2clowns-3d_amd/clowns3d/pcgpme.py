@@ -134,6 +134,18 @@ class Solver:
         neighbour indices, else 32."""
         return L.lib().clowns_solver_index_bits(self._h)
 
+    def set_block_order(self, order=None):
+        """Order of the 512-row blocks in the resident operator's SpMV and PM
+        kernels (clowns_solver_set_block_order): an int32 CUDA tensor holding
+        a permutation of 0..G-1, G = ceil(N/512) rounded up to 8; None for
+        the natural order.  Results are bit-identical either way."""
+        if order is None:
+            L.check(L.lib().clowns_solver_set_block_order(self._h, None, 0), "clowns_solver_set_block_order")
+            return
+        order = order.contiguous()
+        L.check(L.lib().clowns_solver_set_block_order(self._h, _dptr(order), int(order.numel())),
+                "clowns_solver_set_block_order")
+
     def tuning(self):
         """Kernel forms chosen for the resident operator (clowns_solver_tuning):
         dict fuse64, fuse32 (1 fused PM pairs), stream64, stream32 (bit 0

@@ -57,6 +57,16 @@ __device__ __forceinline__ int logical_block() {
     return (b & 7) * (gridDim.x >> 3) + (b >> 3);
 }
 
+// Row block of a gather kernel's workgroup: the XCD-contiguous logical block,
+// mapped through `order` (a permutation of the row blocks) when the solver
+// runs the operator in a block order of its own.  Results do not depend on
+// it: every row's arithmetic, and the dot partials' index, follow the row
+// block.
+__device__ __forceinline__ int row_block(const int *order) {
+    const int lb = logical_block();
+    return order ? order[lb] : lb;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -654,9 +664,10 @@ template <typename T, int NT, int DOT, typename IX, bool ST, typename C = T>
 __global__ void __launch_bounds__(BS)
 k_pm_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
         const C *__restrict__ a1, IX l1, const T *__restrict__ in,
-        T *__restrict__ out, const T *__restrict__ w, double *__restrict__ part) {
+        T *__restrict__ out, const T *__restrict__ w, double *__restrict__ part,
+        const int *__restrict__ order) {
     if (sc->done) return;
-    const int lb = logical_block();
+    const int lb = row_block(order);
     const int i = (lb * BS + threadIdx.x) * 2;
     double s0 = 0.0, s1 = 0.0;
     if (i + 1 < N) {
@@ -710,9 +721,10 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
 template <typename T, typename IX, bool ST, typename C = T>
 __global__ void __launch_bounds__(BS)
 k_pm2_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
-         const C *__restrict__ a1, IX l1, const T *__restrict__ in, T *__restrict__ out) {
+         const C *__restrict__ a1, IX l1, const T *__restrict__ in, T *__restrict__ out,
+         const int *__restrict__ order) {
     if (sc->done) return;
-    const int i = (logical_block() * BS + threadIdx.x) * 2;
+    const int i = (row_block(order) * BS + threadIdx.x) * 2;
     // stage 1 at row r, and at its stage-2 neighbour m (a ghost m reads 0)
     auto one = [&](T c0, int jj0, T c1, int m, T xr) -> T {
         const T o1i = xr - c0 * in[jj0];
@@ -740,7 +752,8 @@ template <typename T, int NL, int MODE, typename IX, bool ST>
 __global__ void __launch_bounds__(BS)
 k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL, IX LL,
           const T *__restrict__ D, const T *__restrict__ v0, const T *__restrict__ v1,
-          const T *__restrict__ v2, T *__restrict__ vout, T *__restrict__ y) {
+          const T *__restrict__ v2, T *__restrict__ vout, T *__restrict__ y,
+          const int *__restrict__ order) {
     if (sc->done) return;
     T c0 = T(0), c1 = T(0);
     if (MODE == 1) {
@@ -754,7 +767,7 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
         if (MODE == 2) return v1[j] - c0 * v2[j];
         return v0[j];
     };
-    const int i = (logical_block() * BS + threadIdx.x) * 2;
+    const int i = (row_block(order) * BS + threadIdx.x) * 2;
     if (i + 1 < N) {
         typename V2<T>::t a[NL];
         int2 l[NL];

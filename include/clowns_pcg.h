@@ -382,6 +382,17 @@ int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid_dev *g,
  * (reporting). */
 int clowns_solver_tuning(clowns_solver *h, int32_t out[6]);
 
+/* Run the resident operator's gather kernels (SpMV, PM stages) over their
+ * 512-row blocks in the given order: d_order[G] in device memory, a
+ * permutation of 0..G-1 with G = ceil(N/512) rounded up to a multiple of 8
+ * (the kernels' block count; checked).  Results are unchanged bit for bit;
+ * only which rows run close together in time changes, and with it how often
+ * a row's +-plane neighbours are still in L2.  d_order = NULL or G = 0
+ * restores the natural order; an order whose G no longer matches the
+ * resident operator is ignored.  No counterpart in the reference
+ * (scheduling). */
+int clowns_solver_set_block_order(clowns_solver *h, const int32_t *d_order, int32_t G);
+
 /* Width of the neighbour indices the resident operator's FP64 solve kernels
  * read: 8 when they use the 8-bit codes (per 512-row block, four offset
  * classes with 6-bit steps; the few slots that do not fit read the 32-bit

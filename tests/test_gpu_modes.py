@@ -3,7 +3,8 @@
 Mixed: FP32 inner PM-Bi-CGSTAB + FP64 refinement must reach the reference's
 stopping value (sqrt(RN/BN) <= 1e-10, here on the true residual) and be as
 accurate as the FP64 solve: error against a tight FP64 solve within
-max(1e-9, 2x the FP64 solve's).  Adjoint: lambda = A^-T g must
+max(1e-9, 2x the FP64 solve's), with the inner preconditioner's
+coefficients in half precision and in FP32.  Adjoint: lambda = A^-T g must
 satisfy A^T lambda = g (checked with a host sparse product of the unscaled
 transpose) and the duality <g, A^-1 b> = <A^-T g, b>.
 """
@@ -15,12 +16,20 @@ from _systems import box_system
 pytestmark = pytest.mark.gpu
 
 
-def _resident(s):
+def _resident(s, pm_half=None):
+    import os
     import torch
     from clowns3d import Solver
     dev = torch.device("cuda:0")
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    sol = Solver(0)
+    if pm_half is None:
+        sol = Solver(0)
+    else:
+        os.environ["CLOWNS_PM_HALF"] = "1" if pm_half else "0"
+        try:
+            sol = Solver(0)  # reads CLOWNS_PM_HALF at creation
+        finally:
+            del os.environ["CLOWNS_PM_HALF"]
     sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
     return sol, t
 
@@ -35,10 +44,13 @@ def apply_AT(s, y):
     return out
 
 
+# pm_half: the inner preconditioner's coefficients in half precision (the
+# default) or FP32; the last case restores the default for later solvers
+@pytest.mark.parametrize("pm_half", [False, True])
 @pytest.mark.parametrize("shape", [(33, 17, 9), (64, 48, 16)])
-def test_mixed_matches_fp64(shape):
+def test_mixed_matches_fp64(shape, pm_half):
     s = box_system(*shape, seed=11)
-    sol, t = _resident(s)
+    sol, t = _resident(s, pm_half)
     b = t(s["B"])
     x64, i64 = sol.solve(b)
     x32, i32 = sol.solve(b, mixed=True)
