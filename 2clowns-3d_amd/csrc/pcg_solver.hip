@@ -1563,7 +1563,10 @@ extern "C" int clowns_solver_solve_transpose(clowns_solver *h, const double *d_g
     if ((rc = ensure_transpose(h))) return rc;
     if (eps <= 1e-15) eps = 1e-9;
     if (maxiter <= 0) maxiter = (int)(10.0 * (std::sqrt((double)h->N) + 1.0));
+    if ((rc = ensure_block_order(h))) return rc;
     Op<double> At{h->N, h->NL, h->lda, h->ALt.p, h->LLt.p, h->Dt.p};
+    // the transpose has the forward operator's rows: the same block order
+    if (h->order_G > 0 && h->order_G == geo_of(h->N, 2).G) At.order = h->order.p;
     return solve_fp64(h, At, h->Wt.p, d_g, d_lambda, eps, maxiter,
                       (flags & CLOWNS_WARM_START) != 0, info);
 }

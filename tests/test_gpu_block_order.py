@@ -1,8 +1,8 @@
 """GPU: the order in which the FP64 SpMV and PM kernels visit their 512-row
 blocks (clowns_solver_set_block_order, and the order the resident assembly
 derives from the grid) changes nothing in the results: solves are
-bit-identical to the natural order.  Orders that are not a permutation of
-the kernels' blocks are refused."""
+bit-identical to the natural order, the adjoint's transpose solve included.
+Orders that are not a permutation of the kernels' blocks are refused."""
 import os
 
 import numpy as np
@@ -32,13 +32,15 @@ def test_assembled_order_bit_identical(name):
     out = []
     for on in (False, True):  # the last restores the default for later solvers
         sol = _solver(on)
-        b = dg.assemble_into(sol)
+        b = dg.assemble_into(sol, keep_raw=True)
         x64, i64 = sol.solve(b, eps=1e-10)
         xt, it = sol.solve(b, eps=2e-15, maxiter=3000)
-        out.append((x64.cpu().numpy(), xt.cpu().numpy(), i64["iters"], it["iters"]))
+        lam, il = sol.solve_transpose(b, eps=1e-10)  # the adjoint's solve, same order
+        out.append((x64.cpu().numpy(), xt.cpu().numpy(), lam.cpu().numpy(), i64["iters"], it["iters"], il["iters"]))
         sol.close()
-    assert out[0][2] == out[1][2] and out[0][3] == out[1][3]
-    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][3:] == out[1][3:]
+    for a, c in zip(out[0][:3], out[1][:3]):
+        assert np.array_equal(a, c)
 
 
 def test_set_block_order_bit_identical_and_checked():
