@@ -115,6 +115,11 @@ struct clowns_solver {
     DevBuf<double> ALt, Dt, Wt;
     DevBuf<int> LLt;
     bool have_t = false;
+    DevBuf<short> LLt16;   // its 16- and 8-bit coded LL (build_codes_t)
+    DevBuf<int2> baset16;
+    DevBuf<uint8_t> LLt8;
+    DevBuf<int4> baset8;
+    bool t16_ok = false, t8_ok = false;
     // Caller-layout copy of the unscaled operator (gradients).
     DevBuf<double> AL0, D0;
     DevBuf<int> LL0;
@@ -1129,6 +1134,7 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     h->AL.release(); h->D.release(); h->W.release(); h->LL.release(); h->LL16.release(); h->base16.release(); h->LL8.release(); h->base8.release(); h->order.release(); h->order_first.release();
     h->AL32.release(); h->D32.release(); h->ALh.release();
     h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
+    h->LLt16.release(); h->baset16.release(); h->LLt8.release(); h->baset8.release();
     h->AL0.release(); h->D0.release(); h->LL0.release();
     h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->will_flags.release(); h->will_cand.release(); h->will_rev.release(); h->cls_scratch.release(); h->pc_b.release(); h->pc_x.release(); h->pc_part.release(); h->sc.release();
     if (h->h_sc) (void)hipHostFree(h->h_sc);
@@ -1521,6 +1527,34 @@ extern "C" int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32
 }
 
 namespace {
+// The 16- and 8-bit codes of the transposed operator's sorted LL, as
+// build_ll16 / build_ll8 make them for the forward one.
+int build_codes_t(clowns_solver *h) {
+    h->t16_ok = h->t8_ok = false;
+    if (!h_ll16 || !h_v2_32 || !h_v2_64 || !h_v2_vec) return CLOWNS_OK;
+    const int N = h->N;
+    const size_t nm = (size_t)h->lda * h->NL;
+    int rc;
+    if ((rc = h->LLt16.ensure(nm)) || (rc = h->baset16.ensure(cdiv(N, LL16_ROWS))) || (rc = h->flag.ensure(4)))
+        return rc;
+    const bool want8 = h_ll8 != 0;
+    if (want8 && ((rc = h->LLt8.ensure(nm)) || (rc = h->baset8.ensure(cdiv(N, LL8_ROWS))))) return rc;
+    hipStream_t s = h->stream;
+    fill<int>(s, h->flag.p, 4, 0);
+    k_ll16_build<<<cdiv(N, LL16_ROWS), BS, 0, s>>>(N, h->lda, h->NL, h->LLt.p, h->LLt16.p, h->baset16.p, h->flag.p,
+                                                   h->flag.p + 1);
+    if (want8)
+        k_ll8_build<<<cdiv(N, LL8_ROWS), BS, 0, s>>>(N, h->lda, h->NL, h->LLt.p, h->LLt8.p, h->baset8.p,
+                                                     h->flag.p + 2, h->flag.p + 3);
+    HIPCHK(hipGetLastError());
+    int res[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(res, h->flag.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    h->t16_ok = res[0] == 0;
+    h->t8_ok = want8 && h->t16_ok && res[2] == 0;
+    return CLOWNS_OK;
+}
+
 // Build and scale/sort A^T from the caller-layout copy kept by set_matrix.
 int ensure_transpose(clowns_solver *h) {
     if (h->have_t) return CLOWNS_OK;
@@ -1548,6 +1582,7 @@ int ensure_transpose(clowns_solver *h) {
     else
         k_scale_sort<4><<<cdiv(N, BS), BS, 0, s>>>(N, lda, h->ALt.p, h->LLt.p, h->Dt.p, h->Wt.p, P1, P2);
     HIPCHK(hipGetLastError());
+    if ((rc = build_codes_t(h))) return rc;
     h->have_t = true;
     return CLOWNS_OK;
 }
@@ -1565,6 +1600,14 @@ extern "C" int clowns_solver_solve_transpose(clowns_solver *h, const double *d_g
     if (maxiter <= 0) maxiter = (int)(10.0 * (std::sqrt((double)h->N) + 1.0));
     if ((rc = ensure_block_order(h))) return rc;
     Op<double> At{h->N, h->NL, h->lda, h->ALt.p, h->LLt.p, h->Dt.p};
+    if (h->t16_ok) {
+        At.LL16 = h->LLt16.p;
+        At.base16 = h->baset16.p;
+    }
+    if (h->t8_ok) {
+        At.LL8 = h->LLt8.p;
+        At.base8 = h->baset8.p;
+    }
     // the transpose has the forward operator's rows: the same block order
     if (h->order_G > 0 && h->order_G == geo_of(h->N, 2).G) At.order = h->order.p;
     return solve_fp64(h, At, h->Wt.p, d_g, d_lambda, eps, maxiter,
