@@ -849,6 +849,12 @@ int build_ll8(clowns_solver *h) {
     h->ll8_ok = false;
     h->ll8_wide = 0;
     if (h_ll8 == 0 || !h->ll16_ok) return CLOWNS_OK;
+    // not needed where the FP64 solve was timed at this size and kept the
+    // 16-bit codes, and FP32 kept them too or has not been timed (then it
+    // keeps them as well): an assembly per time step need not rebuild them
+    const bool k64 = h->ix64 == 0 && tuned_size(h->N, h->ix64_N) && h->st64_f == force_key();
+    const bool t32 = h->ix32 >= 0 && tuned_size(h->N, h->ix32_N) && h->st32_f == force_key();
+    if (h_ll8 < 0 && k64 && (!t32 || h->ix32 == 0)) return CLOWNS_OK;
     const int N = h->N, nblk = cdiv(N, LL8_ROWS);
     int rc;
     if ((rc = h->LL8.ensure((size_t)h->lda * h->NL)) || (rc = h->base8.ensure(nblk)) || (rc = h->flag.ensure(2)))
