@@ -240,7 +240,7 @@ template <typename T> Ix8 ix8(const Op<T> &A, size_t off) { return Ix8{A.LL8 + o
 // 2.45M rows, but at 13.1M rows (512x512 planes) the fused FP32 pair takes
 // 83 us against 2 x 31 us.  So each solver times both forms on its operator
 // the first time it solves at a given size (tune_fuse) and keeps the faster.
-// CLOWNS_FUSE_PM=0/1 forces both precisions off/on.
+// CLOWNS_FUSE_PM=0/1 forces both precisions off/on (-1: timed again).
 int h_fuse_pm = -1;
 // Non-temporal loads/stores of the streamed arrays in the PM stages and the
 // SpMV (the ST kernels).  Measured on MI355X, at 13.1M rows FP64 SpMV 224 ->
@@ -1119,7 +1119,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_TAU_MIN")) h->tau_min = std::max(1e-8, atof(e));
     if (const char *e = getenv("CLOWNS_CLASSIFY_HOST")) h->cls_force_host = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_WILL_MAX_ROUNDS")) h->will_max_rounds = std::max(1, atoi(e));
-    if (const char *e = getenv("CLOWNS_FUSE_PM")) h_fuse_pm = atoi(e) != 0 ? 1 : 0;
+    if (const char *e = getenv("CLOWNS_FUSE_PM")) h_fuse_pm = atoi(e) < 0 ? -1 : (atoi(e) != 0 ? 1 : 0);
     if (const char *e = getenv("CLOWNS_STREAM")) h_stream = atoi(e) < 0 ? -1 : (atoi(e) != 0 ? 1 : 0);
     if (const char *e = getenv("CLOWNS_V2_32")) h_v2_32 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;

@@ -1,7 +1,7 @@
-"""GPU: the non-temporal (streamed) forms of the PM stages and the SpMV
-compute exactly what the cached forms do -- FP64, mixed and transposed
-solves bit-identical with the forms forced on and off -- and the solver
-reports the forms it chose."""
+"""GPU: the non-temporal (streamed) forms of the PM stages and the SpMV, and
+the fused PM stage pairs, compute exactly what the cached and single forms
+do -- FP64, mixed (and transposed) solves bit-identical with the forms
+forced on and off -- and the solver reports the forms it chose."""
 import os
 
 import numpy as np
@@ -48,5 +48,39 @@ def test_streamed_forms_bit_identical(shape):
     assert tune_auto["stream64"] in (0, 1, 2, 3) and tune_auto["fuse64"] in (0, 1)
     for a, b, c in zip(on, off, auto):
         assert np.array_equal(a, b) and np.array_equal(a, c)
+    for a, b in zip(info_on, info_off):
+        assert a["iters"] == b["iters"] and a["ier"] == b["ier"] == 0
+
+
+def _solve_fused(s, fuse):
+    import torch
+    from clowns3d import Solver
+    os.environ["CLOWNS_FUSE_PM"] = str(fuse)
+    try:
+        sol = Solver(0)  # reads CLOWNS_FUSE_PM at creation
+    finally:
+        del os.environ["CLOWNS_FUSE_PM"]
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
+    b = t(s["B"])
+    x64, i64 = sol.solve(b, eps=1e-10)
+    x32, i32 = sol.solve(b, eps=1e-10, mixed=True)
+    tune = sol.tuning()
+    sol.close()
+    return [x64.cpu().numpy(), x32.cpu().numpy()], (i64, i32), tune
+
+
+@pytest.mark.parametrize("shape", [(160, 128, 6), (33, 17, 9)])
+def test_fused_pm_pairs_bit_identical(shape):
+    """Two PM stages fused in one launch (k_pm2_v2 recomputes the first stage
+    at the second's gather point) compute what the two single stages do."""
+    s = box_system(*shape, seed=24)
+    on, info_on, tune_on = _solve_fused(s, 1)
+    off, info_off, tune_off = _solve_fused(s, 0)
+    _solve_fused(box_system(6, 5, 4, seed=1), -1)  # back to timing per operator
+    assert tune_on["fuse64"] == 1 and tune_off["fuse64"] == 0
+    for a, b in zip(on, off):
+        assert np.array_equal(a, b)
     for a, b in zip(info_on, info_off):
         assert a["iters"] == b["iters"] and a["ier"] == b["ier"] == 0
