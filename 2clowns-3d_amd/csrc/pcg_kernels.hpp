@@ -718,6 +718,37 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
     }
 }
 
+// FP32 single-term PM stage, four rows per thread (no dot products): twice
+// the bytes in flight per memory instruction of k_pm_v2 -- its FP32 form
+// with half-precision coefficients loads 4 bytes of coefficients a thread.
+// Same per-row arithmetic.
+template <typename IX, bool ST, typename C>
+__global__ void __launch_bounds__(BS)
+k_pm_v4(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0, const float *__restrict__ in,
+        float *__restrict__ out) {
+    if (sc->done) return;
+    const int i = (logical_block() * BS + threadIdx.x) * 4;
+    if (i + 3 < N) {
+        const auto c01 = ldc2<float, ST>(a0 + i);
+        const auto c23 = ldc2<float, ST>(a0 + i + 2);
+        const int2 j01 = l0.template pair<ST>(i);
+        const int2 j23 = l0.template pair<ST>(i + 2);
+        const float4 x = *reinterpret_cast<const float4 *>(in + i);
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        f4 o;
+        o.x = x.x - c01.x * in[j01.x];
+        o.y = x.y - c01.y * in[j01.y];
+        o.z = x.z - c23.x * in[j23.x];
+        o.w = x.w - c23.y * in[j23.y];
+        if (ST)
+            __builtin_nontemporal_store(o, reinterpret_cast<f4 *>(out + i));
+        else
+            *reinterpret_cast<f4 *>(out + i) = o;
+    } else {
+        for (int r = i; r < N && r < i + 4; ++r) out[r] = in[r] - (float)a0[r] * in[l0.one(r)];
+    }
+}
+
 template <typename T, typename IX, bool ST, typename C = T>
 __global__ void __launch_bounds__(BS)
 k_pm2_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,

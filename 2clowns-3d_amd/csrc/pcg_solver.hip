@@ -260,6 +260,13 @@ template <typename T> bool use_v2() { return sizeof(T) == 4 ? h_v2_32 : h_v2_64;
 // Two rows per thread in the SpMV, p update and x/r update (measured
 // separately from the PM stages; CLOWNS_V2VEC = 0/1).
 bool h_v2_vec = true;
+// Four rows per thread in the FP32 single-term PM stages (k_pm_v4): with
+// half-precision coefficients two rows load only 4 bytes of them.  Measured
+// on MI355X: mixed solve 199.9 -> 195.3 ms at 13.1M rows (single stages),
+// unchanged at 2.45M rows (fused pairs).  The same for the FP32 SpMV was
+// slower (201 ms at 13.1M rows, 29.9 vs 27.9 ms at 2.45M).  CLOWNS_V4_32 = 0
+// keeps two rows.
+bool h_v4_32 = true;
 template <typename T> bool use_v2_vec() { return h_v2_vec; }
 // 16-bit coded neighbour indices in the two-row kernels.  Measured on
 // MI355X: single PM stage 5% faster at 2.45M rows (FP64 10.05 vs 10.61 us,
@@ -364,6 +371,16 @@ void launch_pm_v2(hipStream_t s, int G2, const Scal *sc, int N, const C *a0, IX 
 template <typename T, int NT, typename C, typename IX>
 void launch_pm_stage_ix(hipStream_t s, int G2, const Scal *sc, const Op<T> &A, const C *a0, IX x0, const C *a1,
                         IX x1, const T *in, T *out, int dot, const T *w, double *part) {
+    if constexpr (sizeof(T) == 4 && NT == 1) {
+        if (dot == 0 && h_v4_32) {
+            const int G4 = geo_of(A.N, 4).G;
+            if (A.st_pm)
+                k_pm_v4<IX, true, C><<<G4, BS, 0, s>>>(sc, A.N, a0, x0, in, out);
+            else
+                k_pm_v4<IX, false, C><<<G4, BS, 0, s>>>(sc, A.N, a0, x0, in, out);
+            return;
+        }
+    }
     if (A.st_pm)
         launch_pm_v2<T, NT, IX, true, C>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part, A.order);
     else
@@ -1124,6 +1141,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_V2_32")) h_v2_32 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2VEC")) h_v2_vec = atoi(e) != 0;
+    if (const char *e = getenv("CLOWNS_V4_32")) h_v4_32 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_LL16")) h_ll16 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_PM_HALF")) h_pm_half = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_BLOCK_ORDER")) h_block_order = atoi(e) != 0;

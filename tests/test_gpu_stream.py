@@ -84,3 +84,35 @@ def test_fused_pm_pairs_bit_identical(shape):
         assert np.array_equal(a, b)
     for a, b in zip(info_on, info_off):
         assert a["iters"] == b["iters"] and a["ier"] == b["ier"] == 0
+
+
+def _solve_v4(s, v4):
+    import torch
+    from clowns3d import Solver
+    os.environ["CLOWNS_V4_32"] = str(v4)
+    os.environ["CLOWNS_FUSE_PM"] = "0"  # single stages, the ones with four rows a thread
+    try:
+        sol = Solver(0)  # reads both at creation
+    finally:
+        del os.environ["CLOWNS_V4_32"]
+        os.environ["CLOWNS_FUSE_PM"] = "-1"
+        Solver(0).close()  # the pair/single choice timed again from here on
+        del os.environ["CLOWNS_FUSE_PM"]
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
+    x32, i32 = sol.solve(t(s["B"]), eps=1e-10, mixed=True)
+    sol.close()
+    return x32.cpu().numpy(), i32
+
+
+@pytest.mark.parametrize("shape", [(160, 128, 6), (33, 17, 9)])
+def test_four_row_fp32_stages_bit_identical(shape):
+    """The FP32 single PM stages with four rows a thread (k_pm_v4, 5049 rows:
+    a ragged tail) compute what two rows a thread do."""
+    s = box_system(*shape, seed=25)
+    a, ia = _solve_v4(s, 1)
+    b, ib = _solve_v4(s, 0)
+    _solve_v4(box_system(6, 5, 4, seed=1), 1)  # the default again
+    assert ia["iters"] == ib["iters"] and ia["ier"] == ib["ier"] == 0
+    assert np.array_equal(a, b)
