@@ -263,9 +263,9 @@ bool h_v2_vec = true;
 // Four rows per thread in the FP32 single-term PM stages (k_pm_v4): with
 // half-precision coefficients two rows load only 4 bytes of them.  Measured
 // on MI355X: mixed solve 199.9 -> 195.3 ms at 13.1M rows (single stages),
-// unchanged at 2.45M rows (fused pairs).  The same for the FP32 SpMV was
-// slower (201 ms at 13.1M rows, 29.9 vs 27.9 ms at 2.45M).  CLOWNS_V4_32 = 0
-// keeps two rows.
+// unchanged at 2.45M rows (fused pairs).  The same for the FP32 SpMV (201
+// ms at 13.1M rows, 29.9 vs 27.9 ms at 2.45M) and the FP32 fused pair (30.3
+// vs 28.0 ms at 2.45M rows) was slower.  CLOWNS_V4_32 = 0 keeps two rows.
 bool h_v4_32 = true;
 template <typename T> bool use_v2_vec() { return h_v2_vec; }
 // 16-bit coded neighbour indices in the two-row kernels.  Measured on
