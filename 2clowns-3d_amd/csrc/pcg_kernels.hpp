@@ -1000,27 +1000,28 @@ k_init(const Scal *__restrict__ sc, int N, int R, T *__restrict__ wk1,
 // Each thread sums its partials k, k + FIN_BS, k + 2 FIN_BS, ... in that
 // order, then the block tree adds the threads in fixed order.  The partials
 // are few (one per 512 rows) and latency, not bandwidth, bounds this: every
-// thread issues all its loads (up to eight per array, predicated) before the
+// thread issues its loads (FIN_U per array at a time, predicated) before the
 // first add, and the two-array form loads both arrays at once.
+constexpr int FIN_U = 16;  // loads per array and thread issued together
 template <int NA>
 __device__ __forceinline__ void fin_sums(const double *__restrict__ p, int G, double *out) {
     __shared__ double lds[FIN_BS / 64];
     double s[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) s[a] = 0.0;
-    for (int k0 = 0; k0 < G; k0 += 8 * FIN_BS) {
-        double v[NA][8];
+    for (int k0 = 0; k0 < G; k0 += FIN_U * FIN_BS) {
+        double v[NA][FIN_U];
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < FIN_U; ++u) {
                 const int k = k0 + (int)threadIdx.x + u * FIN_BS;
                 v[a][u] = k < G ? p[(size_t)a * G + k] : 0.0;
             }
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int u = 0; u < FIN_U; ++u)
                 if (k0 + (int)threadIdx.x + u * FIN_BS < G) s[a] += v[a][u];
     }
 #pragma unroll
