@@ -853,6 +853,79 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
     }
 }
 
+// Row-major copies for the FP32 SpMV of the mixed inner solve: the six
+// coefficients of a row together (ALr[6 i + k]) and its six 16-bit codes
+// (Lr[6 i + k]).  Two rows are then three 16-byte and three 8-byte loads
+// instead of twelve, the FP32 kernels' pace being set more by the number of
+// memory instructions than by bytes.
+__global__ void __launch_bounds__(BS)
+k_to_rows32(int N, int lda, const double *__restrict__ AL, const short *__restrict__ L16,
+            float *__restrict__ ALr, short *__restrict__ Lr) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= N) return;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        ALr[(size_t)6 * i + k] = (float)AL[(size_t)k * lda + i];
+        Lr[(size_t)6 * i + k] = L16[(size_t)k * lda + i];
+    }
+}
+
+// FP32 y = A x (MODE 0, NL = 6) from the row-major copies, two rows per
+// thread; the codes decode as Ix16 (wide slots read the column-major LL).
+// Same per-row arithmetic as k_spmv_v2.
+template <bool ST>
+__global__ void __launch_bounds__(BS)
+k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restrict__ ALr,
+              const short *__restrict__ Lr, Ix16 LL, const float *__restrict__ D, const float *__restrict__ x,
+              float *__restrict__ y) {
+    if (sc->done) return;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef short s4 __attribute__((ext_vector_type(4)));
+    const int i = (logical_block() * BS + threadIdx.x) * 2;
+    if (i + 1 < N) {
+        const f4 *ap = reinterpret_cast<const f4 *>(ALr + (size_t)6 * i);
+        const s4 *cp = reinterpret_cast<const s4 *>(Lr + (size_t)6 * i);
+        f4 a0, a1, a2;
+        s4 c0, c1, c2;
+        if (ST) {
+            a0 = __builtin_nontemporal_load(ap);
+            a1 = __builtin_nontemporal_load(ap + 1);
+            a2 = __builtin_nontemporal_load(ap + 2);
+            c0 = __builtin_nontemporal_load(cp);
+            c1 = __builtin_nontemporal_load(cp + 1);
+            c2 = __builtin_nontemporal_load(cp + 2);
+        } else {
+            a0 = ap[0];
+            a1 = ap[1];
+            a2 = ap[2];
+            c0 = cp[0];
+            c1 = cp[1];
+            c2 = cp[2];
+        }
+        const float ar[2][6] = {{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y}, {a1.z, a1.w, a2.x, a2.y, a2.z, a2.w}};
+        const int cr[2][6] = {{c0.x, c0.y, c0.z, c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x, c2.y, c2.z, c2.w}};
+        const int2 b = LL.base[i / LL16_ROWS];
+        const float2 d = lds2<ST>(D + i);
+        const float2 xv = ld2(x + i);
+        float acc0 = xv.x * d.x, acc1 = xv.y * d.y;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const Ix16 c = LL.col((size_t)k * lda);
+            acc0 = acc0 + ar[0][k] * x[c.dec(cr[0][k], i, b)];
+            acc1 = acc1 + ar[1][k] * x[c.dec(cr[1][k], i + 1, b)];
+        }
+        float2 o;
+        o.x = acc0;
+        o.y = acc1;
+        sts2<ST>(y + i, o);
+    } else if (i < N) {
+        float acc = x[i] * D[i];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc = acc + ALr[(size_t)6 * i + k] * x[LL.col((size_t)k * lda).one(i)];
+        y[i] = acc;
+    }
+}
+
 // E = R - ALP*Q (pcgpme_para.F90:173) as its own pass, two rows per thread.
 template <typename T>
 __global__ void __launch_bounds__(BS)

@@ -108,6 +108,9 @@ struct clowns_solver {
     bool ll8_ok = false;
     int ll8_wide = 0;      // slots read from the 32-bit LL (LL8_WIDE)
     DevBuf<float> AL32, D32;
+    DevBuf<float> ALr32;   // row-major FP32 coefficients of the FP32 SpMV (h_rows32)
+    DevBuf<short> Lr16;    // row-major 16-bit codes of the same
+    bool have_rows = false;
     DevBuf<_Float16> ALh;  // half-precision PM coefficients of the FP32 inner solve (h_pm_half)
     bool have32 = false;
     bool have_h = false;   // ALh built with the FP32 copy
@@ -224,6 +227,8 @@ template <typename T> struct Op {
     bool ix8 = false;             // the kernels read LL8 (tune_stream), else LL16
     const int *order = nullptr;   // row-block order of the gather kernels, or null
     const _Float16 *ALh = nullptr;  // FP32 inner solve: half-precision PM coefficients, or null
+    const float *ALr = nullptr;     // FP32 SpMV: row-major coefficients and codes, or null
+    const short *Lr = nullptr;
 };
 
 // The index source of column j for the two-row kernels.
@@ -267,6 +272,12 @@ bool h_v2_vec = true;
 // ms at 13.1M rows, 29.9 vs 27.9 ms at 2.45M) and the FP32 fused pair (30.3
 // vs 28.0 ms at 2.45M rows) was slower.  CLOWNS_V4_32 = 0 keeps two rows.
 bool h_v4_32 = true;
+// The FP32 SpMV from row-major copies of its coefficients and 16-bit codes
+// (k_spmv_rows32): three 16-byte and three 8-byte loads for two rows instead
+// of twelve smaller ones.  Measured on MI355X (mixed solve): 27.9 -> 27.0 ms
+// at 2.45M rows, 196 -> 195 ms at 13.1M rows.  With it the FP32 kernels
+// keep 16-bit codes.  CLOWNS_SPMV_ROWS = 0 keeps the column-major operator.
+bool h_rows32 = true;
 template <typename T> bool use_v2_vec() { return h_v2_vec; }
 // 16-bit coded neighbour indices in the two-row kernels.  Measured on
 // MI355X: single PM stage 5% faster at 2.45M rows (FP64 10.05 vs 10.61 us,
@@ -528,6 +539,16 @@ void launch_spmv_v2_st(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int
 template <typename T, int NL, typename IX>
 void launch_spmv_v2(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int mode, const T *v0,
                     const T *v1, const T *v2, T *vout, T *y) {
+    if constexpr (sizeof(T) == 4 && NL == 6 && std::is_same<IX, Ix16>::value) {
+        if (mode == 0 && A.ALr) {
+            const int G2 = geo_of(A.N, 2).G;
+            if (A.st_spmv)
+                k_spmv_rows32<true><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, A.Lr, ix, A.D, v0, y);
+            else
+                k_spmv_rows32<false><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, A.Lr, ix, A.D, v0, y);
+            return;
+        }
+    }
     if (A.st_spmv)
         launch_spmv_v2_st<T, NL, IX, true>(s, sc, A, ix, mode, v0, v1, v2, vout, y);
     else
@@ -832,6 +853,13 @@ int ensure_f32(clowns_solver *h) {
         k_to_f16<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->NL, h->AL.p, h->ALh.p);
     }
     h->have_h = h_pm_half;
+    h->have_rows = false;
+    if (h_rows32 && h->NL == 6 && h->ll16_ok) {
+        if ((rc = h->ALr32.ensure((size_t)6 * h->N)) || (rc = h->Lr16.ensure((size_t)6 * h->N))) return rc;
+        k_to_rows32<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->AL.p, h->LL16.p, h->ALr32.p,
+                                                         h->Lr16.p);
+        h->have_rows = true;
+    }
     HIPCHK(hipGetLastError());
     h->have32 = true;
     return CLOWNS_OK;
@@ -916,6 +944,14 @@ Op<float> op32(const clowns_solver *h) {
     if (h->ll8_ok) {
         A.LL8 = h->LL8.p;
         A.base8 = h->base8.p;
+    }
+    if (h->have_rows) {
+        // the row-major SpMV reads 16-bit codes: the FP32 kernels keep them
+        // (8-bit codes are not offered to the tuner for this operator)
+        A.ALr = h->ALr32.p;
+        A.Lr = h->Lr16.p;
+        A.LL8 = nullptr;
+        A.base8 = nullptr;
     }
     return A;
 }
@@ -1142,6 +1178,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V2VEC")) h_v2_vec = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_V4_32")) h_v4_32 = atoi(e) != 0;
+    if (const char *e = getenv("CLOWNS_SPMV_ROWS")) h_rows32 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_LL16")) h_ll16 = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_PM_HALF")) h_pm_half = atoi(e) != 0;
     if (const char *e = getenv("CLOWNS_BLOCK_ORDER")) h_block_order = atoi(e) != 0;
@@ -1156,7 +1193,7 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     h->AL.release(); h->D.release(); h->W.release(); h->LL.release(); h->LL16.release(); h->base16.release(); h->LL8.release(); h->base8.release(); h->order.release(); h->order_first.release();
-    h->AL32.release(); h->D32.release(); h->ALh.release();
+    h->AL32.release(); h->D32.release(); h->ALh.release(); h->ALr32.release(); h->Lr16.release();
     h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
     h->LLt16.release(); h->baset16.release(); h->LLt8.release(); h->baset8.release();
     h->AL0.release(); h->D0.release(); h->LL0.release();

@@ -116,3 +116,32 @@ def test_four_row_fp32_stages_bit_identical(shape):
     _solve_v4(box_system(6, 5, 4, seed=1), 1)  # the default again
     assert ia["iters"] == ib["iters"] and ia["ier"] == ib["ier"] == 0
     assert np.array_equal(a, b)
+
+
+def _solve_rows(s, rows):
+    import torch
+    from clowns3d import Solver
+    os.environ["CLOWNS_SPMV_ROWS"] = str(rows)
+    try:
+        sol = Solver(0)  # reads CLOWNS_SPMV_ROWS at creation
+    finally:
+        del os.environ["CLOWNS_SPMV_ROWS"]
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
+    x32, i32 = sol.solve(t(s["B"]), eps=1e-10, mixed=True)
+    sol.close()
+    return x32.cpu().numpy(), i32
+
+
+@pytest.mark.parametrize("shape", [(160, 128, 6), (33, 17, 9)])
+def test_row_major_fp32_spmv_bit_identical(shape):
+    """The FP32 SpMV from row-major copies of the coefficients and 16-bit
+    codes (k_spmv_rows32; 5049 rows: an odd tail row) computes what the
+    column-major one does."""
+    s = box_system(*shape, seed=26)
+    a, ia = _solve_rows(s, 1)
+    b, ib = _solve_rows(s, 0)
+    _solve_rows(box_system(6, 5, 4, seed=1), 1)  # the default again
+    assert ia["iters"] == ib["iters"] and ia["ier"] == ib["ier"] == 0
+    assert np.array_equal(a, b)
