@@ -483,12 +483,13 @@ def main():
         # the FP32 inner preconditioner's coefficient type (CLOWNS_PM_HALF)
         c32 = "float" if os.environ.get("CLOWNS_PM_HALF", "1") == "0" else "_Float16"
         v4 = os.environ.get("CLOWNS_V4_32", "1") != "0"  # FP32 single stages: four rows a thread
+        rows = ix32 == "Ix16" and os.environ.get("CLOWNS_SPMV_ROWS", "1") != "0"  # FP32 SpMV, row-major
         st = {p: ("true" if tune["stream" + p] > 0 and tune["stream" + p] & 1 else "false",
                   "true" if tune["stream" + p] > 0 and tune["stream" + p] & 2 else "false") for p in ("64", "32")}
         res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
                ((f"k_spmv_v2<double,6,0,{ix},{st['64'][1]}>", 0), (f"k_pm_v2<double,1,0,{ix},{st['64'][0]},double>", 1),
                 ("k_copy<double>", 4), (f"k_pm2_v2<double,{ix},{st['64'][0]},double>", 5),
-                (f"k_spmv_v2<float,6,0,{ix32},{st['32'][1]}>", 2), (f"k_pm_v4<{ix32},{st['32'][0]},{c32}>" if v4 else f"k_pm_v2<float,1,0,{ix32},{st['32'][0]},{c32}>", 3),
+                (f"k_spmv_rows32<{st['32'][1]}>" if rows else f"k_spmv_v2<float,6,0,{ix32},{st['32'][1]}>", 2), (f"k_pm_v4<{ix32},{st['32'][0]},{c32}>" if v4 else f"k_pm_v2<float,1,0,{ix32},{st['32'][0]},{c32}>", 3),
                 (f"k_pm2_v2<float,{ix32},{st['32'][0]},{c32}>", 6))}
         print(json.dumps({"rows": N, "avg_us": res, "tuning": tune}))
         return
