@@ -2193,6 +2193,7 @@ extern "C" int clowns_solver_set_block_order(clowns_solver *h, const int32_t *d_
     if (!h || h->N < 1 || G < 0) return CLOWNS_EBADARG;
     int rc = set_device(h);
     if (rc) return rc;
+    h->order_pending = false;  // the caller's order replaces the grid's
     if (!d_order || G == 0) {
         h->order_G = 0;
         return CLOWNS_OK;

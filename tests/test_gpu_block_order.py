@@ -71,3 +71,4 @@ def test_set_block_order_bit_identical_and_checked():
     x2, _ = sol.solve(b, eps=1e-10)
     assert np.array_equal(x2.cpu().numpy(), x0)
     sol.close()
+
