@@ -1,8 +1,9 @@
-"""GPU, configuration 2 at full size (256x256x64, 2.45M rows): the kernel
-forms the solver chooses between per operator -- 8-bit against 16-bit
-neighbour codes (70k wide slots here), the grid's block order against the
-natural one -- give bit-identical FP64 solves on the real operator, and the
-solve meets the reference's stopping rule."""
+"""GPU, configurations 2 and 3 at full size (256x256x64, 2.45M rows;
+512x512x96 with bathymetry, 13.1M rows): the kernel forms the solver
+chooses between per operator -- 8-bit against 16-bit neighbour codes (70k
+and 155k wide slots), the grid's block order against the natural one --
+give bit-identical FP64 solves on the real operators, and the solves meet
+the reference's stopping rule."""
 import os
 
 import pytest
@@ -26,11 +27,12 @@ def _solve(dg, env):
     return x, info, bits
 
 
-def test_config2_forms_bit_identical():
+@pytest.mark.parametrize("cfg,rows", [(2, 2449992), (3, 13066133)])
+def test_full_size_forms_bit_identical(cfg, rows):
     import torch
     from clowns3d import DeviceGrid, Solver, configs
-    dg = DeviceGrid(configs.config2())
-    assert dg.N == 2449992
+    dg = DeviceGrid(configs.CONFIGS[cfg]())
+    assert dg.N == rows
     x8, i8, b8 = _solve(dg, {"CLOWNS_LL8": "1", "CLOWNS_BLOCK_ORDER": "1"})
     x16, i16, b16 = _solve(dg, {"CLOWNS_LL8": "0", "CLOWNS_BLOCK_ORDER": "0"})
     # the defaults again for the solvers that follow
