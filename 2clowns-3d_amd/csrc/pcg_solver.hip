@@ -308,6 +308,7 @@ bool h_pm_half = true;
 // slower than no order (the y neighbours then miss); the FP32 kernels gain
 // nothing and keep the natural order.  CLOWNS_BLOCK_ORDER = 0 switches it off.
 constexpr int ORDER_LINES = 64;
+constexpr int ORDER_MIN_ROWS = 1 << 20;
 bool h_block_order = true;
 
 // Two-row PM launches for coefficient type C (T, or _Float16: the FP32
@@ -1742,7 +1743,9 @@ k_block_first(int nb, const int *__restrict__ row2cell, int *__restrict__ out) {
 int note_block_order(clowns_solver *h, const clowns_grid_dev *g) {
     h->order_G = 0;
     h->order_pending = false;
-    if (!h_block_order) return CLOWNS_OK;
+    // below ~1M rows a plane's worth of rows stays in L2 anyway, and building
+    // the order (two host syncs) would only add latency to a short solve
+    if (!h_block_order || h->N < ORDER_MIN_ROWS) return CLOWNS_OK;
     const int nb = cdiv(h->N, 2 * BS);
     int rc;
     if ((rc = h->order_first.ensure(nb))) return rc;

@@ -26,7 +26,9 @@ def _solver(block_order):
 def test_assembled_order_bit_identical(name):
     from clowns3d import DeviceGrid, configs
     from test_oracle_assembly import GRIDS
-    # configuration 1 (64x64x32): 66 grid lines, two chunks of the order
+    # below 1M rows no order is built (a plane stays in L2 anyway): these
+    # check that the switch changes nothing there; configurations 2 and 3
+    # at full size, with the order, are in test_gpu_full_size.py
     g = configs.config1() if name == "config1" else GRIDS[name]()
     dg = DeviceGrid(g)
     out = []
