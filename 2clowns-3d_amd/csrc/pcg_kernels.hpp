@@ -3,14 +3,16 @@
 // Reference: src2LOWNS/pcgpme_para.F90 (PCGPME, LDUSUB, AXSUB).  Every pass
 // of the reference's OpenMP loops maps to one streaming kernel here:
 //   k_scale_sort   row scaling + ghost indices + per-row exchange sort (:77-112)
-//   k_spmv         AXSUB, Y = A*X in sorted slot order (:323-352), with the
-//                  preceding vector update fused (p = r + beta(p - omega q),
-//                  :244; e = r - alpha q, :173) and recomputed at the gather
-//                  points, so the updated vector is never re-read from HBM
-//   k_pm           one stage of LDUSUB (:279-318); the last stage carries the
+//   k_spmv_v2      AXSUB, y = A x in sorted slot order (:323-352), two rows
+//                  a thread; k_spmv_rows32 its FP32 form from row-major
+//                  copies (mixed inner solve)
+//   k_pm_v2        one stage of LDUSUB (:279-318); the last stage carries the
 //                  dot products that follow it ((r^,q) :157-160,
-//                  (e,v),(v,v) :186-191)
-//   k_update       x += alpha p + omega e; r = e - omega v; (r,r), (r^,r)
+//                  (e,v),(v,v) :186-191); k_pm_v4, k_pm2_v2 its FP32
+//                  four-row and fused-pair forms
+//   k_pupd_vec     p = r + beta (p - omega q) (:244)
+//   k_eupd_vec     e = r - alpha q (:173)
+//   k_update_vec   x += alpha p + omega e; r = e - omega v; (r,r), (r^,r)
 //                  (:203-211)
 //   k_fin_*        the scalar steps (alpha, omega, ERR, exits, beta), one
 //                  block, deterministic order
@@ -198,139 +200,6 @@ k_copy(int N, const T *__restrict__ a, T *__restrict__ b) {
     if (i < N) b[i] = a[i];
 }
 
-// ------------------------------------------------------------ SpMV (a6) --
-// MODE 0: x = v0.
-// MODE 1: x = v1 + beta*(v0 - omega*v2)   (p update, r = v1, p = v0, q = v2)
-// MODE 2: x = v1 - alpha*v2                 (e = r - alpha q)
-// MODE 3: y = v1 - A*v0                     (true residual b_s - A x)
-// MODE 1/2 also store x_i in vout.
-template <typename T, int NL, int MODE>
-__global__ void __launch_bounds__(BS)
-k_spmv(const Scal *__restrict__ sc, int N, int lda, int R,
-       const T *__restrict__ AL, const int *__restrict__ LL,
-       const T *__restrict__ D, const T *__restrict__ v0,
-       const T *__restrict__ v1, const T *__restrict__ v2,
-       T *__restrict__ vout, T *__restrict__ y) {
-    if (sc->done) return;
-    T c0 = T(0), c1 = T(0);
-    if (MODE == 1) {
-        c0 = (T)sc->BETA;
-        c1 = (T)sc->AMU;
-    } else if (MODE == 2) {
-        c0 = (T)sc->ALP;
-    }
-    auto X = [&](int j) -> T {
-        if (MODE == 1) return v1[j] + c0 * (v0[j] - c1 * v2[j]);
-        if (MODE == 2) return v1[j] - c0 * v2[j];
-        return v0[j];
-    };
-    const int base = logical_block() * BS * R + threadIdx.x;
-    for (int rr = 0; rr < R; ++rr) {
-        const int i = base + rr * BS;
-        if (i >= N) return;
-        int l[NL];
-        T a[NL];
-#pragma unroll
-        for (int k = 0; k < NL; ++k) {
-            l[k] = LL[(size_t)k * lda + i] - 1;
-            a[k] = AL[(size_t)k * lda + i];
-        }
-        const T xi = X(i);
-        T acc = xi * D[i];
-#pragma unroll
-        for (int k = 0; k < NL; ++k) acc = acc + a[k] * X(l[k]);
-        if (MODE == 1 || MODE == 2) vout[i] = xi;
-        if (MODE == 3)
-            y[i] = v1[i] - acc;
-        else
-            y[i] = acc;
-    }
-}
-
-// -------------------------------------------------------------- PM (a7) --
-// out_i = in_i - a0_i*in[l0_i] (- a1_i*in[l1_i] when NT == 2).
-// DOT 0: none.  DOT 1: part0 += w_i*out_i.  DOT 2: part0 += w_i*out_i,
-// part1 += out_i*out_i.  DOT 3: part0 += out_i*out_i.
-template <typename T, int NT, int DOT>
-__global__ void __launch_bounds__(BS)
-k_pm(const Scal *__restrict__ sc, int N, int R, const T *__restrict__ a0,
-     const int *__restrict__ l0, const T *__restrict__ a1,
-     const int *__restrict__ l1, const T *__restrict__ in,
-     T *__restrict__ out, const T *__restrict__ w, double *__restrict__ part) {
-    if (sc->done) return;
-    double s0 = 0.0, s1 = 0.0;
-    const int lb = logical_block();
-    const int base = lb * BS * R + threadIdx.x;
-    for (int rr = 0; rr < R; ++rr) {
-        const int i = base + rr * BS;
-        if (i >= N) break;
-        T o;
-        if (NT == 2) {
-            const int j0 = l0[i] - 1, j1 = l1[i] - 1;
-            const T c0 = a0[i], c1 = a1[i];
-            o = in[i] - c0 * in[j0] - c1 * in[j1];
-        } else {
-            const int j0 = l0[i] - 1;
-            const T c0 = a0[i];
-            o = in[i] - c0 * in[j0];
-        }
-        out[i] = o;
-        if (DOT == 1 || DOT == 2) s0 += (double)w[i] * (double)o;
-        if (DOT == 2) s1 += (double)o * (double)o;
-        if (DOT == 3) s0 += (double)o * (double)o;
-    }
-    if (DOT != 0) {
-        __shared__ double lds[BS / 64];
-        const double t0 = block_sum<BS>(s0, lds);
-        if (DOT == 2) {
-            const double t1 = block_sum<BS>(s1, lds);
-            if (threadIdx.x == 0) part[gridDim.x + lb] = t1;
-        }
-        if (threadIdx.x == 0) part[lb] = t0;
-    }
-}
-
-// Two consecutive single-term stages in one pass:
-//   o1_i = in_i - a0_i*in[l0_i];  out_i = o1_i - a1_i*o1[l1_i]
-// with o1 at the gather point m = l1_i recomputed from its own row (m a ghost
-// row reads zero, as the reference's zeroed work array W does).  The values
-// are bit-identical to two separate passes; the intermediate never goes to
-// HBM, and row m's a0/l0 entries are L2 hits (m is a grid neighbour).
-template <typename T>
-__global__ void __launch_bounds__(BS)
-k_pm2(const Scal *__restrict__ sc, int N, int R, const T *__restrict__ a0,
-      const int *__restrict__ l0, const T *__restrict__ a1,
-      const int *__restrict__ l1, const T *__restrict__ in,
-      T *__restrict__ out) {
-    if (sc->done) return;
-    const int base = logical_block() * BS * R + threadIdx.x;
-    for (int rr = 0; rr < R; ++rr) {
-        const int i = base + rr * BS;
-        if (i >= N) return;
-        const int j0 = l0[i] - 1, m = l1[i] - 1;
-        const T c0 = a0[i], c1 = a1[i];
-        const T o1i = in[i] - c0 * in[j0];
-        T o1m = T(0);
-        if (m < N) {
-            const int jm = l0[m] - 1;
-            o1m = in[m] - a0[m] * in[jm];
-        }
-        out[i] = o1i - c1 * o1m;
-    }
-}
-
-// P = R + BETA*(P - AMU*Q) (pcgpme_para.F90:244), in place.
-template <typename T>
-__global__ void __launch_bounds__(BS)
-k_pupd(const Scal *__restrict__ sc, int N, T *__restrict__ p,
-       const T *__restrict__ r, const T *__restrict__ q) {
-    if (sc->done) return;
-    const int i = logical_block() * BS + threadIdx.x;
-    if (i >= N) return;
-    const T beta = (T)sc->BETA, amu = (T)sc->AMU;
-    p[i] = r[i] + beta * (p[i] - amu * q[i]);
-}
-
 // ------------------------------------------------ two rows per thread --
 // The same PM stages with each thread owning two consecutive rows and
 // loading both with one 8-byte (FP32) or 16-byte (FP64) access: twice the
@@ -365,6 +234,21 @@ __device__ __forceinline__ typename NV2<T>::t ldn2(const T *p) {
     if (ST) return __builtin_nontemporal_load(q);
     return *q;
 }
+// Any vector (or scalar) the kernel streams once, with the same choice.
+template <bool ST, typename V>
+__device__ __forceinline__ V ldv(const V *p) {
+    if (ST) return __builtin_nontemporal_load(p);
+    return *p;
+}
+template <bool ST, typename V>
+__device__ __forceinline__ void stv(V *p, V v) {
+    if (ST)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 template <bool ST, typename T>
 __device__ __forceinline__ typename V2<T>::t lds2(const T *p) {
     const typename NV2<T>::t v = ldn2<ST>(p);
@@ -425,6 +309,12 @@ struct Ix32 {
         const NV2<int>::t j = ldn2<ST>(L + i);
         return make_int2(j.x - 1, j.y - 1);
     }
+    // four consecutive rows (i a multiple of 4): one 16-byte load
+    template <bool ST = false> __device__ __forceinline__ int4 quad(int i) const {
+        typedef int i4 __attribute__((ext_vector_type(4)));
+        const i4 j = ldv<ST>(reinterpret_cast<const i4 *>(L + i));
+        return make_int4(j.x - 1, j.y - 1, j.z - 1, j.w - 1);
+    }
     __device__ __forceinline__ int one(int i) const { return L[i] - 1; }
 };
 
@@ -433,6 +323,7 @@ constexpr int LL16_GHOST = -32768;
 constexpr int LL16_WIDE = -32767;
 
 struct Ix16 {
+    static constexpr int ROWS = LL16_ROWS;
     const short *L;
     const int *L32;  // the same column, 32-bit (slots coded LL16_WIDE)
     const int2 *base;
@@ -454,6 +345,13 @@ struct Ix16 {
         const int2 b = base[i / LL16_ROWS];
         const NV2<short>::t v = ldn2<ST>(L + i);
         return make_int2(dec(v.x, i, b), dec(v.y, i + 1, b));
+    }
+    // four consecutive rows (i a multiple of 4): one 8-byte load
+    template <bool ST = false> __device__ __forceinline__ int4 quad(int i) const {
+        const int2 b = base[i / LL16_ROWS];
+        typedef short s4 __attribute__((ext_vector_type(4)));
+        const s4 v = ldv<ST>(reinterpret_cast<const s4 *>(L + i));
+        return make_int4(dec(v.x, i, b), dec(v.y, i + 1, b), dec(v.z, i + 2, b), dec(v.w, i + 3, b));
     }
     __device__ __forceinline__ int one(int i) const { return dec(L[i], i, base[i / LL16_ROWS]); }
 };
@@ -539,6 +437,7 @@ constexpr int LL8_ROWS = 512;
 constexpr int LL8_XM = 252, LL8_XP = 253, LL8_GHOST = 254, LL8_WIDE = 255;
 
 struct Ix8 {
+    static constexpr int ROWS = LL8_ROWS;
     const uint8_t *L;
     const int *L32;   // the same column, 32-bit (slots coded LL8_WIDE)
     const int4 *base; // per block: window starts of near-, near+, far-, far+
@@ -560,6 +459,13 @@ struct Ix8 {
         const uint16_t v = ST ? __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(L + i))
                               : *reinterpret_cast<const uint16_t *>(L + i);
         return make_int2(dec(v & 255, i, b), dec(v >> 8, i + 1, b));
+    }
+    // four consecutive rows (i a multiple of 4): one 4-byte load
+    template <bool ST = false> __device__ __forceinline__ int4 quad(int i) const {
+        const int4 b = base[i / LL8_ROWS];
+        const unsigned v = ldv<ST>(reinterpret_cast<const unsigned *>(L + i));
+        return make_int4(dec(v & 255, i, b), dec((v >> 8) & 255, i + 1, b), dec((v >> 16) & 255, i + 2, b),
+                         dec(v >> 24, i + 3, b));
     }
     __device__ __forceinline__ int one(int i) const { return dec(L[i], i, base[i / LL8_ROWS]); }
 };
@@ -718,34 +624,82 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
     }
 }
 
-// FP32 single-term PM stage, four rows per thread (no dot products): twice
-// the bytes in flight per memory instruction of k_pm_v2 -- its FP32 form
-// with half-precision coefficients loads 4 bytes of coefficients a thread.
-// Same per-row arithmetic.
-template <typename IX, bool ST, typename C>
+// Four consecutive PM coefficients as FP32: one 8-byte (half precision) or
+// 16-byte load.
+template <bool ST, typename C>
+__device__ __forceinline__ f4v ldc4(const C *p) {
+    typedef C c4 __attribute__((ext_vector_type(4)));
+    const c4 v = ldv<ST>(reinterpret_cast<const c4 *>(p));
+    f4v o;
+    o.x = (float)v.x;
+    o.y = (float)v.y;
+    o.z = (float)v.z;
+    o.w = (float)v.w;
+    return o;
+}
+
+// FP32 PM stage with four consecutive rows per thread, the form of the mixed
+// inner solve: the coefficients, the index codes and the input of the four
+// rows are one load each (the FP32 stages' pace is set by the number of
+// memory instructions more than by bytes: measured on MI355X at 13.1M rows,
+// 33.6 us with two loads of each against 28.3 us with one).  NT, DOT as in
+// k_pm_v2; the dot partials of block b go to part[b] (and part[G + b]).
+template <int NT, int DOT, typename IX, bool ST, typename C>
 __global__ void __launch_bounds__(BS)
-k_pm_v4(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0, const float *__restrict__ in,
-        float *__restrict__ out) {
+k_pm_v4(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0, const C *__restrict__ a1, IX l1,
+        const float *__restrict__ in, float *__restrict__ out, const float *__restrict__ w,
+        double *__restrict__ part) {
     if (sc->done) return;
-    const int i = (logical_block() * BS + threadIdx.x) * 4;
+    const int lb = logical_block();
+    const int i = (lb * BS + threadIdx.x) * 4;
+    double s0 = 0.0, s1 = 0.0;
+    auto dots = [&](float o, float wi) {
+        if (DOT == 1 || DOT == 2) s0 += (double)wi * (double)o;
+        if (DOT == 2) s1 += (double)o * (double)o;
+        if (DOT == 3) s0 += (double)o * (double)o;
+    };
     if (i + 3 < N) {
-        const auto c01 = ldc2<float, ST>(a0 + i);
-        const auto c23 = ldc2<float, ST>(a0 + i + 2);
-        const int2 j01 = l0.template pair<ST>(i);
-        const int2 j23 = l0.template pair<ST>(i + 2);
-        const float4 x = *reinterpret_cast<const float4 *>(in + i);
-        typedef float f4 __attribute__((ext_vector_type(4)));
-        f4 o;
-        o.x = x.x - c01.x * in[j01.x];
-        o.y = x.y - c01.y * in[j01.y];
-        o.z = x.z - c23.x * in[j23.x];
-        o.w = x.w - c23.y * in[j23.y];
-        if (ST)
-            __builtin_nontemporal_store(o, reinterpret_cast<f4 *>(out + i));
-        else
-            *reinterpret_cast<f4 *>(out + i) = o;
+        const f4v c0 = ldc4<ST>(a0 + i);
+        const int4 j0 = l0.template quad<ST>(i);
+        const f4v x = *reinterpret_cast<const f4v *>(in + i);
+        f4v o;
+        o.x = x.x - c0.x * in[j0.x];
+        o.y = x.y - c0.y * in[j0.y];
+        o.z = x.z - c0.z * in[j0.z];
+        o.w = x.w - c0.w * in[j0.w];
+        if (NT == 2) {
+            const f4v c1 = ldc4<ST>(a1 + i);
+            const int4 j1 = l1.template quad<ST>(i);
+            o.x = o.x - c1.x * in[j1.x];
+            o.y = o.y - c1.y * in[j1.y];
+            o.z = o.z - c1.z * in[j1.z];
+            o.w = o.w - c1.w * in[j1.w];
+        }
+        stv<ST>(reinterpret_cast<f4v *>(out + i), o);
+        if (DOT != 0) {
+            f4v wv = {0.f, 0.f, 0.f, 0.f};
+            if (DOT == 1 || DOT == 2) wv = ldv<ST>(reinterpret_cast<const f4v *>(w + i));
+            dots(o.x, wv.x);
+            dots(o.y, wv.y);
+            dots(o.z, wv.z);
+            dots(o.w, wv.w);
+        }
     } else {
-        for (int r = i; r < N && r < i + 4; ++r) out[r] = in[r] - (float)a0[r] * in[l0.one(r)];
+        for (int r = i; r < N && r < i + 4; ++r) {
+            float o = in[r] - (float)a0[r] * in[l0.one(r)];
+            if (NT == 2) o = o - (float)a1[r] * in[l1.one(r)];
+            out[r] = o;
+            dots(o, (DOT == 1 || DOT == 2) ? w[r] : 0.f);
+        }
+    }
+    if (DOT != 0) {
+        __shared__ double lds[BS / 64];
+        const double t0 = block_sum<BS>(s0, lds);
+        if (DOT == 2) {
+            const double t1 = block_sum<BS>(s1, lds);
+            if (threadIdx.x == 0) part[gridDim.x + lb] = t1;
+        }
+        if (threadIdx.x == 0) part[lb] = t0;
     }
 }
 
@@ -778,26 +732,16 @@ k_pm2_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
     }
 }
 
-// SpMV with two consecutive rows per thread (same per-row arithmetic).
+// AXSUB (pcgpme_para.F90:323-352) with two consecutive rows per thread:
+// y = A x in sorted slot order (MODE 0), or the true residual y = v1 - A x
+// (MODE 3, the mixed solve's FP64 refinement).
 template <typename T, int NL, int MODE, typename IX, bool ST>
 __global__ void __launch_bounds__(BS)
 k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL, IX LL,
-          const T *__restrict__ D, const T *__restrict__ v0, const T *__restrict__ v1,
-          const T *__restrict__ v2, T *__restrict__ vout, T *__restrict__ y,
+          const T *__restrict__ D, const T *__restrict__ x, const T *__restrict__ v1, T *__restrict__ y,
           const int *__restrict__ order) {
+    static_assert(MODE == 0 || MODE == 3, "y = A x or y = v1 - A x");
     if (sc->done) return;
-    T c0 = T(0), c1 = T(0);
-    if (MODE == 1) {
-        c0 = (T)sc->BETA;
-        c1 = (T)sc->AMU;
-    } else if (MODE == 2) {
-        c0 = (T)sc->ALP;
-    }
-    auto X = [&](int j) -> T {
-        if (MODE == 1) return v1[j] + c0 * (v0[j] - c1 * v2[j]);
-        if (MODE == 2) return v1[j] - c0 * v2[j];
-        return v0[j];
-    };
     const int i = (row_block(order) * BS + threadIdx.x) * 2;
     if (i + 1 < N) {
         typename V2<T>::t a[NL];
@@ -808,32 +752,14 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
             a[k] = lds2<ST>(AL + (size_t)k * lda + i);
         }
         const auto d = lds2<ST>(D + i);
-        T x0, x1;
-        if (MODE == 2) {
-            const auto r = ld2(v1 + i), q = ld2(v2 + i);
-            x0 = r.x - c0 * q.x;
-            x1 = r.y - c0 * q.y;
-        } else if (MODE == 1) {
-            const auto r = ld2(v1 + i), p = ld2(v0 + i), q = ld2(v2 + i);
-            x0 = r.x + c0 * (p.x - c1 * q.x);
-            x1 = r.y + c0 * (p.y - c1 * q.y);
-        } else {
-            const auto xv = ld2(v0 + i);
-            x0 = xv.x;
-            x1 = xv.y;
-        }
-        T acc0 = x0 * d.x, acc1 = x1 * d.y;
+        const auto xv = ld2(x + i);
+        T acc0 = xv.x * d.x, acc1 = xv.y * d.y;
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
-            acc0 = acc0 + a[k].x * X(l[k].x);
-            acc1 = acc1 + a[k].y * X(l[k].y);
+            acc0 = acc0 + a[k].x * x[l[k].x];
+            acc1 = acc1 + a[k].y * x[l[k].y];
         }
         typename V2<T>::t o;
-        if (MODE == 1 || MODE == 2) {
-            o.x = x0;
-            o.y = x1;
-            *reinterpret_cast<typename V2<T>::t *>(vout + i) = o;
-        }
         if (MODE == 3) {
             const auto bv = ld2(v1 + i);
             o.x = bv.x - acc0;
@@ -844,73 +770,84 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
         }
         sts2<ST>(y + i, o);
     } else if (i < N) {
-        const T xi = X(i);
-        T acc = xi * D[i];
+        T acc = x[i] * D[i];
 #pragma unroll
-        for (int k = 0; k < NL; ++k) acc = acc + AL[(size_t)k * lda + i] * X(LL.col((size_t)k * lda).one(i));
-        if (MODE == 1 || MODE == 2) vout[i] = xi;
+        for (int k = 0; k < NL; ++k) acc = acc + AL[(size_t)k * lda + i] * x[LL.col((size_t)k * lda).one(i)];
         y[i] = MODE == 3 ? v1[i] - acc : acc;
     }
 }
 
 // Row-major copies for the FP32 SpMV of the mixed inner solve: the six
 // coefficients of a row together (ALr[6 i + k]) and its six 16-bit codes
-// (Lr[6 i + k]).  Two rows are then three 16-byte and three 8-byte loads
-// instead of twelve, the FP32 kernels' pace being set more by the number of
-// memory instructions than by bytes.
+// (Lr16[6 i + k]) and, when the operator has them, 8-bit codes (Lr8).  Two
+// rows are then three 16-byte loads of coefficients and three 8- (or 4-)
+// byte loads of codes instead of twelve, the FP32 kernels' pace being set
+// more by the number of memory instructions than by bytes.  *nonunit counts
+// rows whose FP32 diagonal is not 1: after PCGPME's scaling every row with
+// max(sum|AL|, D) = D has D W = P2 D / ((1 + P1) D) = 1 up to one rounding,
+// which FP32 drops, so the SpMV can skip the diagonal stream when it is 0.
 __global__ void __launch_bounds__(BS)
 k_to_rows32(int N, int lda, const double *__restrict__ AL, const short *__restrict__ L16,
-            float *__restrict__ ALr, short *__restrict__ Lr) {
+            const uint8_t *__restrict__ L8, const float *__restrict__ D32, float *__restrict__ ALr,
+            short *__restrict__ Lr16, uint8_t *__restrict__ Lr8, int *__restrict__ nonunit) {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i >= N) return;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         ALr[(size_t)6 * i + k] = (float)AL[(size_t)k * lda + i];
-        Lr[(size_t)6 * i + k] = L16[(size_t)k * lda + i];
+        Lr16[(size_t)6 * i + k] = L16[(size_t)k * lda + i];
+        if (L8) Lr8[(size_t)6 * i + k] = L8[(size_t)k * lda + i];
     }
+    if (D32[i] != 1.0f) atomicAdd(nonunit, 1);
 }
 
-// FP32 y = A x (MODE 0, NL = 6) from the row-major copies, two rows per
-// thread; the codes decode as Ix16 (wide slots read the column-major LL).
-// Same per-row arithmetic as k_spmv_v2.
+// The codes of two consecutive rows (i even) from the row-major copies.
 template <bool ST>
+__device__ __forceinline__ void row_codes(const short *__restrict__ Lr, int i, int (&cr)[2][6]) {
+    typedef short s4 __attribute__((ext_vector_type(4)));
+    const s4 *cp = reinterpret_cast<const s4 *>(Lr + (size_t)6 * i);
+    const s4 c0 = ldv<ST>(cp), c1 = ldv<ST>(cp + 1), c2 = ldv<ST>(cp + 2);
+    const int v[12] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y, c2.z, c2.w};
+#pragma unroll
+    for (int k = 0; k < 12; ++k) cr[k / 6][k % 6] = v[k];
+}
+template <bool ST>
+__device__ __forceinline__ void row_codes(const uint8_t *__restrict__ Lr, int i, int (&cr)[2][6]) {
+    const unsigned *cp = reinterpret_cast<const unsigned *>(Lr + (size_t)6 * i);
+    const unsigned w[3] = {ldv<ST>(cp), ldv<ST>(cp + 1), ldv<ST>(cp + 2)};
+#pragma unroll
+    for (int k = 0; k < 12; ++k) cr[k / 6][k % 6] = (int)((w[k / 4] >> (8 * (k % 4))) & 255u);
+}
+
+// FP32 y = A x (NL = 6) from the row-major copies, two rows per thread; IX
+// (Ix16 with Lr16, or Ix8 with Lr8) decodes the codes, wide slots reading
+// the column-major LL.  With *nonunit == 0 every diagonal is 1 and is not
+// read (x * 1 = x exactly).  Same per-row arithmetic as k_spmv_v2.
+template <typename IX, typename CT, bool ST>
 __global__ void __launch_bounds__(BS)
 k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restrict__ ALr,
-              const short *__restrict__ Lr, Ix16 LL, const float *__restrict__ D, const float *__restrict__ x,
-              float *__restrict__ y) {
+              const CT *__restrict__ Lr, IX LL, const float *__restrict__ D, const int *__restrict__ nonunit,
+              const float *__restrict__ x, float *__restrict__ y) {
     if (sc->done) return;
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    typedef short s4 __attribute__((ext_vector_type(4)));
+    const bool unit = *nonunit == 0;
     const int i = (logical_block() * BS + threadIdx.x) * 2;
     if (i + 1 < N) {
-        const f4 *ap = reinterpret_cast<const f4 *>(ALr + (size_t)6 * i);
-        const s4 *cp = reinterpret_cast<const s4 *>(Lr + (size_t)6 * i);
-        f4 a0, a1, a2;
-        s4 c0, c1, c2;
-        if (ST) {
-            a0 = __builtin_nontemporal_load(ap);
-            a1 = __builtin_nontemporal_load(ap + 1);
-            a2 = __builtin_nontemporal_load(ap + 2);
-            c0 = __builtin_nontemporal_load(cp);
-            c1 = __builtin_nontemporal_load(cp + 1);
-            c2 = __builtin_nontemporal_load(cp + 2);
-        } else {
-            a0 = ap[0];
-            a1 = ap[1];
-            a2 = ap[2];
-            c0 = cp[0];
-            c1 = cp[1];
-            c2 = cp[2];
-        }
+        const f4v *ap = reinterpret_cast<const f4v *>(ALr + (size_t)6 * i);
+        const f4v a0 = ldv<ST>(ap), a1 = ldv<ST>(ap + 1), a2 = ldv<ST>(ap + 2);
         const float ar[2][6] = {{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y}, {a1.z, a1.w, a2.x, a2.y, a2.z, a2.w}};
-        const int cr[2][6] = {{c0.x, c0.y, c0.z, c0.w, c1.x, c1.y}, {c1.z, c1.w, c2.x, c2.y, c2.z, c2.w}};
-        const int2 b = LL.base[i / LL16_ROWS];
-        const float2 d = lds2<ST>(D + i);
+        int cr[2][6];
+        row_codes<ST>(Lr, i, cr);
+        const auto b = LL.base[i / IX::ROWS];
         const float2 xv = ld2(x + i);
-        float acc0 = xv.x * d.x, acc1 = xv.y * d.y;
+        float acc0 = xv.x, acc1 = xv.y;
+        if (!unit) {
+            const float2 d = lds2<ST>(D + i);
+            acc0 = xv.x * d.x;
+            acc1 = xv.y * d.y;
+        }
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            const Ix16 c = LL.col((size_t)k * lda);
+            const IX c = LL.col((size_t)k * lda);
             acc0 = acc0 + ar[0][k] * x[c.dec(cr[0][k], i, b)];
             acc1 = acc1 + ar[1][k] * x[c.dec(cr[1][k], i + 1, b)];
         }
@@ -919,44 +856,60 @@ k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restri
         o.y = acc1;
         sts2<ST>(y + i, o);
     } else if (i < N) {
-        float acc = x[i] * D[i];
+        float acc = unit ? x[i] : x[i] * D[i];
 #pragma unroll
         for (int k = 0; k < 6; ++k) acc = acc + ALr[(size_t)6 * i + k] * x[LL.col((size_t)k * lda).one(i)];
         y[i] = acc;
     }
 }
 
-// E = R - ALP*Q (pcgpme_para.F90:173) as its own pass, two rows per thread.
+// The vector kernels take 16 bytes of each vector per access: two rows a
+// thread in FP64, four in FP32.
+template <typename T> struct VecOf;
+template <> struct VecOf<float> {
+    static constexpr int R = 4;
+    typedef float t __attribute__((ext_vector_type(4)));
+};
+template <> struct VecOf<double> {
+    static constexpr int R = 2;
+    typedef double t __attribute__((ext_vector_type(2)));
+};
+
+// E = R - ALP*Q (pcgpme_para.F90:173) as its own pass.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_eupd_v2(const Scal *__restrict__ sc, int N, const T *__restrict__ r, const T *__restrict__ q,
-          T *__restrict__ e) {
+k_eupd_vec(const Scal *__restrict__ sc, int N, const T *__restrict__ r, const T *__restrict__ q,
+           T *__restrict__ e) {
     if (sc->done) return;
-    const int i = (logical_block() * BS + threadIdx.x) * 2;
+    using VT = typename VecOf<T>::t;
+    constexpr int RV = VecOf<T>::R;
+    const int i = (logical_block() * BS + threadIdx.x) * RV;
     const T alp = (T)sc->ALP;
-    if (i + 1 < N) {
-        const auto rv = ld2(r + i), qv = ld2(q + i);
-        typename V2<T>::t o;
-        o.x = rv.x - alp * qv.x;
-        o.y = rv.y - alp * qv.y;
-        *reinterpret_cast<typename V2<T>::t *>(e + i) = o;
-    } else if (i < N) {
-        e[i] = r[i] - alp * q[i];
+    if (i + RV - 1 < N) {
+        const VT rv = *reinterpret_cast<const VT *>(r + i), qv = *reinterpret_cast<const VT *>(q + i);
+        VT o;
+#pragma unroll
+        for (int k = 0; k < RV; ++k) o[k] = rv[k] - alp * qv[k];
+        *reinterpret_cast<VT *>(e + i) = o;
+    } else {
+        for (int k = i; k < N && k < i + RV; ++k) e[k] = r[k] - alp * q[k];
     }
 }
 
-// x/r update with two consecutive rows per thread.
+// X = X + ALP*P + AMU*E;  R = E - AMU*V;  RN += R*R;  C1 += R*R^
+// (pcgpme_para.F90:203-211); per-block partials of RN and C1.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_update_v2(const Scal *__restrict__ sc, int N, T *__restrict__ x,
-            const T *__restrict__ p, const T *__restrict__ e,
-            const T *__restrict__ v, T *__restrict__ r,
-            const T *__restrict__ rh, double *__restrict__ part) {
+k_update_vec(const Scal *__restrict__ sc, int N, T *__restrict__ x, const T *__restrict__ p,
+             const T *__restrict__ e, const T *__restrict__ v, T *__restrict__ r, const T *__restrict__ rh,
+             double *__restrict__ part) {
     if (sc->done) return;
+    using VT = typename VecOf<T>::t;
+    constexpr int RV = VecOf<T>::R;
     const T alp = (T)sc->ALP, amu = (T)sc->AMU;
     double rn = 0.0, c1 = 0.0;
     const int lb = logical_block();
-    const int i = (lb * BS + threadIdx.x) * 2;
+    const int i = (lb * BS + threadIdx.x) * RV;
     auto row = [&](T xi, T pi, T ei, T vi, T rhi, T &xo, T &ro) {
         T t = ei;
         xo = xi + alp * pi + amu * t;
@@ -965,18 +918,27 @@ k_update_v2(const Scal *__restrict__ sc, int N, T *__restrict__ x,
         rn += (double)t * (double)t;
         c1 += (double)t * (double)rhi;
     };
-    if (i + 1 < N) {
-        const auto xv = ld2(x + i), pv = ld2(p + i), ev = ld2(e + i), vv = ld2(v + i), hv = ld2(rh + i);
-        typename V2<T>::t xo, ro;
-        row(xv.x, pv.x, ev.x, vv.x, hv.x, xo.x, ro.x);
-        row(xv.y, pv.y, ev.y, vv.y, hv.y, xo.y, ro.y);
-        *reinterpret_cast<typename V2<T>::t *>(x + i) = xo;
-        *reinterpret_cast<typename V2<T>::t *>(r + i) = ro;
-    } else if (i < N) {
-        T xo, ro;
-        row(x[i], p[i], e[i], v[i], rh[i], xo, ro);
-        x[i] = xo;
-        r[i] = ro;
+    if (i + RV - 1 < N) {
+        const VT xv = *reinterpret_cast<const VT *>(x + i), pv = *reinterpret_cast<const VT *>(p + i),
+                 ev = *reinterpret_cast<const VT *>(e + i), vv = *reinterpret_cast<const VT *>(v + i),
+                 hv = *reinterpret_cast<const VT *>(rh + i);
+        VT xo, ro;
+#pragma unroll
+        for (int k = 0; k < RV; ++k) {
+            T a, b;
+            row(xv[k], pv[k], ev[k], vv[k], hv[k], a, b);
+            xo[k] = a;
+            ro[k] = b;
+        }
+        *reinterpret_cast<VT *>(x + i) = xo;
+        *reinterpret_cast<VT *>(r + i) = ro;
+    } else {
+        for (int k = i; k < N && k < i + RV; ++k) {
+            T xo, ro;
+            row(x[k], p[k], e[k], v[k], rh[k], xo, ro);
+            x[k] = xo;
+            r[k] = ro;
+        }
     }
     __shared__ double lds[BS / 64];
     const double t0 = block_sum<BS>(rn, lds);
@@ -987,54 +949,25 @@ k_update_v2(const Scal *__restrict__ sc, int N, T *__restrict__ x,
     }
 }
 
-// P = R + BETA*(P - AMU*Q), two rows per thread.
+// P = R + BETA*(P - AMU*Q) (pcgpme_para.F90:244), in place.
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_pupd_v2(const Scal *__restrict__ sc, int N, T *__restrict__ p,
-          const T *__restrict__ r, const T *__restrict__ q) {
+k_pupd_vec(const Scal *__restrict__ sc, int N, T *__restrict__ p, const T *__restrict__ r,
+           const T *__restrict__ q) {
     if (sc->done) return;
-    const int i = (logical_block() * BS + threadIdx.x) * 2;
+    using VT = typename VecOf<T>::t;
+    constexpr int RV = VecOf<T>::R;
+    const int i = (logical_block() * BS + threadIdx.x) * RV;
     const T beta = (T)sc->BETA, amu = (T)sc->AMU;
-    if (i + 1 < N) {
-        const auto pv = ld2(p + i), rv = ld2(r + i), qv = ld2(q + i);
-        typename V2<T>::t o;
-        o.x = rv.x + beta * (pv.x - amu * qv.x);
-        o.y = rv.y + beta * (pv.y - amu * qv.y);
-        *reinterpret_cast<typename V2<T>::t *>(p + i) = o;
-    } else if (i < N) {
-        p[i] = r[i] + beta * (p[i] - amu * q[i]);
-    }
-}
-
-// ---------------------------------------------------------- update (a11) --
-// X = X + ALP*P + AMU*E;  R = E - AMU*V;  RN += R*R;  C1 += R*R^.
-template <typename T>
-__global__ void __launch_bounds__(BS)
-k_update(const Scal *__restrict__ sc, int N, int R, T *__restrict__ x,
-         const T *__restrict__ p, const T *__restrict__ e,
-         const T *__restrict__ v, T *__restrict__ r,
-         const T *__restrict__ rh, double *__restrict__ part) {
-    if (sc->done) return;
-    const T alp = (T)sc->ALP, amu = (T)sc->AMU;
-    double rn = 0.0, c1 = 0.0;
-    const int lb = logical_block();
-    const int base = lb * BS * R + threadIdx.x;
-    for (int rr = 0; rr < R; ++rr) {
-        const int i = base + rr * BS;
-        if (i >= N) break;
-        T t = e[i];
-        x[i] = x[i] + alp * p[i] + amu * t;
-        t = t - amu * v[i];
-        r[i] = t;
-        rn += (double)t * (double)t;
-        c1 += (double)t * (double)rh[i];
-    }
-    __shared__ double lds[BS / 64];
-    const double t0 = block_sum<BS>(rn, lds);
-    const double t1 = block_sum<BS>(c1, lds);
-    if (threadIdx.x == 0) {
-        part[lb] = t0;
-        part[gridDim.x + lb] = t1;
+    if (i + RV - 1 < N) {
+        const VT pv = *reinterpret_cast<const VT *>(p + i), rv = *reinterpret_cast<const VT *>(r + i),
+                 qv = *reinterpret_cast<const VT *>(q + i);
+        VT o;
+#pragma unroll
+        for (int k = 0; k < RV; ++k) o[k] = rv[k] + beta * (pv[k] - amu * qv[k]);
+        *reinterpret_cast<VT *>(p + i) = o;
+    } else {
+        for (int k = i; k < N && k < i + RV; ++k) p[k] = r[k] + beta * (p[k] - amu * q[k]);
     }
 }
 
@@ -1042,16 +975,13 @@ k_update(const Scal *__restrict__ sc, int N, int R, T *__restrict__ x,
 // PM*A*x0 (nullptr when x0 = 0, where it is exactly zero).
 template <typename T>
 __global__ void __launch_bounds__(BS)
-k_init(const Scal *__restrict__ sc, int N, int R, T *__restrict__ wk1,
-       const T *__restrict__ wk3, T *__restrict__ r, T *__restrict__ rh,
-       double *__restrict__ part) {
+k_init(const Scal *__restrict__ sc, int N, T *__restrict__ wk1, const T *__restrict__ wk3, T *__restrict__ r,
+       T *__restrict__ rh, double *__restrict__ part) {
     if (sc->done) return;
     double bn = 0.0, c1 = 0.0;
     const int lb = logical_block();
-    const int base = lb * BS * R + threadIdx.x;
-    for (int rr = 0; rr < R; ++rr) {
-        const int i = base + rr * BS;
-        if (i >= N) break;
+    const int i = lb * BS + threadIdx.x;
+    if (i < N) {
         T t = wk1[i];
         bn += (double)t * (double)t;
         if (wk3) t = t - wk3[i];

@@ -66,9 +66,10 @@ template <typename T> struct DevBuf {
 
 // Vectors of one precision for one Bi-CGSTAB run.
 template <typename T> struct VecSet {
-    DevBuf<T> r, rh, p0, p1, q, e, v, y, t, x;
+    DevBuf<T> r, rh, p0, q, e, v, y, t, x;
+    int padN = -1;  // the row count whose ghost rows [N, N+64) were last zeroed
     int ensure(size_t n) {
-        DevBuf<T> *all[] = {&r, &rh, &p0, &p1, &q, &e, &v, &y, &t, &x};
+        DevBuf<T> *all[] = {&r, &rh, &p0, &q, &e, &v, &y, &t, &x};
         for (auto *b : all) {
             int rc = b->ensure(n);
             if (rc) return rc;
@@ -76,14 +77,78 @@ template <typename T> struct VecSet {
         return CLOWNS_OK;
     }
     void release() {
-        DevBuf<T> *all[] = {&r, &rh, &p0, &p1, &q, &e, &v, &y, &t, &x};
+        DevBuf<T> *all[] = {&r, &rh, &p0, &q, &e, &v, &y, &t, &x};
         for (auto *b : all) b->release();
+        padN = -1;
     }
 };
 
 } // namespace
 
+// Run-time switches (environment variables), read once when a solver is
+// created and kept in that solver: creating another solver never changes the
+// kernel paths of a live one.
+struct Knobs {
+    // Pairwise fusion of single-term PM stages (k_pm2_v2) saves one pass over
+    // the intermediate vector but recomputes the first stage at every gather
+    // point (three dependent loads).  Which form is faster depends on the
+    // operator: measured on MI355X, fused is 9% (FP64) and 12% (FP32) faster
+    // per solve at 2.45M rows, but at 13.1M rows (512x512 planes) the fused
+    // FP32 pair takes 83 us against 2 x 31 us.  So each solver times both
+    // forms on its operator the first time it solves at a given size
+    // (tune_fuse) and keeps the faster.  CLOWNS_FUSE_PM=0/1 forces both
+    // precisions off/on (-1: timed).
+    int fuse_pm = -1;
+    // Non-temporal loads/stores of the streamed arrays in the PM stages and
+    // the SpMV (the ST kernels).  Measured on MI355X, at 13.1M rows FP64 SpMV
+    // 224 -> 196 us and PM stage 56 -> 51 us, but FP32 PM stage 28 -> 35 us;
+    // at 2.45M rows (operator inside the 256 MB last-level cache) every
+    // kernel slower.  tune_stream times both per operator and precision;
+    // CLOWNS_STREAM = 0/1 forces them off/on (-1: timed).
+    int stream = -1;
+    // 8-bit coded neighbour indices (Ix8) in place of the 16-bit codes.  Half
+    // the index bytes, but more decoding and a scattered 32-bit read per wide
+    // slot: measured on MI355X, FP64 solve 2.6% faster at 13.1M rows and 2%
+    // slower at 2.45M rows (operator inside the last-level cache), so
+    // tune_stream decides per operator.  CLOWNS_LL8 = 0 off, 1 on, -1
+    // (default) timed.
+    int ll8 = -1;
+    // 16-bit coded neighbour indices.  Measured on MI355X: single PM stage 5%
+    // faster at 2.45M rows (FP64 10.05 vs 10.61 us, FP32 6.72 vs 7.10 us),
+    // equal at 13.1M rows where the +-plane gathers, not the index stream,
+    // set the pace.  CLOWNS_LL16 = 0 reads the 32-bit LL (and builds no
+    // 8-bit codes either).
+    bool ll16 = true;
+    // Four rows per thread in the FP32 single-term PM stages (k_pm_v4): with
+    // half-precision coefficients two rows load only 4 bytes of them.
+    // Measured on MI355X: mixed solve 199.9 -> 195.3 ms at 13.1M rows (single
+    // stages), unchanged at 2.45M rows (fused pairs).  CLOWNS_V4_32 = 0 keeps
+    // two rows.
+    bool v4_32 = true;
+    // The FP32 SpMV from row-major copies of its coefficients and 16-bit
+    // codes (k_spmv_rows32): three 16-byte and three 8-byte loads for two
+    // rows instead of twelve smaller ones.  Measured on MI355X (mixed solve):
+    // 27.9 -> 27.0 ms at 2.45M rows, 196 -> 195 ms at 13.1M rows.  With it the
+    // FP32 kernels keep 16-bit codes.  CLOWNS_SPMV_ROWS = 0 keeps the
+    // column-major operator.
+    bool rows32 = true;
+    // The FP32 inner solve of the mixed form applies the PM preconditioner
+    // with half-precision coefficients (k_to_f16): the outer FP64 loop
+    // measures the true residual with the exact FP64 preconditioner, so this
+    // changes only how much each refinement gains (by about |M~ - M| |M~^-1|,
+    // some 1e-3 against the inner tolerance 1e-2), not the accuracy of the
+    // result; the inner operator keeps FP32.  CLOWNS_PM_HALF = 0 keeps FP32
+    // coefficients.
+    bool pm_half = true;
+    // Block order of the FP64 gather kernels for an assembled operator (see
+    // ensure_block_order).  CLOWNS_BLOCK_ORDER = 0 switches it off.
+    bool block_order = true;
+    // CLOWNS_VERBOSE = 1: report per-operator tuning decisions on stderr.
+    bool verbose = false;
+};
+
 struct clowns_solver {
+    Knobs kn;
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -108,10 +173,13 @@ struct clowns_solver {
     bool ll8_ok = false;
     int ll8_wide = 0;      // slots read from the 32-bit LL (LL8_WIDE)
     DevBuf<float> AL32, D32;
-    DevBuf<float> ALr32;   // row-major FP32 coefficients of the FP32 SpMV (h_rows32)
+    DevBuf<float> ALr32;   // row-major FP32 coefficients of the FP32 SpMV (Knobs::rows32)
     DevBuf<short> Lr16;    // row-major 16-bit codes of the same
+    DevBuf<uint8_t> Lr8;   // row-major 8-bit codes of the same (with LL8)
+    bool have_rows8 = false;
+    DevBuf<int> nonunit;   // rows whose FP32 diagonal is not 1 (k_to_rows32)
     bool have_rows = false;
-    DevBuf<_Float16> ALh;  // half-precision PM coefficients of the FP32 inner solve (h_pm_half)
+    DevBuf<_Float16> ALh;  // half-precision PM coefficients of the FP32 inner solve (Knobs::pm_half)
     bool have32 = false;
     bool have_h = false;   // ALh built with the FP32 copy
     // Transposed operator (adjoint), scaled + sorted the same way.
@@ -131,6 +199,7 @@ struct clowns_solver {
     VecSet<double> v64;
     VecSet<float> v32;
     DevBuf<double> bs;     // scaled right-hand side
+    int bs_padN = -1;      // row count whose ghost rows of bs were last zeroed
     DevBuf<double> stage_b, stage_x;  // clowns_pcgpme host staging
     DevBuf<double> part;   // per-block partial sums
     DevBuf<int> flag;      // device error flag
@@ -144,16 +213,10 @@ struct clowns_solver {
     DevBuf<double> vof_term;   // VOF update: quvwf_modify correction per cell
     DevBuf<int> vof_cnt;       // VOF update: flags and counters (VC_*)
     DevBuf<Scal> sc;       // [0] inner/FP64 run, [1] outer (mixed)
-    Scal *h_sc = nullptr;  // pinned mirror
+    Scal *h_sc = nullptr;  // pinned mirror: [0] inner/FP64 run, [1] outer, [2], [3] poll slots
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    hipEvent_t evp[2] = {nullptr, nullptr};  // ends of the two queued iteration batches
 
-    int R = 1;             // rows per thread in the streaming kernels
-    bool fuse_pupd = false; // p update inside the SpMV gathers (else own pass)
-    // e = r - alp*q as its own pass before a plain SpMV, rather than at each
-    // of the SpMV's gather points (two gathered vectors per neighbour):
-    // measured on MI355X, FP64 solve 1.3% (2.45M rows) and 2% (13.1M rows)
-    // faster, mixed 2% faster at 13.1M rows.  CLOWNS_SPLIT_E = 0 fuses.
-    bool split_e = true;
     // mixed: smallest relative tolerance of an inner solve.  Short inner
     // solves restart Bi-CGSTAB from the true residual more often, which on
     // these operators costs fewer total iterations: measured on MI355X,
@@ -175,11 +238,14 @@ struct clowns_solver {
 
 namespace {
 
-std::mutex g_mu;
+std::mutex g_mu;      // the default solvers behind the drop-in entry points
+std::mutex g_env_mu;  // reading the run-time switches
 clowns_solver *g_default[64] = {};
 
 int n_pad(int N) { return N + PAD; }
 
+// Grid of a kernel taking R consecutive rows a thread: a multiple of 8
+// workgroups (logical_block's XCD mapping).
 struct Geo {
     int G, R;
 };
@@ -194,16 +260,23 @@ int ensure_events(clowns_solver *h) {
         HIPCHK(hipEventCreate(&h->ev0));
         HIPCHK(hipEventCreate(&h->ev1));
         HIPCHK(hipEventCreate(&h->ev2));
+        HIPCHK(hipEventCreateWithFlags(&h->evp[0], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&h->evp[1], hipEventDisableTiming));
     }
     return CLOWNS_OK;
 }
 
+// Dot partials: two arrays of one double per block of the largest grid any
+// kernel of a run launches, the one-row kernels' (k_init).  Every other
+// form covers more rows per block (geo_of(N, R) with R >= 1 never exceeds
+// geo_of(N, 1)).
+size_t part_len(int N) { return (size_t)2 * geo_of(N, 1).G; }
+
 int ensure_common(clowns_solver *h, int N) {
     int rc;
-    Geo g = geo_of(N, h->R);
-    if ((rc = h->part.ensure((size_t)2 * g.G))) return rc;
+    if ((rc = h->part.ensure(part_len(N)))) return rc;
     if ((rc = h->sc.ensure(2))) return rc;
-    if (!h->h_sc) HIPCHK(hipHostMalloc((void **)&h->h_sc, 2 * sizeof(Scal), hipHostMallocDefault));
+    if (!h->h_sc) HIPCHK(hipHostMalloc((void **)&h->h_sc, 4 * sizeof(Scal), hipHostMallocDefault));
     return ensure_events(h);
 }
 
@@ -227,8 +300,11 @@ template <typename T> struct Op {
     bool ix8 = false;             // the kernels read LL8 (tune_stream), else LL16
     const int *order = nullptr;   // row-block order of the gather kernels, or null
     const _Float16 *ALh = nullptr;  // FP32 inner solve: half-precision PM coefficients, or null
+    bool v4 = false;                // FP32 single PM stages with four rows a thread (k_pm_v4)
     const float *ALr = nullptr;     // FP32 SpMV: row-major coefficients and codes, or null
-    const short *Lr = nullptr;
+    const short *Lr16 = nullptr;
+    const uint8_t *Lr8 = nullptr;   // ... 8-bit codes (with LL8), or null
+    const int *nonunit = nullptr;   // ... device count of rows whose FP32 diagonal is not 1
 };
 
 // The index source of column j for the two-row kernels.
@@ -238,67 +314,6 @@ template <typename T> Ix16 ix16(const Op<T> &A, size_t off) { return Ix16{A.LL16
 template <typename T> bool has8(const Op<T> &A) { return A.LL8 != nullptr && A.ix8; }
 template <typename T> Ix8 ix8(const Op<T> &A, size_t off) { return Ix8{A.LL8 + off, A.LL + off, A.base8, A.N}; }
 
-// Pairwise fusion of single-term PM stages (k_pm2) saves one pass over the
-// intermediate vector but recomputes the first stage at every gather point
-// (three dependent loads).  Which form is faster depends on the operator:
-// measured on MI355X, fused is 9% (FP64) and 12% (FP32) faster per solve at
-// 2.45M rows, but at 13.1M rows (512x512 planes) the fused FP32 pair takes
-// 83 us against 2 x 31 us.  So each solver times both forms on its operator
-// the first time it solves at a given size (tune_fuse) and keeps the faster.
-// CLOWNS_FUSE_PM=0/1 forces both precisions off/on (-1: timed again).
-int h_fuse_pm = -1;
-// Non-temporal loads/stores of the streamed arrays in the PM stages and the
-// SpMV (the ST kernels).  Measured on MI355X, at 13.1M rows FP64 SpMV 224 ->
-// 196 us and PM stage 56 -> 51 us, but FP32 PM stage 28 -> 35 us; at 2.45M
-// rows (operator inside the 256 MB last-level cache) every kernel slower.
-// tune_stream times both per operator and precision; CLOWNS_STREAM = 0/1
-// forces them off/on for solvers created from then on (-1: timed again).
-int h_stream = -1;
-// CLOWNS_VERBOSE = 1: report per-operator tuning decisions on stderr.
-bool h_verbose = false;
-// Two rows per thread in the PM stages (k_pm_v2, k_pm2_v2): 8/16-byte
-// accesses, twice the bytes in flight per memory instruction.  Measured on
-// MI355X: single FP64 PM stage 4-5% faster (configurations 2 and 3); on for
-// both precisions (CLOWNS_V2_32 / CLOWNS_V2_64 = 0 to switch off).
-bool h_v2_32 = true, h_v2_64 = true;
-template <typename T> bool use_v2() { return sizeof(T) == 4 ? h_v2_32 : h_v2_64; }
-// Two rows per thread in the SpMV, p update and x/r update (measured
-// separately from the PM stages; CLOWNS_V2VEC = 0/1).
-bool h_v2_vec = true;
-// Four rows per thread in the FP32 single-term PM stages (k_pm_v4): with
-// half-precision coefficients two rows load only 4 bytes of them.  Measured
-// on MI355X: mixed solve 199.9 -> 195.3 ms at 13.1M rows (single stages),
-// unchanged at 2.45M rows (fused pairs).  The same for the FP32 SpMV (201
-// ms at 13.1M rows, 29.9 vs 27.9 ms at 2.45M) and the FP32 fused pair (30.3
-// vs 28.0 ms at 2.45M rows) was slower.  CLOWNS_V4_32 = 0 keeps two rows.
-bool h_v4_32 = true;
-// The FP32 SpMV from row-major copies of its coefficients and 16-bit codes
-// (k_spmv_rows32): three 16-byte and three 8-byte loads for two rows instead
-// of twelve smaller ones.  Measured on MI355X (mixed solve): 27.9 -> 27.0 ms
-// at 2.45M rows, 196 -> 195 ms at 13.1M rows.  With it the FP32 kernels
-// keep 16-bit codes.  CLOWNS_SPMV_ROWS = 0 keeps the column-major operator.
-bool h_rows32 = true;
-template <typename T> bool use_v2_vec() { return h_v2_vec; }
-// 16-bit coded neighbour indices in the two-row kernels.  Measured on
-// MI355X: single PM stage 5% faster at 2.45M rows (FP64 10.05 vs 10.61 us,
-// FP32 6.72 vs 7.10 us), equal at 13.1M rows where the +-plane gathers, not
-// the index stream, set the pace.  CLOWNS_LL16 = 0 switches it off.  (Four
-// rows per thread was measured too: no faster than two in either precision.)
-bool h_ll16 = true;
-// 8-bit coded neighbour indices (Ix8) in the two-row kernels in place of the
-// 16-bit codes.  Half the index bytes, but more decoding and a scattered
-// 32-bit read per wide slot: measured on MI355X, FP64 solve 2.6% faster at
-// 13.1M rows and 2% slower at 2.45M rows (operator inside the last-level
-// cache), so tune_stream decides per operator.  CLOWNS_LL8 = 0 off, 1 on,
-// -1 (default) timed.
-int h_ll8 = -1;
-// The FP32 inner solve of the mixed form applies the PM preconditioner with
-// half-precision coefficients (k_to_f16): the outer FP64 loop measures the
-// true residual with the exact FP64 preconditioner, so this changes only how
-// much each refinement gains (by about |M~ - M| |M~^-1|, some 1e-3 against
-// the inner tolerance 1e-2), not the accuracy of the result; the inner
-// operator keeps FP32.  CLOWNS_PM_HALF = 0 keeps FP32 coefficients.
-bool h_pm_half = true;
 // Block order of the FP64 gather kernels for an assembled operator: the
 // 512-row blocks sorted by (chunk of ORDER_LINES grid lines, plane, block),
 // so that a row's z neighbours were streamed a few thousand rows earlier
@@ -306,10 +321,9 @@ bool h_pm_half = true;
 // (FP64 solve): 306 -> 294 ms at 13.1M rows, 45.3 -> 44.8 ms at 2.45M rows;
 // 16-256 lines per chunk within 1% of each other at 13.1M rows, 2-8 lines
 // slower than no order (the y neighbours then miss); the FP32 kernels gain
-// nothing and keep the natural order.  CLOWNS_BLOCK_ORDER = 0 switches it off.
+// nothing and keep the natural order.
 constexpr int ORDER_LINES = 64;
 constexpr int ORDER_MIN_ROWS = 1 << 20;
-bool h_block_order = true;
 
 // Two-row PM launches for coefficient type C (T, or _Float16: the FP32
 // inner solve's half-precision preconditioner) and the operator's index form.
@@ -334,13 +348,9 @@ void launch_pm2_c(hipStream_t s, const Scal *sc, const Op<T> &A, const C *AL, in
         launch_pm2_ix<T, C>(s, G2, sc, A, AL + o0, ix32(A, o0), AL + o1, ix32(A, o1), in, out);
 }
 
+// Stages j and j+1 of LDUSUB fused (k_pm2_v2).
 template <typename T>
-void launch_pm2(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j, const T *in, T *out) {
-    if (!use_v2<T>()) {
-        const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
-        k_pm2<T><<<g.G, BS, 0, s>>>(sc, A.N, g.R, A.AL + o0, A.LL + o0, A.AL + o1, A.LL + o1, in, out);
-        return;
-    }
+void launch_pm2(hipStream_t s, const Scal *sc, const Op<T> &A, int j, const T *in, T *out) {
     if constexpr (sizeof(T) == 4) {
         if (A.ALh) {
             launch_pm2_c<T, _Float16>(s, sc, A, A.ALh, j, in, out);
@@ -348,25 +358,6 @@ void launch_pm2(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j, con
         }
     }
     launch_pm2_c<T, T>(s, sc, A, A.AL, j, in, out);
-}
-
-template <typename T, int NL>
-void launch_spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
-                 const T *v0, const T *v1, const T *v2, T *vout, T *y) {
-    switch (mode) {
-    case 0:
-        k_spmv<T, NL, 0><<<g.G, BS, 0, s>>>(sc, A.N, A.lda, g.R, A.AL, A.LL, A.D, v0, v1, v2, vout, y);
-        break;
-    case 1:
-        k_spmv<T, NL, 1><<<g.G, BS, 0, s>>>(sc, A.N, A.lda, g.R, A.AL, A.LL, A.D, v0, v1, v2, vout, y);
-        break;
-    case 2:
-        k_spmv<T, NL, 2><<<g.G, BS, 0, s>>>(sc, A.N, A.lda, g.R, A.AL, A.LL, A.D, v0, v1, v2, vout, y);
-        break;
-    default:
-        k_spmv<T, NL, 3><<<g.G, BS, 0, s>>>(sc, A.N, A.lda, g.R, A.AL, A.LL, A.D, v0, v1, v2, vout, y);
-        break;
-    }
 }
 
 template <typename T, int NT, typename IX, bool ST, typename C>
@@ -380,23 +371,39 @@ void launch_pm_v2(hipStream_t s, int G2, const Scal *sc, int N, const C *a0, IX 
     }
 }
 
+template <int NT, typename IX, bool ST, typename C>
+void launch_pm_v4(hipStream_t s, int G4, const Scal *sc, int N, const C *a0, IX x0, const C *a1, IX x1,
+                  const float *in, float *out, int dot, const float *w, double *part) {
+    switch (dot) {
+    case 0: k_pm_v4<NT, 0, IX, ST, C><<<G4, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    case 1: k_pm_v4<NT, 1, IX, ST, C><<<G4, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    case 2: k_pm_v4<NT, 2, IX, ST, C><<<G4, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    default: k_pm_v4<NT, 3, IX, ST, C><<<G4, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    }
+}
+
+// One PM stage in the form for T and the operator; returns the grid (the
+// number of dot partials a dot-carrying stage wrote).
 template <typename T, int NT, typename C, typename IX>
-void launch_pm_stage_ix(hipStream_t s, int G2, const Scal *sc, const Op<T> &A, const C *a0, IX x0, const C *a1,
+int launch_pm_stage_ix(hipStream_t s, int G2, const Scal *sc, const Op<T> &A, const C *a0, IX x0, const C *a1,
                         IX x1, const T *in, T *out, int dot, const T *w, double *part) {
-    if constexpr (sizeof(T) == 4 && NT == 1) {
-        if (dot == 0 && h_v4_32) {
+    if constexpr (sizeof(T) == 4) {
+        // the dot-carrying stages always take four rows a thread; the single
+        // stages unless CLOWNS_V4_32 = 0 (bit-identical either way)
+        if (dot != 0 || A.v4) {
             const int G4 = geo_of(A.N, 4).G;
             if (A.st_pm)
-                k_pm_v4<IX, true, C><<<G4, BS, 0, s>>>(sc, A.N, a0, x0, in, out);
+                launch_pm_v4<NT, IX, true, C>(s, G4, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
             else
-                k_pm_v4<IX, false, C><<<G4, BS, 0, s>>>(sc, A.N, a0, x0, in, out);
-            return;
+                launch_pm_v4<NT, IX, false, C>(s, G4, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
+            return G4;
         }
     }
     if (A.st_pm)
         launch_pm_v2<T, NT, IX, true, C>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part, A.order);
     else
         launch_pm_v2<T, NT, IX, false, C>(s, G2, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part, A.order);
+    return G2;
 }
 
 template <typename T, int NT, typename C>
@@ -407,65 +414,54 @@ int launch_pm_stage_c(hipStream_t s, const Scal *sc, const Op<T> &A, const C *AL
     const int G2 = geo_of(A.N, 2).G;
     if (has8(A)) {
         const Ix8 x0 = ix8(A, o0), x1 = NT == 2 ? ix8(A, o1) : x0;
-        launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
-    } else if (has16(A)) {
-        const Ix16 x0 = ix16(A, o0), x1 = NT == 2 ? ix16(A, o1) : x0;
-        launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
-    } else {
-        const Ix32 x0 = ix32(A, o0), x1 = NT == 2 ? ix32(A, o1) : x0;
-        launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
+        return launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
     }
-    return G2;
+    if (has16(A)) {
+        const Ix16 x0 = ix16(A, o0), x1 = NT == 2 ? ix16(A, o1) : x0;
+        return launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
+    }
+    const Ix32 x0 = ix32(A, o0), x1 = NT == 2 ? ix32(A, o1) : x0;
+    return launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
 }
 
+// One stage of LDUSUB (NT = 2: the two-term stage 5/6); returns the grid
+// whose partials a dot-carrying stage wrote.
 template <typename T, int NT>
-int launch_pm_stage(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int j,
-                    const T *in, T *out, int dot, const T *w, double *part) {
-    if (use_v2<T>()) {
-        if constexpr (sizeof(T) == 4) {
-            if (A.ALh) return launch_pm_stage_c<T, NT, _Float16>(s, sc, A, A.ALh, j, in, out, dot, w, part);
-        }
-        return launch_pm_stage_c<T, NT, T>(s, sc, A, A.AL, j, in, out, dot, w, part);
+int launch_pm_stage(hipStream_t s, const Scal *sc, const Op<T> &A, int j, const T *in, T *out, int dot,
+                    const T *w, double *part) {
+    if constexpr (sizeof(T) == 4) {
+        if (A.ALh) return launch_pm_stage_c<T, NT, _Float16>(s, sc, A, A.ALh, j, in, out, dot, w, part);
     }
-    const size_t o0 = (size_t)j * A.lda, o1 = (size_t)(j + 1) * A.lda;
-    const T *a0 = A.AL + o0, *a1 = NT == 2 ? A.AL + o1 : nullptr;
-    const int *l0 = A.LL + o0, *l1 = NT == 2 ? A.LL + o1 : nullptr;
-    switch (dot) {
-    case 0: k_pm<T, NT, 0><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
-    case 1: k_pm<T, NT, 1><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
-    case 2: k_pm<T, NT, 2><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
-    default: k_pm<T, NT, 3><<<g.G, BS, 0, s>>>(sc, A.N, g.R, a0, l0, a1, l1, in, out, w, part); break;
-    }
-    return g.G;
+    return launch_pm_stage_c<T, NT, T>(s, sc, A, A.AL, j, in, out, dot, w, part);
 }
 
 // LDUSUB (pcgpme_para.F90:263-320): out = PM*in through scratch y, t.
 // `in` may be `out`; neither may be y or t except in == y.
 template <typename T>
-int apply_pm(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, const T *in,
-             T *out, T *y, T *t, int dot, const T *w, double *part) {
+int apply_pm(hipStream_t s, const Scal *sc, const Op<T> &A, const T *in, T *out, T *y, T *t, int dot, const T *w,
+             double *part) {
     if (A.fuse) {
         // stages (1,2) and (3,4) fused pairwise; 5/6 (two-term) carries the dots
-        launch_pm2<T>(s, g, sc, A, 0, in, t);
+        launch_pm2<T>(s, sc, A, 0, in, t);
         if (A.NL == 6) {
-            launch_pm2<T>(s, g, sc, A, 2, t, y);
-            return launch_pm_stage<T, 2>(s, g, sc, A, 4, y, out, dot, w, part);
+            launch_pm2<T>(s, sc, A, 2, t, y);
+            return launch_pm_stage<T, 2>(s, sc, A, 4, y, out, dot, w, part);
         }
         // NL = 4: stage 3 alone, then stage 4 with the dots
-        launch_pm_stage<T, 1>(s, g, sc, A, 2, t, y, 0, w, part);
-        return launch_pm_stage<T, 1>(s, g, sc, A, 3, y, out, dot, w, part);
+        launch_pm_stage<T, 1>(s, sc, A, 2, t, y, 0, w, part);
+        return launch_pm_stage<T, 1>(s, sc, A, 3, y, out, dot, w, part);
     }
     if (A.NL == 6) {
-        launch_pm_stage<T, 1>(s, g, sc, A, 0, in, t, 0, w, part);
-        launch_pm_stage<T, 1>(s, g, sc, A, 1, t, y, 0, w, part);
-        launch_pm_stage<T, 1>(s, g, sc, A, 2, y, t, 0, w, part);
-        launch_pm_stage<T, 1>(s, g, sc, A, 3, t, y, 0, w, part);
-        return launch_pm_stage<T, 2>(s, g, sc, A, 4, y, out, dot, w, part);
+        launch_pm_stage<T, 1>(s, sc, A, 0, in, t, 0, w, part);
+        launch_pm_stage<T, 1>(s, sc, A, 1, t, y, 0, w, part);
+        launch_pm_stage<T, 1>(s, sc, A, 2, y, t, 0, w, part);
+        launch_pm_stage<T, 1>(s, sc, A, 3, t, y, 0, w, part);
+        return launch_pm_stage<T, 2>(s, sc, A, 4, y, out, dot, w, part);
     }
-    launch_pm_stage<T, 1>(s, g, sc, A, 0, in, t, 0, w, part);
-    launch_pm_stage<T, 1>(s, g, sc, A, 1, t, y, 0, w, part);
-    launch_pm_stage<T, 1>(s, g, sc, A, 2, y, t, 0, w, part);
-    return launch_pm_stage<T, 1>(s, g, sc, A, 3, t, out, dot, w, part);
+    launch_pm_stage<T, 1>(s, sc, A, 0, in, t, 0, w, part);
+    launch_pm_stage<T, 1>(s, sc, A, 1, t, y, 0, w, part);
+    launch_pm_stage<T, 1>(s, sc, A, 2, y, t, 0, w, part);
+    return launch_pm_stage<T, 1>(s, sc, A, 3, t, out, dot, w, part);
 }
 
 // Time the first four PM stages of an LDUSUB application as two fused pairs
@@ -481,25 +477,24 @@ int apply_pm(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, const T *in,
 inline bool tuned_size(int N, int decN) { return decN > 0 && std::abs(N - decN) <= decN / 16; }
 
 template <typename T> bool tune_fuse(clowns_solver *h, const Op<T> &A, VecSet<T> &V) {
-    if (h_fuse_pm >= 0) return h_fuse_pm == 1;
+    if (h->kn.fuse_pm >= 0) return h->kn.fuse_pm == 1;
     int &dec = sizeof(T) == 4 ? h->fuse32 : h->fuse64;
     int &decN = sizeof(T) == 4 ? h->fuse32_N : h->fuse64_N;
     if (dec >= 0 && tuned_size(A.N, decN)) return dec == 1;
     hipStream_t s = h->stream;
     Scal *sc = h->sc.p;
-    const Geo g = geo_of(A.N, h->R);
     const int npairs = A.NL == 6 ? 2 : 1;
     k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1, 0);
     auto fused = [&]() {
-        launch_pm2<T>(s, g, sc, A, 0, V.y.p, V.t.p);
-        if (npairs == 2) launch_pm2<T>(s, g, sc, A, 2, V.t.p, V.e.p);
+        launch_pm2<T>(s, sc, A, 0, V.y.p, V.t.p);
+        if (npairs == 2) launch_pm2<T>(s, sc, A, 2, V.t.p, V.e.p);
     };
     auto single = [&]() {
-        launch_pm_stage<T, 1>(s, g, sc, A, 0, V.y.p, V.t.p, 0, nullptr, h->part.p);
-        launch_pm_stage<T, 1>(s, g, sc, A, 1, V.t.p, V.e.p, 0, nullptr, h->part.p);
+        launch_pm_stage<T, 1>(s, sc, A, 0, V.y.p, V.t.p, 0, nullptr, h->part.p);
+        launch_pm_stage<T, 1>(s, sc, A, 1, V.t.p, V.e.p, 0, nullptr, h->part.p);
         if (npairs == 2) {
-            launch_pm_stage<T, 1>(s, g, sc, A, 2, V.e.p, V.t.p, 0, nullptr, h->part.p);
-            launch_pm_stage<T, 1>(s, g, sc, A, 3, V.t.p, V.e.p, 0, nullptr, h->part.p);
+            launch_pm_stage<T, 1>(s, sc, A, 2, V.e.p, V.t.p, 0, nullptr, h->part.p);
+            launch_pm_stage<T, 1>(s, sc, A, 3, V.t.p, V.e.p, 0, nullptr, h->part.p);
         }
     };
     float best[2] = {1e30f, 1e30f};
@@ -518,7 +513,7 @@ template <typename T> bool tune_fuse(clowns_solver *h, const Op<T> &A, VecSet<T>
     }
     dec = best[0] < best[1] ? 1 : 0;
     decN = A.N;
-    if (h_verbose)
+    if (h->kn.verbose)
         fprintf(stderr, "clowns_pcg: N=%d %s PM stages 1-%d: fused pairs %.2f us, singles %.2f us -> %s\n", A.N,
                 sizeof(T) == 4 ? "FP32" : "FP64", 2 * npairs, 1e3f * best[0], 1e3f * best[1],
                 dec ? "fused" : "single");
@@ -526,61 +521,61 @@ template <typename T> bool tune_fuse(clowns_solver *h, const Op<T> &A, VecSet<T>
 }
 
 template <typename T, int NL, typename IX, bool ST>
-void launch_spmv_v2_st(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int mode, const T *v0,
-                       const T *v1, const T *v2, T *vout, T *y) {
+void launch_spmv_v2_st(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int mode, const T *x, const T *v1,
+                       T *y) {
     const int G2 = geo_of(A.N, 2).G;
-    switch (mode) {
-    case 0: k_spmv_v2<T, NL, 0, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y, A.order); break;
-    case 1: k_spmv_v2<T, NL, 1, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y, A.order); break;
-    case 2: k_spmv_v2<T, NL, 2, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y, A.order); break;
-    default: k_spmv_v2<T, NL, 3, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, v0, v1, v2, vout, y, A.order); break;
-    }
+    if (mode == 0)
+        k_spmv_v2<T, NL, 0, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order);
+    else
+        k_spmv_v2<T, NL, 3, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order);
+}
+
+template <typename IX, typename CT>
+void launch_rows32(hipStream_t s, const Scal *sc, const Op<float> &A, IX ix, const CT *Lr, const float *x,
+                   float *y) {
+    const int G2 = geo_of(A.N, 2).G;
+    if (A.st_spmv)
+        k_spmv_rows32<IX, CT, true><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y);
+    else
+        k_spmv_rows32<IX, CT, false><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y);
 }
 
 template <typename T, int NL, typename IX>
-void launch_spmv_v2(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int mode, const T *v0,
-                    const T *v1, const T *v2, T *vout, T *y) {
-    if constexpr (sizeof(T) == 4 && NL == 6 && std::is_same<IX, Ix16>::value) {
+void launch_spmv_v2(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int mode, const T *x, const T *v1,
+                    T *y) {
+    if constexpr (sizeof(T) == 4 && NL == 6 && !std::is_same<IX, Ix32>::value) {
         if (mode == 0 && A.ALr) {
-            const int G2 = geo_of(A.N, 2).G;
-            if (A.st_spmv)
-                k_spmv_rows32<true><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, A.Lr, ix, A.D, v0, y);
-            else
-                k_spmv_rows32<false><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, A.Lr, ix, A.D, v0, y);
-            return;
+            if constexpr (std::is_same<IX, Ix8>::value) {
+                if (A.Lr8) return launch_rows32(s, sc, A, ix, A.Lr8, x, y);
+            } else {
+                return launch_rows32(s, sc, A, ix, A.Lr16, x, y);
+            }
         }
     }
     if (A.st_spmv)
-        launch_spmv_v2_st<T, NL, IX, true>(s, sc, A, ix, mode, v0, v1, v2, vout, y);
+        launch_spmv_v2_st<T, NL, IX, true>(s, sc, A, ix, mode, x, v1, y);
     else
-        launch_spmv_v2_st<T, NL, IX, false>(s, sc, A, ix, mode, v0, v1, v2, vout, y);
+        launch_spmv_v2_st<T, NL, IX, false>(s, sc, A, ix, mode, x, v1, y);
 }
 
-template <typename T, int NL>
-void launch_spmv_v2(hipStream_t s, const Scal *sc, const Op<T> &A, int mode, const T *v0, const T *v1,
-                    const T *v2, T *vout, T *y) {
-    if (has8(A))
-        launch_spmv_v2<T, NL, Ix8>(s, sc, A, ix8(A, 0), mode, v0, v1, v2, vout, y);
-    else if (has16(A))
-        launch_spmv_v2<T, NL, Ix16>(s, sc, A, ix16(A, 0), mode, v0, v1, v2, vout, y);
-    else
-        launch_spmv_v2<T, NL, Ix32>(s, sc, A, ix32(A, 0), mode, v0, v1, v2, vout, y);
-}
-
+// AXSUB: y = A x (mode 0) or the true residual y = v1 - A x (mode 3).
 template <typename T>
-void spmv(hipStream_t s, Geo g, const Scal *sc, const Op<T> &A, int mode,
-          const T *v0, const T *v1, const T *v2, T *vout, T *y) {
-    if (use_v2_vec<T>()) {
-        if (A.NL == 6)
-            launch_spmv_v2<T, 6>(s, sc, A, mode, v0, v1, v2, vout, y);
+void spmv(hipStream_t s, const Scal *sc, const Op<T> &A, int mode, const T *x, const T *v1, T *y) {
+    if (A.NL == 6) {
+        if (has8(A))
+            launch_spmv_v2<T, 6, Ix8>(s, sc, A, ix8(A, 0), mode, x, v1, y);
+        else if (has16(A))
+            launch_spmv_v2<T, 6, Ix16>(s, sc, A, ix16(A, 0), mode, x, v1, y);
         else
-            launch_spmv_v2<T, 4>(s, sc, A, mode, v0, v1, v2, vout, y);
-        return;
+            launch_spmv_v2<T, 6, Ix32>(s, sc, A, ix32(A, 0), mode, x, v1, y);
+    } else {
+        if (has8(A))
+            launch_spmv_v2<T, 4, Ix8>(s, sc, A, ix8(A, 0), mode, x, v1, y);
+        else if (has16(A))
+            launch_spmv_v2<T, 4, Ix16>(s, sc, A, ix16(A, 0), mode, x, v1, y);
+        else
+            launch_spmv_v2<T, 4, Ix32>(s, sc, A, ix32(A, 0), mode, x, v1, y);
     }
-    if (A.NL == 6)
-        launch_spmv<T, 6>(s, g, sc, A, mode, v0, v1, v2, vout, y);
-    else
-        launch_spmv<T, 4>(s, g, sc, A, mode, v0, v1, v2, vout, y);
 }
 
 // Kernel forms as bits: bit 0 streamed PM stages, bit 1 streamed SpMV, bit 2
@@ -592,12 +587,12 @@ template <typename T> void set_forms(Op<T> &A, int f) {
     A.ix8 = (f & 4) != 0 && can_ix8(A);
 }
 // The environment's forcing, which the stored decisions are keyed on.
-inline int force_key() { return (h_stream + 1) | ((h_ll8 + 1) << 2); }
+inline int force_key(const clowns_solver *h) { return (h->kn.stream + 1) | ((h->kn.ll8 + 1) << 2); }
 // Bits fixed for operator A (mask) and their values.
-template <typename T> void fixed_forms(const Op<T> &A, int &mask, int &val) {
+template <typename T> void fixed_forms(const clowns_solver *h, const Op<T> &A, int &mask, int &val) {
     mask = val = 0;
-    if (h_stream >= 0) mask |= 3, val |= 3 * h_stream;
-    if (!can_ix8(A) || h_ll8 >= 0) mask |= 4, val |= (can_ix8(A) && h_ll8 == 1) ? 4 : 0;
+    if (h->kn.stream >= 0) mask |= 3, val |= 3 * h->kn.stream;
+    if (!can_ix8(A) || h->kn.ll8 >= 0) mask |= 4, val |= (can_ix8(A) && h->kn.ll8 == 1) ? 4 : 0;
 }
 // The stored decisions, if they hold for A: forms in *f.  The index-code
 // decision is kept apart from the streamed forms', so an operator without
@@ -605,12 +600,12 @@ template <typename T> void fixed_forms(const Op<T> &A, int &mask, int &val) {
 // operator.
 template <typename T> bool stored_forms(const clowns_solver *h, const Op<T> &A, int *f) {
     int mask, val;
-    fixed_forms(A, mask, val);
+    fixed_forms(h, A, mask, val);
     const bool f32 = sizeof(T) == 4;
     const int st = f32 ? h->st32 : h->st64, stN = f32 ? h->st32_N : h->st64_N;
     const int stF = f32 ? h->st32_f : h->st64_f;
     const int ix = f32 ? h->ix32 : h->ix64, ixN = f32 ? h->ix32_N : h->ix64_N;
-    const bool key = stF == force_key();
+    const bool key = stF == force_key(h);
     const bool st_ok = (mask & 3) == 3 || (st >= 0 && tuned_size(A.N, stN) && key);
     const bool ix_ok = (mask & 4) != 0 || (ix >= 0 && tuned_size(A.N, ixN) && key);
     if (!st_ok || !ix_ok) return false;
@@ -628,7 +623,7 @@ template <typename T> bool stored_forms(const clowns_solver *h, const Op<T> &A, 
 // favours the streamed form, which finds data the cached form left behind.)
 template <typename T> void tune_stream(clowns_solver *h, Op<T> &A, VecSet<T> &V) {
     int mask, fval, have;
-    fixed_forms(A, mask, fval);
+    fixed_forms(h, A, mask, fval);
     if (stored_forms(h, A, &have)) {
         set_forms(A, have);
         return;
@@ -641,25 +636,23 @@ template <typename T> void tune_stream(clowns_solver *h, Op<T> &A, VecSet<T> &V)
     int &ixN = f32 ? h->ix32_N : h->ix64_N;
     hipStream_t s = h->stream;
     Scal *sc = h->sc.p;
-    const Geo g = geo_of(A.N, h->R);
     k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1, 0);
     // two Bi-CGSTAB iterations' kernels in run_bicgstab's order, without the
     // one-block finalizations (the vectors are scratch here: the solve
     // initializes every one of them after tuning)
-    const int G2 = geo_of(A.N, 2).G;
+    const int Gv = geo_of(A.N, VecOf<T>::R).G;
     double *part = h->part.p;
     auto seq = [&](int form) {
         Op<T> B = A;
         set_forms(B, form);
         for (int it = 0; it < 2; ++it) {
-            if (use_v2_vec<T>()) k_pupd_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.p0.p, V.r.p, V.q.p);
-            spmv<T>(s, g, sc, B, 0, V.p0.p, nullptr, nullptr, nullptr, V.y.p);
-            apply_pm<T>(s, g, sc, B, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
-            if (use_v2_vec<T>()) k_eupd_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.r.p, V.q.p, V.e.p);
-            spmv<T>(s, g, sc, B, 0, V.e.p, nullptr, nullptr, nullptr, V.y.p);
-            apply_pm<T>(s, g, sc, B, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
-            if (use_v2_vec<T>())
-                k_update_v2<T><<<G2, BS, 0, s>>>(sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
+            k_pupd_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.p0.p, V.r.p, V.q.p);
+            spmv<T>(s, sc, B, 0, V.p0.p, nullptr, V.y.p);
+            apply_pm<T>(s, sc, B, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
+            k_eupd_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.r.p, V.q.p, V.e.p);
+            spmv<T>(s, sc, B, 0, V.e.p, nullptr, V.y.p);
+            apply_pm<T>(s, sc, B, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
+            k_update_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
         }
     };
     float best[8];
@@ -693,8 +686,8 @@ template <typename T> void tune_stream(clowns_solver *h, Op<T> &A, VecSet<T> &V)
         ix = (f_best >> 2) & 1;
         ixN = A.N;
     }
-    stF = force_key();
-    if (h_verbose) {
+    stF = force_key(h);
+    if (h->kn.verbose) {
         fprintf(stderr, "clowns_pcg: N=%d %s two iterations (us):", A.N, f32 ? "FP32" : "FP64");
         for (int f = 0; f < 8; ++f)
             if (best[f] < 1e29f)
@@ -713,65 +706,104 @@ template <typename T> void known_stream(const clowns_solver *h, Op<T> &A) {
     if (stored_forms(h, A, &f)) set_forms(A, f);
 }
 
+// How many iterations to enqueue next.  The device decides convergence
+// (k_fin_beta sets `done`, after which every kernel returns at once), so the
+// host only chooses how far ahead of the last status it has seen to run:
+// too short and the GPU idles while the host waits, too long and the
+// iterations queued past convergence each cost a dozen empty launches.  From
+// the residual at the last two statuses the host predicts the iteration at
+// which sqrt(RN/BN) reaches EPS (Bi-CGSTAB converges about geometrically)
+// and queues up to it; without a prediction the batch doubles, up to 32.
+struct PollPlan {
+    int n = 8;                          // last batch length
+    int k0 = 0, k1 = 0;                 // iterations of the last two statuses seen
+    double e0 = 0.0, e1 = 0.0;          // their ERR
+    // status after iteration k: ERR err, target eps
+    void seen(int k, double err) {
+        if (k1 > 0 && k - k1 < 4) return;  // too close to the last one for a slope
+        k0 = k1;
+        e0 = e1;
+        k1 = k;
+        e1 = err;
+    }
+    // length of the batch that starts at iteration `next` (statuses seen up
+    // to k1; iterations k1+1 .. next-1 already queued)
+    int next_len(int next, double eps) {
+        int len = std::min(32, 2 * n);
+        if (k0 > 0 && e0 > 0.0 && e1 > 0.0 && e1 < e0 && eps > 0.0) {
+            const double rate = std::log(e0 / e1) / (double)(k1 - k0);
+            const double left = e1 > eps ? std::log(e1 / eps) / rate : 0.0;
+            const double stop = (double)k1 + left;  // predicted last iteration
+            const double want = std::ceil(stop - (double)next + 1.0);
+            len = !(want < 32.0) ? 32 : (want <= 2.0 ? 2 : (int)want);  // NaN: 32
+        }
+        n = len;
+        return len;
+    }
+};
+
 // The PCGPME iteration (pcgpme_para.F90:148-247) on already initialised
 // vectors: V.p0 = V.r = V.rh = r0, V.x = x0, scalars BN/C1 set by k_fin_init.
-// Returns after the device has set `done`.
+// Batches of iterations are queued one ahead of the status the host reads
+// (two pinned status slots), so the stream never drains while the host
+// looks.  Returns after the device has set `done`; the final scalars are in
+// h->h_sc[sc_index].  eps: the run's stopping value (for the batch plan only).
 template <typename T>
-int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc,
-                 int itermax, int sc_index) {
+int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc, int itermax, int sc_index,
+                 double eps) {
     hipStream_t s = h->stream;
-    const Geo g = geo_of(A.N, h->R);
     double *part = h->part.p;
-    T *pc = V.p0.p, *pn = V.p1.p;
-    int poll = h->poll;
+    const int Gv = geo_of(A.N, VecOf<T>::R).G;
+    auto iteration = [&](int K) {
+        // Q = PM*A*P (:149-151), after P = R + BETA*(P - AMU*Q) (:244) for K > 1.
+        if (K > 1) k_pupd_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.p0.p, V.r.p, V.q.p);
+        spmv<T>(s, sc, A, 0, V.p0.p, nullptr, V.y.p);
+        const int Gq = apply_pm<T>(s, sc, A, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
+        k_fin_alpha<<<1, FIN_BS, 0, s>>>(sc, part, Gq);
+        // V = PM*A*E with E = R - ALP*Q (:173-181).
+        k_eupd_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.r.p, V.q.p, V.e.p);
+        spmv<T>(s, sc, A, 0, V.e.p, nullptr, V.y.p);
+        const int Gw = apply_pm<T>(s, sc, A, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
+        k_fin_omega<<<1, FIN_BS, 0, s>>>(sc, part, Gw);
+        k_update_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
+        k_fin_beta<<<1, FIN_BS, 0, s>>>(sc, part, Gv, K);
+    };
+    Scal *slot[2] = {h->h_sc + 2, h->h_sc + 3};
+    PollPlan plan;
+    plan.n = std::max(1, h->poll) / 2;
     int K = 1;
-    while (true) {
-        int kend = std::min(itermax, K + poll - 1);
-        for (; K <= kend; ++K) {
-            // Q = PM*A*P, with P = R + BETA*(P - AMU*Q) fused for K > 1.
-            if (K == 1 || !h->fuse_pupd) {
-                if (K > 1) {
-                    if (use_v2_vec<T>())
-                        k_pupd_v2<T><<<geo_of(A.N, 2).G, BS, 0, s>>>(sc, A.N, pc, V.r.p, V.q.p);
-                    else
-                        k_pupd<T><<<geo_of(A.N, 1).G, BS, 0, s>>>(sc, A.N, pc, V.r.p, V.q.p);
-                }
-                spmv<T>(s, g, sc, A, 0, pc, nullptr, nullptr, nullptr, V.y.p);
-            } else {
-                spmv<T>(s, g, sc, A, 1, pc, V.r.p, V.q.p, pn, V.y.p);
-                std::swap(pc, pn);
-            }
-            const int Gq = apply_pm<T>(s, g, sc, A, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
-            k_fin_alpha<<<1, FIN_BS, 0, s>>>(sc, part, Gq);
-            // V = PM*A*E with E = R - ALP*Q fused.
-            if (h->split_e && use_v2_vec<T>()) {
-                k_eupd_v2<T><<<geo_of(A.N, 2).G, BS, 0, s>>>(sc, A.N, V.r.p, V.q.p, V.e.p);
-                spmv<T>(s, g, sc, A, 0, V.e.p, nullptr, nullptr, nullptr, V.y.p);
-            } else {
-                spmv<T>(s, g, sc, A, 2, nullptr, V.r.p, V.q.p, V.e.p, V.y.p);
-            }
-            const int Gv = apply_pm<T>(s, g, sc, A, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
-            k_fin_omega<<<1, FIN_BS, 0, s>>>(sc, part, Gv);
-            int Gu = g.G;
-            if (use_v2_vec<T>()) {
-                Gu = geo_of(A.N, 2).G;
-                k_update_v2<T><<<Gu, BS, 0, s>>>(sc, A.N, V.x.p, pc, V.e.p, V.v.p, V.r.p, V.rh.p, part);
-            } else {
-                k_update<T><<<g.G, BS, 0, s>>>(sc, A.N, g.R, V.x.p, pc, V.e.p, V.v.p, V.r.p, V.rh.p, part);
-            }
-            k_fin_beta<<<1, FIN_BS, 0, s>>>(sc, part, Gu, K);
-        }
+    auto enqueue = [&](int len, int si) -> int {
+        const int kend = std::min(itermax, K + len - 1);
+        for (; K <= kend; ++K) iteration(K);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(h->h_sc + sc_index, sc, sizeof(Scal), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        const Scal &hs = h->h_sc[sc_index];
-        if (hs.done) break;
-        if (K > itermax) break; // cannot happen: the device sets done at itermax
-        // Poll less often while far from convergence.
-        poll = std::min(32, poll * 2);
+        HIPCHK(hipMemcpyAsync(slot[si], sc, sizeof(Scal), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(h->evp[si], s));
+        return CLOWNS_OK;
+    };
+    int rc, cur = 0;
+    if ((rc = enqueue(plan.next_len(K, eps), cur))) return rc;
+    for (;;) {
+        const bool ahead = K <= itermax;
+        if (ahead && (rc = enqueue(plan.next_len(K, eps), cur ^ 1))) return rc;
+        HIPCHK(hipEventSynchronize(h->evp[cur]));
+        const Scal hs = *slot[cur];
+        if (hs.done) {
+            // a batch queued behind the converged one only returns at once
+            h->h_sc[sc_index] = hs;
+            if (ahead) {
+                HIPCHK(hipEventSynchronize(h->evp[cur ^ 1]));
+                h->h_sc[sc_index] = *slot[cur ^ 1];
+            }
+            break;
+        }
+        if (!ahead) {
+            // unreachable: the device sets done at itermax
+            h->h_sc[sc_index] = hs;
+            break;
+        }
+        plan.seen(hs.K, hs.ERR);
+        cur ^= 1;
     }
-    // Leave the current search direction in p0 for callers that inspect it.
-    if (pc != V.p0.p) std::swap(V.p0, V.p1);
     return CLOWNS_OK;
 }
 
@@ -847,19 +879,24 @@ int ensure_f32(clowns_solver *h) {
     int rc;
     const size_t nm = (size_t)h->lda * h->NL;
     if ((rc = h->AL32.ensure(nm)) || (rc = h->D32.ensure(h->N))) return rc;
-    k_to_f32<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->NL, h->AL.p, h->D.p,
-                                                   h->AL32.p, h->D32.p);
-    if (h_pm_half) {
+    hipStream_t s = h->stream;
+    k_to_f32<<<cdiv(h->N, BS), BS, 0, s>>>(h->N, h->lda, h->NL, h->AL.p, h->D.p, h->AL32.p, h->D32.p);
+    if (h->kn.pm_half) {
         if ((rc = h->ALh.ensure(nm))) return rc;
-        k_to_f16<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->NL, h->AL.p, h->ALh.p);
+        k_to_f16<<<cdiv(h->N, BS), BS, 0, s>>>(h->N, h->lda, h->NL, h->AL.p, h->ALh.p);
     }
-    h->have_h = h_pm_half;
-    h->have_rows = false;
-    if (h_rows32 && h->NL == 6 && h->ll16_ok) {
-        if ((rc = h->ALr32.ensure((size_t)6 * h->N)) || (rc = h->Lr16.ensure((size_t)6 * h->N))) return rc;
-        k_to_rows32<<<cdiv(h->N, BS), BS, 0, h->stream>>>(h->N, h->lda, h->AL.p, h->LL16.p, h->ALr32.p,
-                                                         h->Lr16.p);
+    h->have_h = h->kn.pm_half;
+    h->have_rows = h->have_rows8 = false;
+    if (h->kn.rows32 && h->NL == 6 && h->ll16_ok) {
+        const size_t n6 = (size_t)6 * h->N;
+        if ((rc = h->ALr32.ensure(n6)) || (rc = h->Lr16.ensure(n6)) || (rc = h->nonunit.ensure(1))) return rc;
+        if (h->ll8_ok && (rc = h->Lr8.ensure(n6))) return rc;
+        fill<int>(s, h->nonunit.p, 1, 0);
+        k_to_rows32<<<cdiv(h->N, BS), BS, 0, s>>>(h->N, h->lda, h->AL.p, h->LL16.p, h->ll8_ok ? h->LL8.p : nullptr,
+                                                  h->D32.p, h->ALr32.p, h->Lr16.p, h->ll8_ok ? h->Lr8.p : nullptr,
+                                                  h->nonunit.p);
         h->have_rows = true;
+        h->have_rows8 = h->ll8_ok;
     }
     HIPCHK(hipGetLastError());
     h->have32 = true;
@@ -871,7 +908,7 @@ int ensure_f32(clowns_solver *h) {
 int build_ll16(clowns_solver *h) {
     h->ll16_ok = false;
     h->ll16_wide = 0;
-    if (!h_ll16 || !h_v2_32 || !h_v2_64 || !h_v2_vec) return CLOWNS_OK;
+    if (!h->kn.ll16) return CLOWNS_OK;
     const int N = h->N, nblk = cdiv(N, LL16_ROWS);
     int rc;
     if ((rc = h->LL16.ensure((size_t)h->lda * h->NL)) || (rc = h->base16.ensure(nblk)) || (rc = h->flag.ensure(2)))
@@ -894,13 +931,12 @@ int build_ll16(clowns_solver *h) {
 int build_ll8(clowns_solver *h) {
     h->ll8_ok = false;
     h->ll8_wide = 0;
-    if (h_ll8 == 0 || !h->ll16_ok) return CLOWNS_OK;
-    // not needed where the FP64 solve was timed at this size and kept the
-    // 16-bit codes, and FP32 kept them too or has not been timed (then it
-    // keeps them as well): an assembly per time step need not rebuild them
-    const bool k64 = h->ix64 == 0 && tuned_size(h->N, h->ix64_N) && h->st64_f == force_key();
-    const bool t32 = h->ix32 >= 0 && tuned_size(h->N, h->ix32_N) && h->st32_f == force_key();
-    if (h_ll8 < 0 && k64 && (!t32 || h->ix32 == 0)) return CLOWNS_OK;
+    if (h->kn.ll8 == 0 || !h->ll16_ok) return CLOWNS_OK;
+    // not needed where both precisions were timed at this size and kept the
+    // 16-bit codes: an assembly per time step need not rebuild them
+    const bool k64 = h->ix64 == 0 && tuned_size(h->N, h->ix64_N) && h->st64_f == force_key(h);
+    const bool k32 = h->ix32 == 0 && tuned_size(h->N, h->ix32_N) && h->st32_f == force_key(h);
+    if (h->kn.ll8 < 0 && k64 && k32) return CLOWNS_OK;
     const int N = h->N, nblk = cdiv(N, LL8_ROWS);
     int rc;
     if ((rc = h->LL8.ensure((size_t)h->lda * h->NL)) || (rc = h->base8.ensure(nblk)) || (rc = h->flag.ensure(2)))
@@ -914,7 +950,7 @@ int build_ll8(clowns_solver *h) {
     HIPCHK(hipStreamSynchronize(s));
     h->ll8_ok = res[0] == 0;
     h->ll8_wide = res[1];
-    if (h_verbose)
+    if (h->kn.verbose)
         fprintf(stderr, "clowns_pcg: N=%d 8-bit neighbour codes: %d of %d slots wide\n", N, res[1], N * h->NL);
     return CLOWNS_OK;
 }
@@ -946,26 +982,45 @@ Op<float> op32(const clowns_solver *h) {
         A.LL8 = h->LL8.p;
         A.base8 = h->base8.p;
     }
+    A.v4 = h->kn.v4_32;
     if (h->have_rows) {
-        // the row-major SpMV reads 16-bit codes: the FP32 kernels keep them
-        // (8-bit codes are not offered to the tuner for this operator)
+        // the row-major SpMV: codes of the width the FP32 kernels run with
         A.ALr = h->ALr32.p;
-        A.Lr = h->Lr16.p;
-        A.LL8 = nullptr;
-        A.base8 = nullptr;
+        A.Lr16 = h->Lr16.p;
+        A.Lr8 = h->have_rows8 ? h->Lr8.p : nullptr;
+        A.nonunit = h->nonunit.p;
+        if (!h->have_rows8) {
+            A.LL8 = nullptr;
+            A.base8 = nullptr;
+        }
     }
     return A;
 }
 
+// Ghost rows [N, N+64) of every vector of a run read as zero, as the
+// reference's zeroed work arrays do (pcgpme_para.F90:59-75): a gather through
+// a padding slot (LL = 0 -> N+1+mod(I,64), :80) then adds AL * 0 even where a
+// caller's AL is not 0 there.  Rows past N keep whatever a larger earlier
+// solve left, so the ghost rows are cleared whenever N changes.
 template <typename T> int ensure_vecs(clowns_solver *h, VecSet<T> &V, int N) {
-    const size_t n = (size_t)n_pad(N);
-    const bool grew = V.r.n < n;
-    int rc = V.ensure(n);
+    int rc = V.ensure((size_t)n_pad(N));
     if (rc) return rc;
-    if (grew) {
-        // Ghost rows must read as zero in every vector.
-        DevBuf<T> *all[] = {&V.r, &V.rh, &V.p0, &V.p1, &V.q, &V.e, &V.v, &V.y, &V.t, &V.x};
-        for (auto *b : all) fill<T>(h->stream, b->p, b->n, T(0));
+    if (V.padN != N) {
+        T *all[] = {V.r.p, V.rh.p, V.p0.p, V.q.p, V.e.p, V.v.p, V.y.p, V.t.p, V.x.p};
+        for (T *b : all) fill<T>(h->stream, b + N, PAD, T(0));
+        V.padN = N;
+    }
+    return CLOWNS_OK;
+}
+
+// The scaled right-hand side of a run (the mixed solve gathers it in its
+// first PM application): N + 64 entries, ghost rows zero.
+int ensure_bs(clowns_solver *h, int N) {
+    int rc = h->bs.ensure((size_t)n_pad(N));
+    if (rc) return rc;
+    if (h->bs_padN != N) {
+        fill<double>(h->stream, h->bs.p + N, PAD, 0.0);
+        h->bs_padN = N;
     }
     return CLOWNS_OK;
 }
@@ -979,12 +1034,12 @@ int solve_fp64(clowns_solver *h, const Op<double> &A_in, const double *Wsc, cons
     hipStream_t s = h->stream;
     int rc;
     VecSet<double> &V = h->v64;
-    if ((rc = ensure_vecs(h, V, A_in.N)) || (rc = h->bs.ensure(n_pad(A_in.N)))) return rc;
+    if ((rc = ensure_vecs(h, V, A_in.N)) || (rc = ensure_bs(h, A_in.N))) return rc;
     Op<double> A = A_in;
     A.fuse = tune_fuse(h, A, V);
     tune_stream(h, A, V);
     const int N = A.N;
-    const Geo g = geo_of(N, h->R);
+    const int G1 = geo_of(N, 1).G;
     Scal *sc = h->sc.p;
     double *part = h->part.p;
     HIPCHK(hipEventRecord(h->ev0, s));
@@ -992,22 +1047,21 @@ int solve_fp64(clowns_solver *h, const Op<double> &A_in, const double *Wsc, cons
     // WK1 = W*B; X = x0.
     k_scale_b<<<cdiv(N, BS), BS, 0, s>>>(N, d_b, Wsc, V.p0.p, warm ? d_x : nullptr, V.x.p);
     // WK1 = PM*WK1 (:123).
-    apply_pm<double>(s, g, sc, A, V.p0.p, V.p0.p, V.y.p, V.t.p, 0, nullptr, part);
+    apply_pm<double>(s, sc, A, V.p0.p, V.p0.p, V.y.p, V.t.p, 0, nullptr, part);
     const double *wk3 = nullptr;
     if (warm) {
         // WK3 = PM*A*X (:126-127).
-        spmv<double>(s, g, sc, A, 0, V.x.p, nullptr, nullptr, nullptr, V.y.p);
-        apply_pm<double>(s, g, sc, A, V.y.p, V.q.p, V.y.p, V.t.p, 0, nullptr, part);
+        spmv<double>(s, sc, A, 0, V.x.p, nullptr, V.y.p);
+        apply_pm<double>(s, sc, A, V.y.p, V.q.p, V.y.p, V.t.p, 0, nullptr, part);
         wk3 = V.q.p;
     }
-    k_init<double><<<g.G, BS, 0, s>>>(sc, N, g.R, V.p0.p, wk3, V.r.p, V.rh.p, part);
-    k_fin_init<<<1, FIN_BS, 0, s>>>(sc, part, g.G);
+    k_init<double><<<G1, BS, 0, s>>>(sc, N, V.p0.p, wk3, V.r.p, V.rh.p, part);
+    k_fin_init<<<1, FIN_BS, 0, s>>>(sc, part, G1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(h->ev2, s));
-    if ((rc = run_bicgstab<double>(h, A, V, sc, itermax, 0))) return rc;
+    if ((rc = run_bicgstab<double>(h, A, V, sc, itermax, 0, eps))) return rc;
     HIPCHK(hipMemcpyAsync(d_x, V.x.p, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, s));
     HIPCHK(hipEventRecord(h->ev1, s));
-    HIPCHK(hipMemcpyAsync(h->h_sc, sc, sizeof(Scal), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     const Scal &hs = h->h_sc[0];
     if (info) {
@@ -1035,8 +1089,7 @@ int solve_mixed(clowns_solver *h, const Op<double> &A_in, const Op<float> &A32_i
     int rc;
     VecSet<double> &V = h->v64;
     VecSet<float> &F = h->v32;
-    if ((rc = ensure_vecs(h, V, A_in.N)) || (rc = ensure_vecs(h, F, A_in.N)) ||
-        (rc = h->bs.ensure(n_pad(A_in.N))))
+    if ((rc = ensure_vecs(h, V, A_in.N)) || (rc = ensure_vecs(h, F, A_in.N)) || (rc = ensure_bs(h, A_in.N)))
         return rc;
     Op<double> A = A_in;
     Op<float> A32 = A32_in;
@@ -1045,7 +1098,7 @@ int solve_mixed(clowns_solver *h, const Op<double> &A_in, const Op<float> &A32_i
     tune_stream(h, A, V);
     tune_stream(h, A32, F);
     const int N = A.N;
-    const Geo g = geo_of(N, h->R);
+    const int G1 = geo_of(N, 1).G;
     Scal *sc = h->sc.p, *outer = h->sc.p + 1;
     double *part = h->part.p;
     HIPCHK(hipEventRecord(h->ev0, s));
@@ -1053,7 +1106,7 @@ int solve_mixed(clowns_solver *h, const Op<double> &A_in, const Op<float> &A32_i
     k_reset<<<1, 64, 0, s>>>(sc, eps, itermax, 0);
     // bs = W*b ; BN = ||PM*bs||^2.
     k_scale_b<<<cdiv(N, BS), BS, 0, s>>>(N, d_b, Wsc, h->bs.p, warm ? d_x : nullptr, V.x.p);
-    const int Gb = apply_pm<double>(s, g, sc, A, h->bs.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
+    const int Gb = apply_pm<double>(s, sc, A, h->bs.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
     k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, Gb);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h->h_sc + 1, outer, sizeof(Scal), hipMemcpyDeviceToHost, s));
@@ -1073,8 +1126,8 @@ int solve_mixed(clowns_solver *h, const Op<double> &A_in, const Op<float> &A32_i
         HIPCHK(hipMemcpyAsync(outer, &init, sizeof(Scal), hipMemcpyHostToDevice, s));
         if (warm) {
             // r = PM*(bs - A x).
-            spmv<double>(s, g, sc, A, 3, V.x.p, h->bs.p, nullptr, nullptr, V.y.p);
-            const int Gr = apply_pm<double>(s, g, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
+            spmv<double>(s, sc, A, 3, V.x.p, h->bs.p, V.y.p);
+            const int Gr = apply_pm<double>(s, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
             k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, Gr);
         }
         const double tau_min = h->tau_min;
@@ -1096,17 +1149,17 @@ int solve_mixed(clowns_solver *h, const Op<double> &A_in, const Op<float> &A32_i
             k_reset<<<1, 64, 0, s>>>(sc, tau, std::max(1, itermax - total_inner), 0);
             k_rhs32<<<cdiv(N, BS), BS, 0, s>>>(N, outer, V.r.p, F.p0.p);
             fill<float>(s, F.x.p, (size_t)N, 0.f);
-            k_init<float><<<g.G, BS, 0, s>>>(sc, N, g.R, F.p0.p, nullptr, F.r.p, F.rh.p, part);
-            k_fin_init<<<1, FIN_BS, 0, s>>>(sc, part, g.G);
-            if ((rc = run_bicgstab<float>(h, A32, F, sc, std::max(1, itermax - total_inner), 0)))
+            k_init<float><<<G1, BS, 0, s>>>(sc, N, F.p0.p, nullptr, F.r.p, F.rh.p, part);
+            k_fin_init<<<1, FIN_BS, 0, s>>>(sc, part, G1);
+            if ((rc = run_bicgstab<float>(h, A32, F, sc, std::max(1, itermax - total_inner), 0, tau)))
                 return rc;
             total_inner += std::max(0, std::min(h->h_sc[0].kout, itermax));
             ++refinements;
             // x += ||r|| d ; r = PM*(bs - A x) ; ERR.
             k_correct<<<cdiv(N, BS), BS, 0, s>>>(N, outer, F.x.p, V.x.p);
             k_reset<<<1, 64, 0, s>>>(sc, eps, itermax, 0);
-            spmv<double>(s, g, sc, A, 3, V.x.p, h->bs.p, nullptr, nullptr, V.y.p);
-            const int Gr = apply_pm<double>(s, g, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
+            spmv<double>(s, sc, A, 3, V.x.p, h->bs.p, V.y.p);
+            const int Gr = apply_pm<double>(s, sc, A, V.y.p, V.r.p, V.y.p, V.t.p, 3, nullptr, part);
             k_fin_true<<<1, FIN_BS, 0, s>>>(outer, part, Gr);
             HIPCHK(hipGetLastError());
         }
@@ -1166,25 +1219,31 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
     // ordered with the caller's default-stream work.
     h->stream = (hipStream_t)stream;
     h->own_stream = false;
-    if (const char *e = getenv("CLOWNS_ROWS_PER_THREAD")) h->R = std::max(1, atoi(e));
-    if (const char *e = getenv("CLOWNS_POLL")) h->poll = std::max(1, atoi(e));
-    if (const char *e = getenv("CLOWNS_FUSE_PUPD")) h->fuse_pupd = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_SPLIT_E")) h->split_e = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_TAU_MIN")) h->tau_min = std::max(1e-8, atof(e));
-    if (const char *e = getenv("CLOWNS_CLASSIFY_HOST")) h->cls_force_host = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_WILL_MAX_ROUNDS")) h->will_max_rounds = std::max(1, atoi(e));
-    if (const char *e = getenv("CLOWNS_FUSE_PM")) h_fuse_pm = atoi(e) < 0 ? -1 : (atoi(e) != 0 ? 1 : 0);
-    if (const char *e = getenv("CLOWNS_STREAM")) h_stream = atoi(e) < 0 ? -1 : (atoi(e) != 0 ? 1 : 0);
-    if (const char *e = getenv("CLOWNS_V2_32")) h_v2_32 = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_V2_64")) h_v2_64 = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_V2VEC")) h_v2_vec = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_V4_32")) h_v4_32 = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_SPMV_ROWS")) h_rows32 = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_LL16")) h_ll16 = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_PM_HALF")) h_pm_half = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_BLOCK_ORDER")) h_block_order = atoi(e) != 0;
-    if (const char *e = getenv("CLOWNS_LL8")) h_ll8 = atoi(e) < 0 ? -1 : (atoi(e) != 0 ? 1 : 0);
-    if (const char *e = getenv("CLOWNS_VERBOSE")) h_verbose = atoi(e) != 0;
+    {
+        // getenv under a lock of its own (clowns_pcgpme creates its default
+        // solver while holding g_mu); the values stay with this solver
+        std::lock_guard<std::mutex> lock(g_env_mu);
+        auto flag = [](const char *name, bool &v) {
+            if (const char *e = getenv(name)) v = atoi(e) != 0;
+        };
+        auto tri = [](const char *name, int &v) {
+            if (const char *e = getenv(name)) v = atoi(e) < 0 ? -1 : (atoi(e) != 0 ? 1 : 0);
+        };
+        Knobs &k = h->kn;
+        tri("CLOWNS_FUSE_PM", k.fuse_pm);
+        tri("CLOWNS_STREAM", k.stream);
+        tri("CLOWNS_LL8", k.ll8);
+        flag("CLOWNS_LL16", k.ll16);
+        flag("CLOWNS_V4_32", k.v4_32);
+        flag("CLOWNS_SPMV_ROWS", k.rows32);
+        flag("CLOWNS_PM_HALF", k.pm_half);
+        flag("CLOWNS_BLOCK_ORDER", k.block_order);
+        flag("CLOWNS_VERBOSE", k.verbose);
+        flag("CLOWNS_CLASSIFY_HOST", h->cls_force_host);
+        if (const char *e = getenv("CLOWNS_POLL")) h->poll = std::max(1, atoi(e));
+        if (const char *e = getenv("CLOWNS_TAU_MIN")) h->tau_min = std::max(1e-8, atof(e));
+        if (const char *e = getenv("CLOWNS_WILL_MAX_ROUNDS")) h->will_max_rounds = std::max(1, atoi(e));
+    }
     *out = h;
     return CLOWNS_OK;
 }
@@ -1194,13 +1253,14 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     h->AL.release(); h->D.release(); h->W.release(); h->LL.release(); h->LL16.release(); h->base16.release(); h->LL8.release(); h->base8.release(); h->order.release(); h->order_first.release();
-    h->AL32.release(); h->D32.release(); h->ALh.release(); h->ALr32.release(); h->Lr16.release();
+    h->AL32.release(); h->D32.release(); h->ALh.release(); h->ALr32.release(); h->Lr16.release(); h->Lr8.release(); h->nonunit.release();
     h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
     h->LLt16.release(); h->baset16.release(); h->LLt8.release(); h->baset8.release();
     h->AL0.release(); h->D0.release(); h->LL0.release();
     h->v64.release(); h->v32.release(); h->bs.release(); h->stage_b.release(); h->stage_x.release(); h->part.release(); h->flag.release(); h->will_flags.release(); h->will_cand.release(); h->will_rev.release(); h->cls_scratch.release(); h->pc_b.release(); h->pc_x.release(); h->pc_part.release(); h->sc.release();
     if (h->h_sc) (void)hipHostFree(h->h_sc);
     if (h->ev0) { (void)hipEventDestroy(h->ev0); (void)hipEventDestroy(h->ev1); (void)hipEventDestroy(h->ev2); }
+    for (hipEvent_t e : h->evp) if (e) (void)hipEventDestroy(e);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return CLOWNS_OK;
@@ -1560,7 +1620,6 @@ extern "C" int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32
     }
     Scal *sc = h->sc.p;
     k_reset<<<1, 64, 0, s>>>(sc, 0.0, 1, 0);
-    const Geo g = geo_of(N, h->R);
     if ((rc = ensure_block_order(h))) return rc;
     Op<double> A = op64(h);
     Op<float> A32 = op32(h);
@@ -1568,13 +1627,13 @@ extern "C" int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32
     known_stream(h, A32);
     auto one = [&]() {
         switch (kernel) {
-        case 0: spmv<double>(s, g, sc, A, 0, h->v64.p0.p, nullptr, nullptr, nullptr, h->v64.y.p); break;
-        case 1: launch_pm_stage<double, 1>(s, g, sc, A, 0, h->v64.y.p, h->v64.t.p, 0, nullptr, h->part.p); break;
-        case 2: spmv<float>(s, g, sc, A32, 0, h->v32.p0.p, nullptr, nullptr, nullptr, h->v32.y.p); break;
-        case 3: launch_pm_stage<float, 1>(s, g, sc, A32, 0, h->v32.y.p, h->v32.t.p, 0, nullptr, h->part.p); break;
+        case 0: spmv<double>(s, sc, A, 0, h->v64.p0.p, nullptr, h->v64.y.p); break;
+        case 1: launch_pm_stage<double, 1>(s, sc, A, 0, h->v64.y.p, h->v64.t.p, 0, nullptr, h->part.p); break;
+        case 2: spmv<float>(s, sc, A32, 0, h->v32.p0.p, nullptr, h->v32.y.p); break;
+        case 3: launch_pm_stage<float, 1>(s, sc, A32, 0, h->v32.y.p, h->v32.t.p, 0, nullptr, h->part.p); break;
         case 4: k_copy<double><<<cdiv(N, BS), BS, 0, s>>>(N, h->v64.r.p, h->v64.e.p); break;
-        case 5: launch_pm2<double>(s, g, sc, A, 0, h->v64.y.p, h->v64.t.p); break;
-        default: launch_pm2<float>(s, g, sc, A32, 0, h->v32.y.p, h->v32.t.p); break;
+        case 5: launch_pm2<double>(s, sc, A, 0, h->v64.y.p, h->v64.t.p); break;
+        default: launch_pm2<float>(s, sc, A32, 0, h->v32.y.p, h->v32.t.p); break;
         }
     };
     one();
@@ -1593,13 +1652,13 @@ namespace {
 // build_ll16 / build_ll8 make them for the forward one.
 int build_codes_t(clowns_solver *h) {
     h->t16_ok = h->t8_ok = false;
-    if (!h_ll16 || !h_v2_32 || !h_v2_64 || !h_v2_vec) return CLOWNS_OK;
+    if (!h->kn.ll16) return CLOWNS_OK;
     const int N = h->N;
     const size_t nm = (size_t)h->lda * h->NL;
     int rc;
     if ((rc = h->LLt16.ensure(nm)) || (rc = h->baset16.ensure(cdiv(N, LL16_ROWS))) || (rc = h->flag.ensure(4)))
         return rc;
-    const bool want8 = h_ll8 != 0;
+    const bool want8 = h->kn.ll8 != 0;
     if (want8 && ((rc = h->LLt8.ensure(nm)) || (rc = h->baset8.ensure(cdiv(N, LL8_ROWS))))) return rc;
     hipStream_t s = h->stream;
     fill<int>(s, h->flag.p, 4, 0);
@@ -1745,7 +1804,7 @@ int note_block_order(clowns_solver *h, const clowns_grid_dev *g) {
     h->order_pending = false;
     // below ~1M rows a plane's worth of rows stays in L2 anyway, and building
     // the order (two host syncs) would only add latency to a short solve
-    if (!h_block_order || h->N < ORDER_MIN_ROWS) return CLOWNS_OK;
+    if (!h->kn.block_order || h->N < ORDER_MIN_ROWS) return CLOWNS_OK;
     const int nb = cdiv(h->N, 2 * BS);
     int rc;
     if ((rc = h->order_first.ensure(nb))) return rc;
@@ -2180,8 +2239,9 @@ extern "C" int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid
 extern "C" int clowns_solver_tuning(clowns_solver *h, int32_t out[6]) {
     if (!h || !out || h->N < 1) return CLOWNS_EBADARG;
     const int N = h->N;
-    out[0] = h_fuse_pm >= 0 ? h_fuse_pm : (tuned_size(N, h->fuse64_N) ? h->fuse64 : -1);
-    out[1] = h_fuse_pm >= 0 ? h_fuse_pm : (tuned_size(N, h->fuse32_N) ? h->fuse32 : -1);
+    const int fz = h->kn.fuse_pm;
+    out[0] = fz >= 0 ? fz : (tuned_size(N, h->fuse64_N) ? h->fuse64 : -1);
+    out[1] = fz >= 0 ? fz : (tuned_size(N, h->fuse32_N) ? h->fuse32 : -1);
     int f[2] = {-1, -1};
     if (!stored_forms(h, op64(h), &f[0])) f[0] = -1;
     if (!stored_forms(h, op32(h), &f[1])) f[1] = -1;
@@ -2219,7 +2279,7 @@ extern "C" int clowns_solver_set_block_order(clowns_solver *h, const int32_t *d_
 extern "C" int clowns_solver_index_bits(clowns_solver *h) {
     if (!h || h->N < 1) return -CLOWNS_EBADARG;
     int f;
-    if (h->ll8_ok && (h_ll8 == 1 || (stored_forms(h, op64(h), &f) && (f & 4)))) return 8;
+    if (h->ll8_ok && (h->kn.ll8 == 1 || (stored_forms(h, op64(h), &f) && (f & 4)))) return 8;
     return h->ll16_ok ? 16 : 32;
 }
 

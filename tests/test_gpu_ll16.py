@@ -45,9 +45,9 @@ def _solve_all(s, form):
     xt, it = sol.solve_transpose(b, eps=1e-10)
     bits = sol.index_bits()
     tune = sol.tuning()
-    # FP64 reads the forced width; FP32 keeps 16-bit codes (its row-major
-    # SpMV reads them) unless they are off
-    assert tune["index64"] == bits and tune["index32"] == (32 if bits == 32 else 16)
+    # both precisions read the forced width (the FP32 row-major SpMV has
+    # copies of the 8- and the 16-bit codes)
+    assert tune["index64"] == bits and tune["index32"] == bits
     out = [x.cpu().numpy() for x in (x64, x32, xt)], (i64, i32, it), bits
     sol.close()
     return out

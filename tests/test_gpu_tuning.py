@@ -17,7 +17,7 @@ def test_tuning_kept_for_nearby_sizes(capfd):
     from clowns3d import Solver
     os.environ["CLOWNS_VERBOSE"] = "1"
     try:
-        sol = Solver(0)  # the switch is process-wide, read at creation
+        sol = Solver(0)  # the switch is read at creation
     finally:
         del os.environ["CLOWNS_VERBOSE"]
     dev = torch.device("cuda:0")
