@@ -221,6 +221,27 @@ class Solver:
                 "clowns_solver_get_scaled")
         return AL, LL, D, W, B
 
+    KERNEL_CLASSES = ("spmv", "pm_stage", "pm_pair", "pm_dot", "p_update", "e_update", "xr_update", "finalize",
+                      "residual_spmv")
+
+    def kernel_profile(self, enable):
+        """In-solve kernel durations (clowns_solver_kernel_profile).
+        kernel_profile(True) arms the solver; kernel_profile(False) disarms
+        it and returns {"fp64": {class: (launches, total_ms)}, "fp32": ...}
+        over the solves in between (launches that did work only)."""
+        if enable:
+            L.check(L.lib().clowns_solver_kernel_profile(self._h, 1, None, None), "clowns_solver_kernel_profile")
+            return None
+        n = len(self.KERNEL_CLASSES)
+        ms = np.zeros(2 * n)
+        cnt = np.zeros(2 * n, np.int32)
+        L.check(L.lib().clowns_solver_kernel_profile(self._h, 0, ctypes.c_void_p(ms.ctypes.data),
+                                                     ctypes.c_void_p(cnt.ctypes.data)), "clowns_solver_kernel_profile")
+        out = {}
+        for p, off in (("fp64", 0), ("fp32", n)):
+            out[p] = {c: (int(cnt[off + i]), float(ms[off + i])) for i, c in enumerate(self.KERNEL_CLASSES)}
+        return out
+
     def time_kernel(self, kernel, reps=50):
         ms = ctypes.c_double()
         L.check(L.lib().clowns_solver_time_kernel(self._h, int(kernel), int(reps), ctypes.byref(ms)),

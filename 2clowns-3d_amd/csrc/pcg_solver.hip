@@ -85,6 +85,55 @@ template <typename T> struct VecSet {
 
 } // namespace
 
+// In-solve kernel durations (clowns_solver_kernel_profile): while armed, HIP
+// events on the solver's stream bracket every launch of the solve's kernels,
+// tagged with the kernel class, the Bi-CGSTAB run and its iteration; the
+// launches a run queued past its last iteration (they return at once) are
+// left out of the sums.  Disarmed (the default) it records nothing.
+enum KClass { KC_SPMV, KC_PM, KC_PM2, KC_PMDOT, KC_PUPD, KC_EUPD, KC_UPDATE, KC_FIN, KC_RESID, KC_N };
+struct KProf {
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<hipEvent_t> pool;  // two per record
+    struct Rec {
+        int cls, run, K;
+    };
+    std::vector<Rec> rec;
+    std::vector<int> kout;  // per run: its last working iteration
+    int run = -1;           // run being enqueued (-1: outside run_bicgstab)
+    int K = 0;              // its iteration being enqueued
+    int begin(int cls) {
+        const size_t i = rec.size();
+        while (pool.size() < 2 * (i + 1)) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            pool.push_back(e);
+        }
+        if (hipEventRecord(pool[2 * i], s) != hipSuccess) return -1;
+        rec.push_back({cls, run, K});
+        return (int)i;
+    }
+    void end(int i) {
+        if (i >= 0) (void)hipEventRecord(pool[2 * i + 1], s);
+    }
+    void release() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+        pool.clear();
+        rec.clear();
+        kout.clear();
+    }
+};
+struct KScope {
+    KProf *p;
+    int i = -1;
+    KScope(KProf *prof, int cls) : p(prof) {
+        if (p) i = p->begin(cls);
+    }
+    ~KScope() {
+        if (p) p->end(i);
+    }
+};
+
 // Run-time switches (environment variables), read once when a solver is
 // created and kept in that solver: creating another solver never changes the
 // kernel paths of a live one.
@@ -104,7 +153,8 @@ struct Knobs {
     // 224 -> 196 us and PM stage 56 -> 51 us, but FP32 PM stage 28 -> 35 us;
     // at 2.45M rows (operator inside the 256 MB last-level cache) every
     // kernel slower.  tune_stream times both per operator and precision;
-    // CLOWNS_STREAM = 0/1 forces them off/on (-1: timed).
+    // CLOWNS_STREAM forces them: 0 neither, 1 both, 2 the PM stages only, 3
+    // the SpMV only (-1: timed).  As a mask: bit 0 PM stages, bit 1 SpMV.
     int stream = -1;
     // 8-bit coded neighbour indices (Ix8) in place of the 16-bit codes.  Half
     // the index bytes, but more decoding and a scattered 32-bit read per wide
@@ -149,6 +199,7 @@ struct Knobs {
 
 struct clowns_solver {
     Knobs kn;
+    KProf prof;
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -305,7 +356,11 @@ template <typename T> struct Op {
     const short *Lr16 = nullptr;
     const uint8_t *Lr8 = nullptr;   // ... 8-bit codes (with LL8), or null
     const int *nonunit = nullptr;   // ... device count of rows whose FP32 diagonal is not 1
+    KProf *prof = nullptr;          // armed kernel profile, or null
 };
+
+// Kernel-class tag of a launch for the profile: FP32 classes after FP64's.
+template <typename T> int kcls(int c) { return c + (sizeof(T) == 4 ? KC_N : 0); }
 
 // The index source of column j for the two-row kernels.
 template <typename T> bool has16(const Op<T> &A) { return A.LL16 != nullptr; }
@@ -351,6 +406,7 @@ void launch_pm2_c(hipStream_t s, const Scal *sc, const Op<T> &A, const C *AL, in
 // Stages j and j+1 of LDUSUB fused (k_pm2_v2).
 template <typename T>
 void launch_pm2(hipStream_t s, const Scal *sc, const Op<T> &A, int j, const T *in, T *out) {
+    KScope ks(A.prof, kcls<T>(KC_PM2));
     if constexpr (sizeof(T) == 4) {
         if (A.ALh) {
             launch_pm2_c<T, _Float16>(s, sc, A, A.ALh, j, in, out);
@@ -429,6 +485,7 @@ int launch_pm_stage_c(hipStream_t s, const Scal *sc, const Op<T> &A, const C *AL
 template <typename T, int NT>
 int launch_pm_stage(hipStream_t s, const Scal *sc, const Op<T> &A, int j, const T *in, T *out, int dot,
                     const T *w, double *part) {
+    KScope ks(A.prof, kcls<T>(NT == 2 || dot != 0 ? KC_PMDOT : KC_PM));
     if constexpr (sizeof(T) == 4) {
         if (A.ALh) return launch_pm_stage_c<T, NT, _Float16>(s, sc, A, A.ALh, j, in, out, dot, w, part);
     }
@@ -561,6 +618,7 @@ void launch_spmv_v2(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int mo
 // AXSUB: y = A x (mode 0) or the true residual y = v1 - A x (mode 3).
 template <typename T>
 void spmv(hipStream_t s, const Scal *sc, const Op<T> &A, int mode, const T *x, const T *v1, T *y) {
+    KScope ks(A.prof, kcls<T>(mode == 3 ? KC_RESID : KC_SPMV));
     if (A.NL == 6) {
         if (has8(A))
             launch_spmv_v2<T, 6, Ix8>(s, sc, A, ix8(A, 0), mode, x, v1, y);
@@ -591,7 +649,7 @@ inline int force_key(const clowns_solver *h) { return (h->kn.stream + 1) | ((h->
 // Bits fixed for operator A (mask) and their values.
 template <typename T> void fixed_forms(const clowns_solver *h, const Op<T> &A, int &mask, int &val) {
     mask = val = 0;
-    if (h->kn.stream >= 0) mask |= 3, val |= 3 * h->kn.stream;
+    if (h->kn.stream >= 0) mask |= 3, val |= h->kn.stream;
     if (!can_ix8(A) || h->kn.ll8 >= 0) mask |= 4, val |= (can_ix8(A) && h->kn.ll8 == 1) ? 4 : 0;
 }
 // The stored decisions, if they hold for A: forms in *f.  The index-code
@@ -754,18 +812,37 @@ int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc, int i
     hipStream_t s = h->stream;
     double *part = h->part.p;
     const int Gv = geo_of(A.N, VecOf<T>::R).G;
+    KProf *pf = A.prof;
+    if (pf) pf->run = (int)pf->kout.size();
     auto iteration = [&](int K) {
+        if (pf) pf->K = K;
         // Q = PM*A*P (:149-151), after P = R + BETA*(P - AMU*Q) (:244) for K > 1.
-        if (K > 1) k_pupd_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.p0.p, V.r.p, V.q.p);
+        if (K > 1) {
+            KScope ks(pf, kcls<T>(KC_PUPD));
+            k_pupd_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.p0.p, V.r.p, V.q.p);
+        }
         spmv<T>(s, sc, A, 0, V.p0.p, nullptr, V.y.p);
         const int Gq = apply_pm<T>(s, sc, A, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
-        k_fin_alpha<<<1, FIN_BS, 0, s>>>(sc, part, Gq);
+        {
+            KScope ks(pf, kcls<T>(KC_FIN));
+            k_fin_alpha<<<1, FIN_BS, 0, s>>>(sc, part, Gq);
+        }
         // V = PM*A*E with E = R - ALP*Q (:173-181).
-        k_eupd_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.r.p, V.q.p, V.e.p);
+        {
+            KScope ks(pf, kcls<T>(KC_EUPD));
+            k_eupd_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.r.p, V.q.p, V.e.p);
+        }
         spmv<T>(s, sc, A, 0, V.e.p, nullptr, V.y.p);
         const int Gw = apply_pm<T>(s, sc, A, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
-        k_fin_omega<<<1, FIN_BS, 0, s>>>(sc, part, Gw);
-        k_update_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
+        {
+            KScope ks(pf, kcls<T>(KC_FIN));
+            k_fin_omega<<<1, FIN_BS, 0, s>>>(sc, part, Gw);
+        }
+        {
+            KScope ks(pf, kcls<T>(KC_UPDATE));
+            k_update_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
+        }
+        KScope ks(pf, kcls<T>(KC_FIN));
         k_fin_beta<<<1, FIN_BS, 0, s>>>(sc, part, Gv, K);
     };
     Scal *slot[2] = {h->h_sc + 2, h->h_sc + 3};
@@ -803,6 +880,11 @@ int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc, int i
         }
         plan.seen(hs.K, hs.ERR);
         cur ^= 1;
+    }
+    if (pf) {
+        pf->kout.push_back(h->h_sc[sc_index].kout);
+        pf->run = -1;
+        pf->K = 0;
     }
     return CLOWNS_OK;
 }
@@ -1038,6 +1120,7 @@ int solve_fp64(clowns_solver *h, const Op<double> &A_in, const double *Wsc, cons
     Op<double> A = A_in;
     A.fuse = tune_fuse(h, A, V);
     tune_stream(h, A, V);
+    if (h->prof.on) A.prof = &h->prof;
     const int N = A.N;
     const int G1 = geo_of(N, 1).G;
     Scal *sc = h->sc.p;
@@ -1097,6 +1180,7 @@ int solve_mixed(clowns_solver *h, const Op<double> &A_in, const Op<float> &A32_i
     A32.fuse = tune_fuse(h, A32, F);
     tune_stream(h, A, V);
     tune_stream(h, A32, F);
+    if (h->prof.on) A.prof = A32.prof = &h->prof;
     const int N = A.N;
     const int G1 = geo_of(N, 1).G;
     Scal *sc = h->sc.p, *outer = h->sc.p + 1;
@@ -1231,7 +1315,10 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
         };
         Knobs &k = h->kn;
         tri("CLOWNS_FUSE_PM", k.fuse_pm);
-        tri("CLOWNS_STREAM", k.stream);
+        if (const char *e = getenv("CLOWNS_STREAM")) {
+            const int v = atoi(e);
+            k.stream = v < 0 ? -1 : (v == 0 ? 0 : (v == 1 ? 3 : (v == 2 ? 1 : 2)));
+        }
         tri("CLOWNS_LL8", k.ll8);
         flag("CLOWNS_LL16", k.ll16);
         flag("CLOWNS_V4_32", k.v4_32);
@@ -1261,6 +1348,7 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     if (h->h_sc) (void)hipHostFree(h->h_sc);
     if (h->ev0) { (void)hipEventDestroy(h->ev0); (void)hipEventDestroy(h->ev1); (void)hipEventDestroy(h->ev2); }
     for (hipEvent_t e : h->evp) if (e) (void)hipEventDestroy(e);
+    h->prof.release();
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return CLOWNS_OK;
@@ -1604,6 +1692,40 @@ extern "C" int clowns_will_filter(clowns_solver *h, const int32_t *d_LL, int32_t
     if ((rc = will_filter(h, d_LL, NL, lda, N, d_X, d_R, d_Xout, d_B, rounds, false))) return rc;
     h->will_rev_valid = false;  // will_rev now holds the caller's LL
     HIPCHK(hipStreamSynchronize(s));
+    return CLOWNS_OK;
+}
+
+extern "C" int clowns_solver_kernel_profile(clowns_solver *h, int32_t enable, double *ms, int32_t *count) {
+    if (!h) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    KProf &p = h->prof;
+    if (enable) {
+        p.rec.clear();
+        p.kout.clear();
+        p.run = -1;
+        p.K = 0;
+        p.s = h->stream;
+        p.on = true;
+        return CLOWNS_OK;
+    }
+    if (!p.on) return CLOWNS_EBADARG;
+    p.on = false;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < 2 * KC_N; ++c) {
+        if (ms) ms[c] = 0.0;
+        if (count) count[c] = 0;
+    }
+    for (size_t i = 0; i < p.rec.size(); ++i) {
+        const KProf::Rec &r = p.rec[i];
+        if (r.run >= 0 && r.run < (int)p.kout.size() && r.K > p.kout[r.run]) continue;  // returned at once
+        float e = 0.f;
+        HIPCHK(hipEventElapsedTime(&e, p.pool[2 * i], p.pool[2 * i + 1]));
+        if (ms) ms[r.cls] += e;
+        if (count) count[r.cls] += 1;
+    }
+    p.rec.clear();
+    p.kout.clear();
     return CLOWNS_OK;
 }
 

@@ -1,16 +1,33 @@
 """Benchmark: one 2CLOWNS-3D pressure solve per step on one MI355X.
 
-A step is one pass of the hot path over the configuration-2 workload
-(256x256x64 cells, wavy free surface at 0.6 H, 64x64x24 solid obstacle,
-seeded U(-0.1,0.1) face velocities): device assembly of the pressure system
-fused with PCGPME's row scaling and sort (init_matrix + cal_matrix_element +
-RHS + pcgpme_para.F90:77-112), then the PM-preconditioned Bi-CGSTAB solve to
-the reference's stopping rule sqrt(RN/BN) <= 1e-10, FP64 end to end.  The
-geometry (classified once on the host) is resident in HBM before timing; no
-state carries from one step to the next (x0 = 0, operator rebuilt).
+A step is one pass of the hot path over the configuration-3 workload (by
+default; --config 1|2 picks the smaller ones): 512x512x96 cells, 40 seeded
+Gaussian bed bumps with partial cells (fb in [0.05, 1]), wavy free surface at
+0.6 H, seeded U(-0.1,0.1) face velocities -- 13.07M matrix rows.  The step is
+the device assembly of the pressure system fused with PCGPME's row scaling and
+sort (init_matrix + cal_matrix_element + RHS + pcgpme_para.F90:77-112), then
+the PM-preconditioned Bi-CGSTAB solve to the reference's stopping rule
+sqrt(RN/BN) <= 1e-10, FP64 end to end (`value`).  The geometry (classified
+once on the host) is resident in HBM before timing; no state carries from one
+step to the next (x0 = 0, operator rebuilt).
 
-The same steps are then run in mixed precision (FP32 inner solve, FP64
-refinement), reported under "mixed" with the error against the FP64 solution.
+The same steps then run in mixed precision (FP32 inner solve, FP64
+refinement), reported under "mixed" with its error against a tight FP64
+solve and against the FP64 solution, and its own roofline entry.
+
+roofline: one more step of each form with HIP events around every kernel
+launch on the solver's stream (clowns_solver_kernel_profile): the kernel
+class with the most in-solve time, its average launch duration and its
+algorithmic bytes per launch (CLASS_BYTES x rows); `traffic` from
+profiles/traffic_config<n>.json, written by --traffic from two rocprofv3
+--pmc runs (FETCH_SIZE, WRITE_SIZE) of --pmc-run.
+
+cpu_baseline: the reference's own PCGPME (flang build of pcgpme_para.F90,
+oracle/_ref) on the same system assembled on the host, run to convergence at
+the host's cores (OMP_PROC_BIND=close), median of --cpu-runs after a warm-up,
+with the reference's ITER and the relative L2 distance of each GPU solution
+from the reference's X.  boundary: the drop-in clowns_pcgpme with the
+caller's host arrays (PCIe in and out) beside the resident solve.
 
 Prints one JSON line.  `value` is seconds per step (lower is better).
 
@@ -34,6 +51,7 @@ import ctypes
 import json
 import math
 import os
+import subprocess
 import sys
 import threading
 import time
@@ -53,7 +71,7 @@ ALG_BYTES = {
     "iteration": 512,
     "setup_cold": 288,
     "iteration32": 256,   # FP32 inner iteration of the mixed solve
-    "refine64": 216,      # FP64 true residual: SpMV with b - Ax (80) + one PM application (136)
+    "refine64": 240,      # per FP64 refinement of the mixed solve (true residual, x update, downcast)
 }
 
 
@@ -62,14 +80,22 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3])
+    ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3])
     ap.add_argument("--eps", type=float, default=1e-10)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--no-boundary", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--pmc-run", default=None, metavar="DIR",
+                    help="one FP64 and one mixed solve plus the copy kernel, for rocprofv3 --pmc (row count to DIR)")
+    ap.add_argument("--traffic", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/traffic_config<n>.json from two rocprofv3 --pmc runs of --pmc-run")
     ap.add_argument("--profile-kernels", action="store_true",
-                    help="only assemble and run the timed kernel loops (for rocprofv3 --pmc)")
+                    help="--will/--vof: only run the timed kernel loops (for rocprofv3 --pmc)")
+    ap.add_argument("--reference-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--trace-summary", default=None, metavar="CSV",
+                    help="print a markdown summary of a rocprofv3 --kernel-trace CSV (working launches)")
     ap.add_argument("--will", default=None, metavar="NXxNY",
                     help="measure the 2DH dispersion solve PCGPME_WILL on an NX x NY grid instead")
     ap.add_argument("--vof", type=int, default=None, choices=[2, 3],
@@ -341,6 +367,11 @@ def bench_vof(args):
     print(json.dumps(res))
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
+
+
 class _quiet_stdout:
     """Route file descriptor 1 to /dev/null around a call into the flang
     build of the reference, which writes progress lines to unit 6 (flushed
@@ -365,86 +396,382 @@ class _quiet_stdout:
         return False
 
 
-def cpu_baseline(g, iters_full, n_iter_hi, threads):
-    """The reference's own PCGPME (flang build in oracle/_ref) -- or, if that
-    was not built, the C oracle -- on the same configuration's system built
-    on the host, run for 5 and n_iter_hi iterations; solve time =
-    setup + iters_full * per-iteration time."""
+def host_info():
+    """The host the CPU baseline runs on: logical CPUs, the CPUs this process
+    may run on, and lscpu's model, sockets and cores."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def host_system(g):
+    """The configuration's pressure system assembled on the host by the C
+    oracle (the checker's restatement of set_nflag, the row numbering,
+    init_matrix, cal_matrix_element and the RHS), in the reference caller's
+    layout: AL(NA,NL), LL(NA,NL) column-major with NA = N + 60
+    (1sub-momentum.F90:254-259), D, B length N."""
     import numpy as np
     orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
     P = lambda a: ctypes.c_void_p(a.ctypes.data)
     A = g.arrays()
     nc = g.ncell
-    nf = np.zeros(nc, np.int8)
-    nb = np.zeros(nc, np.int8)
-    r2c = np.zeros(nc, np.int32)
-    mno = np.zeros(nc, np.int32)
-    lda = nc
-    AL = np.zeros((6, lda))
-    LL = np.zeros((6, lda), np.int32)
-    D = np.zeros(lda)
-    B = np.zeros(lda)
+    nf, nb = np.zeros(nc, np.int8), np.zeros(nc, np.int8)
+    r2c, mno = np.zeros(nc, np.int32), np.zeros(nc, np.int32)
+    AL = np.zeros((6, nc))
+    LL = np.zeros((6, nc), np.int32)
+    D, B = np.zeros(nc), np.zeros(nc)
     orc.oracle_build_system.restype = ctypes.c_int
     N = orc.oracle_build_system(g.nx, g.ny, g.nz, P(A["solid"]), P(A["F"]), P(A["fb"]), P(A["a"]), P(A["un"]),
                                 P(A["dx"]), P(A["xf"]), g.stride, ctypes.c_double(g.dt), P(nf), P(nb), P(r2c),
-                                P(mno), lda, P(AL), P(LL), P(D), P(B))
+                                P(mno), nc, P(AL), P(LL), P(D), P(B))
     NA = N + 60
-    ALf = np.asfortranarray(AL[:, :NA].T)
-    LLf = np.asfortranarray(LL[:, :NA].T)
+    return {"N": N, "NA": NA, "AL": np.asfortranarray(AL[:, :NA].T), "LL": np.asfortranarray(LL[:, :NA].T),
+            "D": D[:N].copy(), "B": B[:N].copy()}
+
+
+def reference_solve(sysm, threads, eps, in_thread=True):
+    """One call of the reference's own PCGPME (flang build of
+    pcgpme_para.F90 in oracle/_ref; the C oracle if that was not built) on
+    copies of the system, at `threads` OpenMP threads, X = 0 as the caller
+    passes it (1sub-momentum.F90:559-564).  The reference's work arrays are
+    automatic (stack) arrays: in_thread runs it in a thread with a stack of
+    their size, else in the calling thread.  Returns (seconds, ITER, IER, X,
+    kind)."""
+    import numpy as np
+    N, NA = sysm["N"], sysm["NA"]
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libref_pcgpme.so")
     kind = "reference" if os.path.exists(ref_path) else "port"
+    a, l = sysm["AL"].copy(order="F"), sysm["LL"].copy(order="F")
+    d, b = sysm["D"].copy(), sysm["B"].copy()
+    x = np.zeros(N + 64)
+    epsv = np.array([eps, 0.0])
+    it, ier = ctypes.c_int(10000), ctypes.c_int(0)
+    lib = None
+    if kind == "reference":
+        lib = ctypes.CDLL(ref_path)
+        i32 = lambda v: ctypes.byref(ctypes.c_int(v))
 
-    def run(iters):
-        a, l, d, b = ALf.copy(order="F"), LLf.copy(order="F"), D[:N].copy(), B[:N].copy()
-        x = np.zeros(N + 64)
-        eps = np.array([1e-10, 0.0])
-        it = ctypes.c_int(iters)
-        ier = ctypes.c_int(0)
-        lib = None
-        if kind == "reference":
-            lib = ctypes.CDLL(ref_path)
-            i32 = lambda v: ctypes.byref(ctypes.c_int(v))
+        def call():
+            # the OpenMP thread count is a per-thread setting: set it in the
+            # thread that calls the reference
+            lib.ref_set_threads(threads)
+            lib.pcgpme_(P(a), i32(6), i32(NA), P(l), P(d), i32(N), i32(64), P(b), P(x), P(epsv),
+                        ctypes.byref(ctypes.c_double(5.0)), ctypes.byref(it), ctypes.byref(ier))
+    else:
+        orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
 
-            def call():
-                # the OpenMP thread count is a per-thread setting: set it in
-                # the thread that calls the reference
-                lib.ref_set_threads(threads)
-                lib.pcgpme_(P(a), i32(6), i32(NA), P(l), P(d), i32(N), i32(64), P(b), P(x), P(eps),
-                            ctypes.byref(ctypes.c_double(5.0)), ctypes.byref(it), ctypes.byref(ier))
-        else:
+        def call():
             orc.oracle_set_threads(threads)
-            call = lambda: orc.oracle_pcgpme(P(a), 6, NA, P(l), P(d), N, 64, P(b), P(x), P(eps),
-                                             ctypes.c_double(5.0), ctypes.byref(it), ctypes.byref(ier))
-        out = {}
+            orc.oracle_pcgpme(P(a), 6, NA, P(l), P(d), N, 64, P(b), P(x), P(epsv), ctypes.c_double(5.0),
+                              ctypes.byref(it), ctypes.byref(ier))
+    out = {}
 
-        def body():
-            t0 = time.perf_counter()
-            call()
-            out["t"] = time.perf_counter() - t0
+    def body():
+        t0 = time.perf_counter()
+        call()
+        out["t"] = time.perf_counter() - t0
 
-        old = threading.stack_size()
-        # the reference's work arrays are automatic (stack) arrays
-        threading.stack_size(max(32 << 20, 128 * (N + 64) + 32 * NA + (64 << 20)))
-        th = threading.Thread(target=body)
-        with _quiet_stdout(lib):
+    with _quiet_stdout(lib):
+        if in_thread:
+            old = threading.stack_size()
+            threading.stack_size(max(32 << 20, 128 * (N + 64) + 32 * NA + (64 << 20)))
+            th = threading.Thread(target=body)
             th.start()
             th.join()
-        threading.stack_size(old)
-        return out["t"], it.value
+            threading.stack_size(old)
+        else:
+            body()
+    return out["t"], it.value, ier.value, x[:N].copy(), kind
 
-    lo = 5
-    t_lo, _ = run(lo)
-    t_hi, _ = run(n_iter_hi)
-    t_it = (t_hi - t_lo) / (n_iter_hi - lo)
-    t_setup = max(0.0, t_lo - lo * t_it)
-    value = t_setup + iters_full * t_it
-    return {"value": value, "unit": "s/step", "cores": threads, "kind": kind,
-            "sample": (f"configuration {g.name} system (N={N}) built on the host; "
+
+def reference_worker(args):
+    """Child process of the CPU baseline: builds the configuration's system
+    on the host and times the reference's PCGPME in its main thread, so that
+    the OpenMP runtime starts in this thread and binds its threads to
+    consecutive cores (OMP_PROC_BIND=close).  Writes X and the timings next
+    to args.reference_worker."""
+    import importlib.util
+    import numpy as np
+    # the configuration module alone: the package would load torch, whose
+    # OpenMP runtime binds this thread to one core under OMP_PROC_BIND
+    spec = importlib.util.spec_from_file_location(
+        "clowns_configs", os.path.join(ROOT, "2clowns-3d_amd", "clowns3d", "configs.py"))
+    configs = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(configs)
+    g = configs.CONFIGS[args.config]()
+    # The oracle's assembly uses the GNU OpenMP runtime, which binds the
+    # thread it starts in to one core under OMP_PROC_BIND; a thread of its
+    # own keeps the main thread, where the reference's runtime starts, free.
+    out = {}
+    th = threading.Thread(target=lambda: out.setdefault("sys", host_system(g)))
+    th.start()
+    th.join()
+    sysm = out["sys"]
+    t, it, _, _, _ = reference_solve(sysm, args.cpu_threads, args.eps, in_thread=False)
+    progress(f"reference warm-up at {args.cpu_threads} threads: {t:.2f} s, ITER {it}")
+    times, iters, ier, x, kind = [], [], 0, None, ""
+    for r in range(args.cpu_runs):
+        t, it, ier, x, kind = reference_solve(sysm, args.cpu_threads, args.eps, in_thread=False)
+        progress(f"reference run {r + 1}/{args.cpu_runs}: {t:.2f} s, ITER {it}, IER {ier}")
+        times.append(t)
+        iters.append(it)
+    np.save(args.reference_worker + ".npy", x)
+    with open(args.reference_worker + ".json", "w") as fh:
+        json.dump({"times": times, "iters": iters, "ier": ier, "kind": kind, "N": sysm["N"]}, fh)
+
+
+def cpu_baseline(config, threads, runs, eps, gpu_x):
+    """The reference solve on the host cores, in a child process started
+    with OMP_NUM_THREADS = threads, OMP_PROC_BIND = close, OMP_PLACES = cores
+    and a stack limit the reference's automatic arrays fit in: one warm-up
+    call, then the median of `runs` timed calls, each to convergence.  Also
+    the reference's ITER and how far each GPU solution lies from the
+    reference's X."""
+    import resource
+    import tempfile
+    import numpy as np
+    fd, base = tempfile.mkstemp(prefix="clowns_ref_")
+    os.close(fd)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+
+    def big_stack():
+        soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+        want = resource.RLIM_INFINITY if hard == resource.RLIM_INFINITY else hard
+        resource.setrlimit(resource.RLIMIT_STACK, (want, hard))
+
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--config", str(config), "--eps", repr(eps),
+           "--cpu-threads", str(threads), "--cpu-runs", str(runs), "--reference-worker", base]
+    r = subprocess.run(cmd, env=env, preexec_fn=big_stack, stdout=subprocess.DEVNULL)
+    try:
+        if r.returncode != 0:
+            raise RuntimeError(f"CPU reference worker exited with status {r.returncode}")
+        with open(base + ".json") as fh:
+            w = json.load(fh)
+        x_ref = np.load(base + ".npy")
+    finally:
+        for f in (base, base + ".json", base + ".npy"):
+            if os.path.exists(f):
+                os.remove(f)
+    hi = host_info()
+    rel = {k: float(np.linalg.norm(v - x_ref) / np.linalg.norm(x_ref)) for k, v in gpu_x.items()}
+    kind = w["kind"]
+    return {"value": float(np.median(w["times"])), "unit": "s/step", "cores": threads, "kind": kind,
+            "sample": (f"the configuration's system (N={w['N']}) assembled on the host, "
                        f"{'flang build of pcgpme_para.F90' if kind == 'reference' else 'C oracle of PCGPME'} "
-                       f"timed for {lo} and {n_iter_hi} iterations at {threads} OpenMP threads "
-                       f"({t_it * 1e3:.1f} ms/iteration, setup {t_setup * 1e3:.0f} ms); "
-                       f"value = setup + {iters_full} iterations (the GPU FP64 count)"),
-            "ms_per_iteration": t_it * 1e3}
+                       f"run to convergence (its own stopping rule, eps {eps:g}) at {threads} OpenMP threads "
+                       f"(OMP_PROC_BIND=close, OMP_PLACES=cores) in a child process; median of {runs} after "
+                       f"one warm-up"),
+            "runs_s": w["times"], "ref_iters": w["iters"][-1], "ref_ier": w["ier"], "host": hi,
+            "rel_l2_gpu_vs_ref": rel}
+
+
+# Algorithmic bytes per row of each kernel class (values and vectors the
+# kernel must stream once; index arrays excluded), FP64.  The FP32 classes of
+# the mixed inner solve move half.  Whole iteration: 512 B/row (FP64), 256
+# (FP32); per FP64 true residual of the mixed solve 216.
+CLASS_BYTES = {"spmv": 72, "pm_stage": 24, "pm_pair": 32, "pm_dot": 40, "p_update": 32, "e_update": 24,
+               "xr_update": 56, "residual_spmv": 80}
+
+
+def kernel_names(tune, precision):
+    """Names of the kernel forms the solver runs on this operator, per class."""
+    if precision == "fp64":
+        st = tune["stream64"]
+        pm, sp = ("true" if st > 0 and st & 1 else "false"), ("true" if st > 0 and st & 2 else "false")
+        ix = "Ix%d" % tune["index64"]
+        return {"spmv": f"k_spmv_v2<double,6,0,{ix},{sp}>", "pm_stage": f"k_pm_v2<double,1,0,{ix},{pm},double>",
+                "pm_pair": f"k_pm2_v2<double,{ix},{pm},double>", "pm_dot": f"k_pm_v2<double,2,DOT,{ix},{pm},double>",
+                "p_update": "k_pupd_vec<double>", "e_update": "k_eupd_vec<double>",
+                "xr_update": "k_update_vec<double>", "residual_spmv": f"k_spmv_v2<double,6,3,{ix},{sp}>"}
+    st = tune["stream32"]
+    pm, sp = ("true" if st > 0 and st & 1 else "false"), ("true" if st > 0 and st & 2 else "false")
+    ix = "Ix%d" % tune["index32"]
+    codes = "unsigned char" if ix == "Ix8" else "short"
+    c = "_Float16" if os.environ.get("CLOWNS_PM_HALF", "1") != "0" else "float"
+    return {"spmv": f"k_spmv_rows32<{ix},{codes},{sp}>", "pm_stage": f"k_pm_v4<1,0,{ix},{pm},{c}>",
+            "pm_pair": f"k_pm2_v2<float,{ix},{pm},{c}>", "pm_dot": f"k_pm_v4<2,DOT,{ix},{pm},{c}>",
+            "p_update": "k_pupd_vec<float>", "e_update": "k_eupd_vec<float>", "xr_update": "k_update_vec<float>"}
+
+
+def dominant(prof, precision, N, tune, traffic):
+    """The kernel class with the most in-solve time: its name, launches,
+    average duration and algorithmic rate."""
+    scale = 1.0 if precision == "fp64" else 0.5
+    classes = {c: v for c, v in prof[precision].items() if v[0] > 0 and c in CLASS_BYTES and c != "residual_spmv"}
+    cls = max(classes, key=lambda c: classes[c][1])
+    n, ms = classes[cls]
+    avg_us = ms * 1e3 / n
+    alg = CLASS_BYTES[cls] * scale * N
+    name = kernel_names(tune, precision)[cls]
+    achieved = alg / (avg_us * 1e-6) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+           "frac": achieved / PEAK_HBM_GBS, "traffic": (traffic or {}).get(name), "kernel": name,
+           "kernel_class": cls, "launches": n, "avg_us": avg_us, "alg_bytes_per_launch": alg,
+           "alg_bytes_per_row": CLASS_BYTES[cls] * scale,
+           "in_solve_us": {c: round(v[1] * 1e3 / v[0], 2) for c, v in prof[precision].items() if v[0] > 0},
+           "launches_by_class": {c: v[0] for c, v in prof[precision].items() if v[0] > 0}}
+    return out
+
+
+def traffic_from_pmc(fetch_dir, write_dir, rows, out_path):
+    """HBM traffic per launch from two rocprofv3 --pmc runs of
+    `bench.py --pmc-run` (FETCH_SIZE in one, WRITE_SIZE in the other; KiB per
+    dispatch), each scaled by what the counter reads for the plain copy
+    kernel of 8 N bytes read and 8 N written that the run launches.  A
+    kernel's figure is the median over its dispatches that moved data (those
+    queued past convergence read only the run's scalars)."""
+    import csv
+    import glob
+    import statistics
+
+    def read(d, counter):
+        vals = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name = _short_name(row["Kernel_Name"])
+                    vals.setdefault(name, []).append(float(row["Counter_Value"]))
+        return vals
+
+    fv, wv = read(fetch_dir, "FETCH_SIZE"), read(write_dir, "WRITE_SIZE")
+    copy = "k_copy<double>"
+    ff = statistics.median(fv[copy]) * 1024 / (8 * rows)
+    wf = statistics.median(wv[copy]) * 1024 / (8 * rows)
+    out = {"rows": rows, "fetch_factor": ff, "write_factor": wf}
+    for name in sorted(set(fv) & set(wv)):
+        if not name.startswith("k_"):
+            continue
+        f = [v for v in fv[name] if v > 0.01 * max(fv[name])]
+        w = [v for v in wv[name] if v > 0.01 * max(wv[name])] or [0.0]
+        out[name] = (statistics.median(f) * 1024 / ff + statistics.median(w) * 1024 / wf)
+    with open(out_path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    return out
+
+
+def _demangle_args(m):
+    """Template arguments of a mangled clowns:: kernel name that the
+    profiler left mangled (its demangler stops at _Float16, 'DF16_'):
+    '_ZN6clowns7k_pm_v4ILi1ELi0ENS_3Ix8ELb0EDF16_EE...' -> 'k_pm_v4<1,0,Ix8,false,_Float16>'."""
+    import re
+    head = re.match(r"_ZN6clowns(\d+)", m)
+    if not head:
+        return m
+    n = int(head.group(1))
+    pos = head.end()
+    name = m[pos:pos + n]
+    pos += n
+    if pos >= len(m) or m[pos] != "I":
+        return name
+    pos += 1
+    args = []
+    while pos < len(m) and m[pos] != "E":
+        t = m[pos:]
+        if t.startswith("Li"):
+            e = t.index("E")
+            args.append(t[2:e])
+            pos += e + 1
+        elif t.startswith("Lb"):
+            args.append("true" if t[2] == "1" else "false")
+            pos += 4
+        elif t.startswith("NS_"):
+            mm = re.match(r"NS_(\d+)", t)
+            k = int(mm.group(1))
+            args.append(t[mm.end():mm.end() + k])
+            pos += mm.end() + k + 1
+        elif t.startswith("DF16_"):
+            args.append("_Float16")
+            pos += 5
+        elif t[0] in "fdish":
+            args.append({"f": "float", "d": "double", "i": "int", "s": "short", "h": "unsigned char"}[t[0]])
+            pos += 1
+        else:
+            return m
+    return name + "<" + ",".join(args) + ">"
+
+
+def _short_name(k):
+    """rocprof's demangled kernel name in the form kernel_names uses:
+    'void clowns::k_pm_v2<double, 1, 0, clowns::Ix8, true, double>(...)' ->
+    'k_pm_v2<double,1,0,Ix8,true,double>' (the dot-carrying stages' DOT
+    argument becomes the word DOT)."""
+    if k.startswith("_ZN6clowns"):
+        k = _demangle_args(k)
+    k = k.split("(")[0].replace("void ", "").replace("clowns::", "").replace(" ", "").strip()
+    for pre in ("k_pm_v2<double,2,", "k_pm_v4<2,", "k_pm_v4<1,"):
+        if k.startswith(pre):
+            rest = k[len(pre):]
+            dot, _, tail = rest.partition(",")
+            if pre.startswith("k_pm_v4<1,") and dot == "0":
+                return k
+            return pre + "DOT," + tail
+    if k.startswith("k_pm_v2<double,1,") and not k.startswith("k_pm_v2<double,1,0,"):
+        rest = k[len("k_pm_v2<double,1,"):]
+        return "k_pm_v2<double,1,DOT," + rest.partition(",")[2]
+    return k.replace("unsignedchar", "unsigned char")
+
+
+def trace_summary(csv_path, top=24):
+    """Markdown table of a rocprofv3 --kernel-trace CSV: per kernel, all
+    dispatches and the working ones -- those longer than a quarter of the
+    kernel's 90th-percentile duration (launches queued past a solve's
+    convergence return at once) -- with the working launches' average."""
+    import csv
+    durs = {}
+    for row in csv.DictReader(open(csv_path)):
+        name = _short_name(row["Kernel_Name"])
+        durs.setdefault(name, []).append((int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3)
+    out = []
+    for name, d in durs.items():
+        d_sorted = sorted(d)
+        p90 = d_sorted[int(0.9 * (len(d_sorted) - 1))]
+        work = [v for v in d if v > 0.25 * p90]
+        out.append((sum(work), name, len(d), len(work), sum(work) / max(1, len(work))))
+    out.sort(reverse=True)
+    total = sum(o[0] for o in out)
+    lines = ["| kernel | launches | working launches | avg us (working) | total ms (working) | % |",
+             "|---|---|---|---|---|---|"]
+    for tot, name, n, nw, avg in out[:top]:
+        lines.append(f"| `{name}` | {n} | {nw} | {avg:.2f} | {tot / 1e3:.2f} | {100 * tot / total:.1f} |")
+    return "\n".join(lines)
+
+
+def boundary_mode(sysm, eps, reps=2):
+    """The drop-in clowns_pcgpme with the caller's host arrays (pageable
+    numpy memory, as a Fortran caller holds them): matrix, right-hand side
+    and solution cross PCIe on every call.  The last of `reps` calls is
+    reported (the first carries the library solver's one-time form tuning)."""
+    import numpy as np
+    import clowns3d
+    N, NA = sysm["N"], sysm["NA"]
+    t = None
+    for _ in range(reps):
+        a, l = sysm["AL"].copy(order="F"), sysm["LL"].copy(order="F")
+        d, b = sysm["D"].copy(), sysm["B"].copy()
+        x = np.zeros(N + 64)
+        epsv = np.array([eps, 0.0])
+        t0 = time.perf_counter()
+        it, ier = clowns3d.pcgpme(a, 6, NA, l, d, N, 64, b, x, epsv, 5.0, 10000)
+        t = time.perf_counter() - t0
+    mb = (a.nbytes + l.nbytes + d.nbytes + b.nbytes + 8 * (N + 64)) / 1e6
+    return {"s_per_solve": t, "iters": it, "ier": ier, "host_MB_each_way": mb}
 
 
 def main():
@@ -453,6 +780,20 @@ def main():
         print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               "(a multi-GPU run would shard the rows into z-slabs, one per GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.reference_worker:
+        reference_worker(args)
+        return
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary))
+        return
+    if args.traffic:
+        fetch_dir, write_dir = args.traffic
+        with open(os.path.join(fetch_dir, "rows.json")) as fh:
+            rows = json.load(fh)["rows"]
+        out = traffic_from_pmc(fetch_dir, write_dir, rows,
+                               os.path.join(ROOT, "profiles", f"traffic_config{args.config}.json"))
+        print(json.dumps(out))
+        return
     if args.will:
         bench_will(args)
         return
@@ -461,7 +802,6 @@ def main():
         return
     import numpy as np
     import torch
-    import clowns3d
     from clowns3d import DeviceGrid, Solver, configs
 
     dev = torch.device("cuda:0")
@@ -469,36 +809,46 @@ def main():
     g = configs.CONFIGS[args.config]()
     dg = DeviceGrid(g)
     N = dg.N
+    progress(f"configuration {args.config}: {N} rows classified and resident")
     sol = Solver(0)
     b = torch.empty(N, dtype=torch.float64, device=dev)
     x = torch.zeros(N, dtype=torch.float64, device=dev)
-
-    if args.profile_kernels:
-        # the kernels in the forms the solver picks for this operator
-        dg.assemble_into(sol, b=b)
-        sol.solve(b, x, eps=args.eps)
-        sol.solve(b, x, eps=args.eps, mixed=True)
-        tune = sol.tuning()
-        ix, ix32 = "Ix%d" % tune["index64"], "Ix%d" % tune["index32"]
-        # the FP32 inner preconditioner's coefficient type (CLOWNS_PM_HALF)
-        c32 = "float" if os.environ.get("CLOWNS_PM_HALF", "1") == "0" else "_Float16"
-        v4 = os.environ.get("CLOWNS_V4_32", "1") != "0"  # FP32 single stages: four rows a thread
-        rows = ix32 == "Ix16" and os.environ.get("CLOWNS_SPMV_ROWS", "1") != "0"  # FP32 SpMV, row-major
-        st = {p: ("true" if tune["stream" + p] > 0 and tune["stream" + p] & 1 else "false",
-                  "true" if tune["stream" + p] > 0 and tune["stream" + p] & 2 else "false") for p in ("64", "32")}
-        res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
-               ((f"k_spmv_v2<double,6,0,{ix},{st['64'][1]}>", 0), (f"k_pm_v2<double,1,0,{ix},{st['64'][0]},double>", 1),
-                ("k_copy<double>", 4), (f"k_pm2_v2<double,{ix},{st['64'][0]},double>", 5),
-                (f"k_spmv_rows32<{st['32'][1]}>" if rows else f"k_spmv_v2<float,6,0,{ix32},{st['32'][1]}>", 2), (f"k_pm_v4<{ix32},{st['32'][0]},{c32}>" if v4 else f"k_pm_v2<float,1,0,{ix32},{st['32'][0]},{c32}>", 3),
-                (f"k_pm2_v2<float,{ix32},{st['32'][0]},{c32}>", 6))}
-        print(json.dumps({"rows": N, "avg_us": res, "tuning": tune}))
-        return
 
     def step(mixed):
         dg.assemble_into(sol, b=b)
         x.zero_()
         _, info = sol.solve(b, x, eps=args.eps, maxiter=10000, mixed=mixed)
         return info
+
+    if args.pmc_run:
+        # an assembled FP64 solve and a mixed solve in each of the kernel
+        # forms the solver may pick here (streamed or cached PM stages and
+        # SpMV, 8- or 16-bit index codes), so the traffic of whichever form a
+        # timed run picks was counted inside a solve; then the copy kernel
+        # the counters are scaled by
+        tunes = []
+        for st in ("0", "1", "2", "3"):
+            for ll8 in ("0", "1"):
+                os.environ["CLOWNS_STREAM"], os.environ["CLOWNS_LL8"] = st, ll8
+                try:
+                    s2 = Solver(0)
+                finally:
+                    del os.environ["CLOWNS_STREAM"], os.environ["CLOWNS_LL8"]
+                for mixed in (False, True):
+                    dg.assemble_into(s2, b=b)
+                    x.zero_()
+                    s2.solve(b, x, eps=args.eps, maxiter=10000, mixed=mixed)
+                tunes.append(s2.tuning())
+                s2.close()
+                progress(f"counted forms {tunes[-1]}")
+        dg.assemble_into(sol, b=b)
+        sol.time_kernel(4, 3)
+        torch.cuda.synchronize()
+        os.makedirs(args.pmc_run, exist_ok=True)
+        with open(os.path.join(args.pmc_run, "rows.json"), "w") as fh:
+            json.dump({"rows": N, "forms": tunes}, fh)
+        print(json.dumps({"rows": N, "forms": tunes}))
+        return
 
     def run(mixed):
         for _ in range(args.warmup):
@@ -511,6 +861,7 @@ def main():
         return (t1 - t0) / args.steps, infos
 
     t64, infos64 = run(False)
+    progress(f"FP64: {t64 * 1e3:.1f} ms/step, {infos64[-1]['iters']} iterations")
     x64 = x.clone()
     it64 = infos64[-1]["iters"]
     assert all(i["ier"] == 0 for i in infos64), infos64
@@ -521,57 +872,67 @@ def main():
     sol.solve(b, xt, eps=2e-15, maxiter=20000)
     err64 = ((x64 - xt).norm() / xt.norm()).item()
     mixed = None
+    x32 = None
     if not args.no_mixed:
         t32, infos32 = run(True)
+        progress(f"mixed: {t32 * 1e3:.1f} ms/step, {infos32[-1]['iters']} inner iterations")
+        x32 = x.clone()
         i32 = infos32[-1]
-        mixed_bytes = N * (i32["iters"] * ALG_BYTES["iteration32"] + (i32["refinements"] + 1) * ALG_BYTES["refine64"])
+        mixed_bytes = N * (ALG_BYTES["setup_cold"] + i32["iters"] * ALG_BYTES["iteration32"] +
+                           i32["refinements"] * ALG_BYTES["refine64"])
         mixed_gbs = mixed_bytes / (i32["ms"] * 1e-3) / 1e9
         mixed = {"value": t32, "ms_per_step": t32 * 1e3, "inner_iters": i32["iters"],
                  "refinements": i32["refinements"], "err": i32["err"], "ier": i32["ier"],
                  "solve_ms_hip_events": i32["ms"], "solve_alg_GBps": mixed_gbs,
                  "solve_alg_frac": mixed_gbs / PEAK_HBM_GBS,
-                 "rel_l2_vs_tight_fp64": ((x - xt).norm() / xt.norm()).item(),
-                 "rel_l2_vs_fp64": ((x - x64).norm() / x64.norm()).item(), "speedup_vs_fp64": t64 / t32}
+                 "rel_l2_vs_tight_fp64": ((x32 - xt).norm() / xt.norm()).item(),
+                 "rel_l2_vs_fp64": ((x32 - x64).norm() / x64.norm()).item(), "speedup_vs_fp64": t64 / t32}
 
-    # dominant kernel of the FP64 iteration, timed live with HIP events on
-    # the solver's stream.  Per iteration: 2 SpMV, and 8 single-term PM stages
-    # or 4 fused pairs (whichever the solver times faster on this operator,
-    # as it does itself).  A fused pair's algorithmic bytes are its own
-    # minimum (two coefficient columns, the input, the output: 32 B/row), not
-    # the two passes it replaces.
-    dg.assemble_into(sol, b=b)
-    sol.solve(b, x, eps=args.eps)
-    t_spmv = sol.time_kernel(0, 50)
-    t_pm = sol.time_kernel(1, 50)
-    t_pair = sol.time_kernel(5, 50)
-    t_copy = sol.time_kernel(4, 50)
-    ix = "Ix%d" % sol.index_bits()
+    # In-solve kernel durations: HIP events around every launch of one more
+    # step of each form (identical to the timed ones), on the solver's stream.
+    sol.kernel_profile(True)
+    step(False)
+    if not args.no_mixed:
+        step(True)
+    prof = sol.kernel_profile(False)
+    progress("in-solve kernel profile taken")
     tune = sol.tuning()
-    st_pm = "true" if tune["stream64"] > 0 and tune["stream64"] & 1 else "false"
-    st_sp = "true" if tune["stream64"] > 0 and tune["stream64"] & 2 else "false"
-    cands = [(2 * t_spmv, f"k_spmv_v2<double,6,0,{ix},{st_sp}>", t_spmv, ALG_BYTES["spmv"] * N)]
-    if tune["fuse64"] == 1:
-        cands.append((4 * t_pair, f"k_pm2_v2<double,{ix},{st_pm},double>", t_pair, ALG_BYTES["pm_pair"] * N))
-    else:
-        cands.append((8 * t_pm, f"k_pm_v2<double,1,0,{ix},{st_pm},double>", t_pm, ALG_BYTES["pm_stage"] * N))
-    _, kname, kms, kbytes = max(cands)
-    achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = None
     pmc = os.path.join(ROOT, "profiles", f"traffic_config{args.config}.json")
     if os.path.exists(pmc):
-        with open(pmc) as f:
-            traffic = json.load(f).get(kname)
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "kernel": kname,
-                "alg_bytes_per_launch": kbytes, "avg_us": kms * 1e3,
-                "copy_GBps": 16 * N / (t_copy * 1e-3) / 1e9}
+        with open(pmc) as fh:
+            traffic = json.load(fh)
+    roofline = dominant(prof, "fp64", N, tune, traffic)
+    roofline["copy_GBps"] = 16 * N / (sol.time_kernel(4, 20) * 1e-3) / 1e9
+    if mixed:
+        mixed["roofline"] = dominant(prof, "fp32", N, tune, traffic)
     solve_ms = infos64[-1]["ms"]
     solve_gbs = N * (ALG_BYTES["setup_cold"] + it64 * ALG_BYTES["iteration"]) / (solve_ms * 1e-3) / 1e9
 
-    cpu = None
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(g, it64, args.cpu_iters, threads)
+    cpu, boundary = None, None
+    if not (args.no_cpu and args.no_boundary):
+        sysm = host_system(g)
+        progress(f"host system assembled ({sysm['N']} rows)")
+        assert sysm["N"] == N, (sysm["N"], N)
+        # the host system is the device's, bit for bit
+        bh = b.cpu().numpy()
+        system_same = bool(np.array_equal(bh, sysm["B"]))
+        if not args.no_boundary:
+            boundary = boundary_mode(sysm, args.eps)
+            progress(f"drop-in with host arrays: {boundary['s_per_solve'] * 1e3:.1f} ms")
+            boundary["resident_s_per_solve"] = t64
+            boundary["ratio_to_resident"] = boundary["s_per_solve"] / t64
+        del sysm
+        if not args.no_cpu:
+            # the host's cores, up to the 16 a one-GPU box grants a process
+            threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+            gx = {"fp64": x64.cpu().numpy()}
+            if x32 is not None:
+                gx["mixed"] = x32.cpu().numpy()
+            gx["tight_fp64"] = xt.cpu().numpy()
+            cpu = cpu_baseline(args.config, threads, args.cpu_runs, args.eps, gx)
+            cpu["gpu_iters"] = it64
+            cpu["host_rhs_bitexact"] = system_same
 
     out = {
         "metric": "pressure-solve s/step to 1e-10 rel. residual, FP64 vs mixed-prec; HBM GB/s",
@@ -585,6 +946,7 @@ def main():
                    "solve_alg_GBps": solve_gbs, "solve_alg_frac": solve_gbs / PEAK_HBM_GBS},
         "mixed": mixed,
         "roofline": roofline,
+        "boundary": boundary,
         "cpu_baseline": cpu,
     }
     if cpu:

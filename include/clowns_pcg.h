@@ -125,6 +125,21 @@ int clowns_grad_coefficients(clowns_solver *h, const double *d_lambda,
 int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32_t reps,
                               double *avg_ms);
 
+/* In-solve kernel durations.  enable = 1 arms the solver: every kernel
+ * launch of the solves that follow is bracketed by HIP events on the
+ * solver's stream.  enable = 0 disarms it and returns, per kernel class, the
+ * summed device time ms[c] and the number count[c] of the launches that did
+ * work (launches a run queued past its last iteration return at once and are
+ * left out).  Classes c = 0..8 FP64, 9..17 FP32 (the mixed inner solve), in
+ * this order: SpMV, single PM stage, fused PM stage pair, two-term or
+ * dot-carrying PM stage, p update, e update, x/r update with its dots,
+ * one-block finalize, true-residual SpMV.  ms and count hold 18 entries (or
+ * are NULL).  No counterpart in the reference (pcgpme_para.F90 has no
+ * timing): it measures the kernels behind the benchmark's roofline figure. */
+#define CLOWNS_KPROF_CLASSES 18
+int clowns_solver_kernel_profile(clowns_solver *h, int32_t enable, double *ms,
+                                 int32_t *count);
+
 /* ---- grid geometry and assembly --------------------------------------
  * Box convention: cells i=0..nx+1, j=0..ny+1, k=0..nz+1 whose outer ring is
  * solid (closed walls); box index c = i + X*(j + Y*k), X = nx+2, Y = ny+2,

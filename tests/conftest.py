@@ -120,6 +120,32 @@ def call_ref(s, iters=10000, eps=1e-10, nthr=1, parm=5.0, x0=None):
     return c
 
 
+def have_ref():
+    return os.path.exists(REF_PCG_SO)
+
+
+def host_config_system(g):
+    """A configs.Grid's pressure system assembled on the host by the C oracle
+    (set_nflag, row numbering, init_matrix, cal_matrix_element, RHS), in the
+    reference caller's layout: AL(NA,NL), LL(NA,NL) column-major, NA = N + 60
+    (src2LOWNS/1sub-momentum.F90:254-259)."""
+    A = g.arrays()
+    nc = g.ncell
+    nf, nb = np.zeros(nc, np.int8), np.zeros(nc, np.int8)
+    r2c, mno = np.zeros(nc, np.int32), np.zeros(nc, np.int32)
+    AL = np.zeros((6, nc))
+    LL = np.zeros((6, nc), np.int32)
+    D, B = np.zeros(nc), np.zeros(nc)
+    o = oracle()
+    o.oracle_build_system.restype = ctypes.c_int
+    N = o.oracle_build_system(g.nx, g.ny, g.nz, ptr(A["solid"]), ptr(A["F"]), ptr(A["fb"]), ptr(A["a"]),
+                              ptr(A["un"]), ptr(A["dx"]), ptr(A["xf"]), g.stride, ctypes.c_double(g.dt), ptr(nf),
+                              ptr(nb), ptr(r2c), ptr(mno), nc, ptr(AL), ptr(LL), ptr(D), ptr(B))
+    NA = N + 60
+    return dict(AL=np.asfortranarray(AL[:, :NA].T), LL=np.asfortranarray(LL[:, :NA].T), D=D[:N].copy(),
+                B=B[:N].copy(), N=N, NA=NA, NL=6)
+
+
 def dense_matrix(s):
     """Dense A (unscaled) from the reference-layout arrays."""
     N, NL = s["N"], s["NL"]

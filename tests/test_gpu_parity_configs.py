@@ -1,0 +1,117 @@
+"""GPU parity at the benchmark configurations.
+
+Configuration 1 at full size (64x64x32 cells, 77,776 rows): the drop-in's
+FP64 solve and the resident mixed solve against the reference's converged X.
+The reference here is the oracle at one OpenMP thread, which is bit-exact to
+the flang build of pcgpme_para.F90 at one thread (test_oracle_pcgpme.py); its
+spread over 1, 2, 4 and 8 threads is measured on the same system with the
+flang build itself (the C oracle where that was not built).  The bar:
+  * iteration count inside the reference's own range widened by 10%;
+  * relative L2 distance from the reference's X (the north_star figure) within
+    max(1e-9, 2x the reference's thread spread);
+  * distance from a tightly converged solution (EPS = 2e-15, stagnation exit)
+    no worse than twice the reference's own.
+Configuration 3 at full size (512x512x96 with bathymetry, 13.07M rows): the
+mixed solve against the FP64 solve of the same resident operator and a tight
+FP64 solve, plus the unscaled residual of both (size-independent checks; the
+reference takes seconds per solve there even at 16 threads, so the benchmark,
+not this test, compares with it).
+"""
+import numpy as np
+import pytest
+
+from conftest import call_oracle, call_ref, have_ref, host_config_system
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_config1_fp64_and_mixed_against_reference():
+    import torch
+    import clowns3d
+    from clowns3d import Solver, configs
+    g = configs.config1()
+    s = host_config_system(g)
+    N = s["N"]
+    ref1 = call_oracle(s)
+    refs = {1: ref1}
+    for nt in (2, 4, 8):
+        refs[nt] = call_ref(s, nthr=nt) if have_ref() else call_oracle(s, nthr=nt)
+    x1 = ref1["X"][:N]
+    tight = call_oracle(s, eps=2e-15)["X"][:N]
+    spread = max(rel(r["X"][:N], x1) for r in refs.values())
+    ref_err = max(rel(r["X"][:N], tight) for r in refs.values())
+    its = [r["ITER"] for r in refs.values()]
+    # the drop-in, FP64 (host arrays, the reference's argument list)
+    a, l = s["AL"].copy(order="F"), s["LL"].copy(order="F")
+    d, b = s["D"].copy(), s["B"].copy()
+    x = np.zeros(N + 64)
+    epsv = np.array([1e-10, 0.0])
+    it64, ier64 = clowns3d.pcgpme(a, 6, s["NA"], l, d, N, 64, b, x, epsv, 5.0, 10000)
+    x64 = x[:N]
+    # the resident mixed solve of the same system
+    dev = torch.device("cuda:0")
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+    sol = Solver(0)
+    sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
+    xm, im = sol.solve(t(s["B"]), eps=1e-10, mixed=True)
+    xm = xm.cpu().numpy()
+    sol.close()
+    print(f"config 1 (N={N}): reference ITER at 1/2/4/8 threads {its}, thread spread {spread:.2e}, "
+          f"reference error vs tight {ref_err:.2e}")
+    print(f"  FP64 drop-in: ITER {it64} (change {it64 - ref1['ITER']:+d}), rel L2 vs reference X "
+          f"{rel(x64, x1):.2e}, vs tight {rel(x64, tight):.2e}")
+    print(f"  mixed: {im['iters']} inner iterations, {im['refinements']} refinements, ERR {im['err']:.2e}, "
+          f"rel L2 vs reference X {rel(xm, x1):.2e}, vs tight {rel(xm, tight):.2e}")
+    assert ier64 == 0 and ref1["IER"] == 0 and im["ier"] == 0
+    assert epsv[0] <= 1e-10 and im["err"] <= 1e-10
+    assert 0.9 * min(its) - 1 <= it64 <= 1.1 * max(its) + 1
+    bar = max(1e-9, 2.0 * spread)
+    assert rel(x64, x1) <= bar
+    assert rel(x64, tight) <= max(1e-9, 2.0 * ref_err)
+    assert rel(xm, tight) <= max(1e-9, 2.0 * ref_err)
+    assert rel(xm, x1) <= max(bar, 2.0 * ref_err)
+
+
+def _unscaled_residual(dg, x):
+    """||b - A x|| / ||b|| of the unscaled system in the reference layout,
+    from the device assembly (torch gathers)."""
+    import torch
+    AL, LL, D, B = dg.assemble()
+    y = D * x
+    for k in range(AL.shape[0]):
+        idx = LL[k].long()
+        m = idx > 0
+        y = y + torch.where(m, AL[k] * x[(idx - 1).clamp(min=0)], torch.zeros_like(y))
+    return ((B - y).norm() / B.norm()).item()
+
+
+def test_config3_mixed_full_size():
+    import torch
+    from clowns3d import DeviceGrid, Solver, configs
+    g = configs.config3()
+    dg = DeviceGrid(g)
+    N = dg.N
+    sol = Solver(0)
+    b = dg.assemble_into(sol)
+    x64, i64 = sol.solve(b, eps=1e-10)
+    x32, i32 = sol.solve(b, eps=1e-10, mixed=True)
+    xt, it = sol.solve(b, eps=2e-15, maxiter=20000)
+    torch.cuda.synchronize()
+    e64 = ((x64 - xt).norm() / xt.norm()).item()
+    e32 = ((x32 - xt).norm() / xt.norm()).item()
+    d = ((x32 - x64).norm() / x64.norm()).item()
+    r64, r32 = _unscaled_residual(dg, x64), _unscaled_residual(dg, x32)
+    print(f"config 3 (N={N}): FP64 {i64['iters']} iterations ERR {i64['err']:.2e} vs tight {e64:.2e} "
+          f"residual {r64:.2e} | mixed {i32['iters']} inner / {i32['refinements']} refinements ERR "
+          f"{i32['err']:.2e} vs tight {e32:.2e} residual {r32:.2e} | mixed vs FP64 {d:.2e}")
+    sol.close()
+    assert N == 13066133
+    assert i64["ier"] == 0 and i32["ier"] == 0
+    assert i64["err"] <= 1e-10 and i32["err"] <= 1e-10
+    # as accurate as the FP64 solve at the same stopping rule
+    assert e32 <= max(2.0 * e64, 1e-9)
+    assert r32 <= max(2.0 * r64, 1e-12)

@@ -39,17 +39,19 @@ def _solve_all(s, stream):
 
 @pytest.mark.parametrize("shape", [(160, 128, 6), (33, 17, 9)])
 def test_streamed_forms_bit_identical(shape):
+    """CLOWNS_STREAM: 1 both streamed, 0 neither, 2 the PM stages only, 3
+    the SpMV only (the solver reports bit 0 PM, bit 1 SpMV), -1 timed."""
     s = box_system(*shape, seed=23)
-    on, info_on, tune_on = _solve_all(s, 1)
-    off, info_off, tune_off = _solve_all(s, 0)
-    auto, _, tune_auto = _solve_all(s, -1)  # back to timing per operator
-    assert tune_on["stream64"] == 3 and tune_on["stream32"] == 3
-    assert tune_off["stream64"] == 0 and tune_off["stream32"] == 0
-    assert tune_auto["stream64"] in (0, 1, 2, 3) and tune_auto["fuse64"] in (0, 1)
-    for a, b, c in zip(on, off, auto):
-        assert np.array_equal(a, b) and np.array_equal(a, c)
-    for a, b in zip(info_on, info_off):
-        assert a["iters"] == b["iters"] and a["ier"] == b["ier"] == 0
+    runs = {st: _solve_all(s, st) for st in (1, 0, 2, 3, -1)}
+    for st, bits in ((1, 3), (0, 0), (2, 1), (3, 2)):
+        assert runs[st][2]["stream64"] == bits and runs[st][2]["stream32"] == bits
+    assert runs[-1][2]["stream64"] in (0, 1, 2, 3) and runs[-1][2]["fuse64"] in (0, 1)
+    ref, ref_info, _ = runs[1]
+    for st in (0, 2, 3, -1):
+        for a, b in zip(ref, runs[st][0]):
+            assert np.array_equal(a, b)
+        for a, b in zip(ref_info, runs[st][1]):
+            assert a["iters"] == b["iters"] and a["ier"] == b["ier"] == 0
 
 
 def _solve_fused(s, fuse):
