@@ -75,6 +75,7 @@ SIGNATURES = {
     "clowns_grad_coefficients": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "clowns_solver_time_kernel": (ctypes.c_int, [_P, _I, _I, ctypes.POINTER(_D)]),
     "clowns_solver_kernel_profile": (ctypes.c_int, [_P, _I, _P, _P]),
+    "clowns_solver_patterns": (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "clowns_pcgpme_will": (None, [_P] * 12),
     "clowns_solver_solve_will": (ctypes.c_int, [_P, _P, _P, _D, _I, ctypes.POINTER(Info), ctypes.POINTER(_I)]),
     "clowns_will_filter": (ctypes.c_int, [_P, _P, _I, _I, _I, _P, _P, _P, _P, ctypes.POINTER(_I)]),

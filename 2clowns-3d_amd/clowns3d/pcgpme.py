@@ -242,6 +242,13 @@ class Solver:
             out[p] = {c: (int(cnt[off + i]), float(ms[off + i])) for i, c in enumerate(self.KERNEL_CLASSES)}
         return out
 
+    def patterns(self):
+        """(table rows, matrix rows on them) of the resident operator's row
+        patterns (clowns_solver_patterns); (0, 0) when not used."""
+        n, r = ctypes.c_int32(), ctypes.c_int32()
+        L.check(L.lib().clowns_solver_patterns(self._h, ctypes.byref(n), ctypes.byref(r)), "clowns_solver_patterns")
+        return n.value, r.value
+
     def time_kernel(self, kernel, reps=50):
         ms = ctypes.c_double()
         L.check(L.lib().clowns_solver_time_kernel(self._h, int(kernel), int(reps), ctypes.byref(ms)),

@@ -624,6 +624,89 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
     }
 }
 
+// ------------------------------------------------------- row patterns --
+// After PCGPME's scaling and sort most rows of an assembled pressure
+// operator repeat one of a few rows exactly: every interior cell of a
+// uniform region gives the same six coefficients and diagonal (at
+// 512x512x96 with bathymetry 95.4% of the rows fall on four such rows, at
+// 256x256x64 with an obstacle 97.3%).  A per-operator table of up to PAT_MAX
+// rows (six coefficients and D, FP64) and a one-byte pattern id per row
+// (PAT_NONE: the row's own values) let the SpMV read one byte for those rows
+// instead of 56 (FP64) or 24 (FP32) bytes of coefficients.  (The PM stages
+// read one coefficient a row: measured on MI355X, an id in its place gains
+// nothing in FP64 and 6% in FP32 with 8-bit index codes, lost 7% with
+// 16-bit ones; they keep the coefficient columns.)  The values it takes from the table are
+// the same bits (or the same FP32 / half roundings of them) the arrays
+// hold, so results do not change.
+constexpr int PAT_MAX = 16;
+constexpr int PAT_NONE = 255;
+constexpr int PAT_W = 7;  // values per table row: AL(1..6), D
+
+// Sampled rows of the scaled, sorted operator for the table: row s * stride
+// for s < S, its six coefficients and D.
+__global__ void __launch_bounds__(BS)
+k_pat_sample(int N, int lda, const double *__restrict__ AL, const double *__restrict__ D, int S, int stride,
+             double *__restrict__ keys) {
+    const int s = blockIdx.x * BS + threadIdx.x;
+    if (s >= S) return;
+    const long row = (long)s * stride;
+    if (row >= N) return;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) keys[(size_t)s * PAT_W + k] = AL[(size_t)k * lda + row];
+    keys[(size_t)s * PAT_W + 6] = D[row];
+}
+
+// Pattern id of every row: the table row whose seven values equal the
+// row's bit for bit, else PAT_NONE; *nnone counts the latter.  The ids past
+// N (up to a multiple of 4 plus 64) are PAT_NONE.
+__global__ void __launch_bounds__(BS)
+k_pat_assign(int N, int npad, int lda, const double *__restrict__ AL, const double *__restrict__ D,
+             const double *__restrict__ tab, int npat, uint8_t *__restrict__ pid, int *__restrict__ nnone) {
+    __shared__ unsigned long long t[PAT_MAX * PAT_W];
+    if (threadIdx.x < PAT_MAX * PAT_W)
+        t[threadIdx.x] = threadIdx.x < npat * PAT_W ? (unsigned long long)__double_as_longlong(tab[threadIdx.x]) : 0ull;
+    __syncthreads();
+    const int i = blockIdx.x * BS + threadIdx.x;
+    int none = 0;
+    if (i < N) {
+        unsigned long long v[PAT_W];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = (unsigned long long)__double_as_longlong(AL[(size_t)k * lda + i]);
+        v[6] = (unsigned long long)__double_as_longlong(D[i]);
+        int id = PAT_NONE;
+        for (int p = 0; p < npat && id == PAT_NONE; ++p) {
+            bool eq = true;
+#pragma unroll
+            for (int k = 0; k < PAT_W; ++k) eq = eq && v[k] == t[p * PAT_W + k];
+            if (eq) id = p;
+        }
+        pid[i] = (uint8_t)id;
+        none = id == PAT_NONE;
+    } else if (i < npad) {
+        pid[i] = (uint8_t)PAT_NONE;
+    }
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    if (none) atomicAdd(&cnt, 1);
+    __syncthreads();
+    if (threadIdx.x == 0 && cnt) atomicAdd(nnone, cnt);
+}
+
+// The pattern table in LDS as the kernel's type: TC = T for the SpMV,
+// half-rounded FP32 (the inner preconditioner's coefficients) for H.
+template <typename TC, bool HALF = false>
+__device__ __forceinline__ void pat_load(TC *t, const double *__restrict__ tab, int n) {
+    for (int k = threadIdx.x; k < n; k += BS) {
+        const double v = tab[k];
+        if constexpr (HALF)
+            t[k] = (TC)(float)(_Float16)(float)v;
+        else
+            t[k] = (TC)v;
+    }
+    __syncthreads();
+}
+
 // Four consecutive PM coefficients as FP32: one 8-byte (half precision) or
 // 16-byte load.
 template <bool ST, typename C>
@@ -735,23 +818,59 @@ k_pm2_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
 // AXSUB (pcgpme_para.F90:323-352) with two consecutive rows per thread:
 // y = A x in sorted slot order (MODE 0), or the true residual y = v1 - A x
 // (MODE 3, the mixed solve's FP64 refinement).
-template <typename T, int NL, int MODE, typename IX, bool ST>
+// PAT (NL = 6): the coefficients and D of pattern rows come from the table
+// ptab (pid the rows' pattern ids).
+template <typename T, int NL, int MODE, typename IX, bool ST, bool PAT = false>
 __global__ void __launch_bounds__(BS)
 k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL, IX LL,
           const T *__restrict__ D, const T *__restrict__ x, const T *__restrict__ v1, T *__restrict__ y,
-          const int *__restrict__ order) {
+          const int *__restrict__ order, const uint8_t *__restrict__ pid = nullptr,
+          const double *__restrict__ ptab = nullptr) {
     static_assert(MODE == 0 || MODE == 3, "y = A x or y = v1 - A x");
+    static_assert(!PAT || NL == 6, "patterns of six-slot rows");
     if (sc->done) return;
+    __shared__ T tab[PAT ? PAT_MAX * PAT_W : 1];
+    if (PAT) pat_load<T>(tab, ptab, PAT_MAX * PAT_W);
     const int i = (row_block(order) * BS + threadIdx.x) * 2;
     if (i + 1 < N) {
         typename V2<T>::t a[NL];
         int2 l[NL];
 #pragma unroll
-        for (int k = 0; k < NL; ++k) {
-            l[k] = LL.col((size_t)k * lda).template pair<ST>(i);
-            a[k] = lds2<ST>(AL + (size_t)k * lda + i);
+        for (int k = 0; k < NL; ++k) l[k] = LL.col((size_t)k * lda).template pair<ST>(i);
+        typename V2<T>::t d;
+        bool both = false;
+        int p0 = PAT_NONE, p1 = PAT_NONE;
+        if (PAT) {
+            const uint16_t pp = ldv<ST>(reinterpret_cast<const uint16_t *>(pid + i));
+            p0 = pp & 255;
+            p1 = pp >> 8;
+            both = p0 < PAT_MAX && p1 < PAT_MAX;
         }
-        const auto d = lds2<ST>(D + i);
+        if (both) {
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                a[k].x = tab[p0 * PAT_W + k];
+                a[k].y = tab[p1 * PAT_W + k];
+            }
+            d.x = tab[p0 * PAT_W + 6];
+            d.y = tab[p1 * PAT_W + 6];
+        } else {
+#pragma unroll
+            for (int k = 0; k < NL; ++k) a[k] = lds2<ST>(AL + (size_t)k * lda + i);
+            d = lds2<ST>(D + i);
+            if (PAT) {
+                if (p0 < PAT_MAX) {
+#pragma unroll
+                    for (int k = 0; k < NL; ++k) a[k].x = tab[p0 * PAT_W + k];
+                    d.x = tab[p0 * PAT_W + 6];
+                }
+                if (p1 < PAT_MAX) {
+#pragma unroll
+                    for (int k = 0; k < NL; ++k) a[k].y = tab[p1 * PAT_W + k];
+                    d.y = tab[p1 * PAT_W + 6];
+                }
+            }
+        }
         const auto xv = ld2(x + i);
         T acc0 = xv.x * d.x, acc1 = xv.y * d.y;
 #pragma unroll
@@ -823,18 +942,41 @@ __device__ __forceinline__ void row_codes(const uint8_t *__restrict__ Lr, int i,
 // (Ix16 with Lr16, or Ix8 with Lr8) decodes the codes, wide slots reading
 // the column-major LL.  With *nonunit == 0 every diagonal is 1 and is not
 // read (x * 1 = x exactly).  Same per-row arithmetic as k_spmv_v2.
-template <typename IX, typename CT, bool ST>
+// PAT: the coefficients of pattern rows come from the table ptab (FP32
+// roundings of its values, as ALr holds them).
+template <typename IX, typename CT, bool ST, bool PAT = false>
 __global__ void __launch_bounds__(BS)
 k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restrict__ ALr,
               const CT *__restrict__ Lr, IX LL, const float *__restrict__ D, const int *__restrict__ nonunit,
-              const float *__restrict__ x, float *__restrict__ y) {
+              const float *__restrict__ x, float *__restrict__ y, const uint8_t *__restrict__ pid = nullptr,
+              const double *__restrict__ ptab = nullptr) {
     if (sc->done) return;
+    __shared__ float tab[PAT ? PAT_MAX * PAT_W : 1];
+    if (PAT) pat_load<float>(tab, ptab, PAT_MAX * PAT_W);
     const bool unit = *nonunit == 0;
     const int i = (logical_block() * BS + threadIdx.x) * 2;
     if (i + 1 < N) {
-        const f4v *ap = reinterpret_cast<const f4v *>(ALr + (size_t)6 * i);
-        const f4v a0 = ldv<ST>(ap), a1 = ldv<ST>(ap + 1), a2 = ldv<ST>(ap + 2);
-        const float ar[2][6] = {{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y}, {a1.z, a1.w, a2.x, a2.y, a2.z, a2.w}};
+        float ar[2][6];
+        int p[2] = {PAT_NONE, PAT_NONE};
+        if (PAT) {
+            const uint16_t pp = ldv<ST>(reinterpret_cast<const uint16_t *>(pid + i));
+            p[0] = pp & 255;
+            p[1] = pp >> 8;
+        }
+        if (PAT && p[0] < PAT_MAX && p[1] < PAT_MAX) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int k = 0; k < 6; ++k) ar[r][k] = tab[p[r] * PAT_W + k];
+        } else {
+            const f4v *ap = reinterpret_cast<const f4v *>(ALr + (size_t)6 * i);
+            const f4v a0 = ldv<ST>(ap), a1 = ldv<ST>(ap + 1), a2 = ldv<ST>(ap + 2);
+            const float av[12] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int k = 0; k < 6; ++k) ar[r][k] = (PAT && p[r] < PAT_MAX) ? tab[p[r] * PAT_W + k] : av[6 * r + k];
+        }
         int cr[2][6];
         row_codes<ST>(Lr, i, cr);
         const auto b = LL.base[i / IX::ROWS];

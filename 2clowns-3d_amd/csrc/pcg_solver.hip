@@ -11,10 +11,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -193,6 +195,13 @@ struct Knobs {
     // Block order of the FP64 gather kernels for an assembled operator (see
     // ensure_block_order).  CLOWNS_BLOCK_ORDER = 0 switches it off.
     bool block_order = true;
+    // Row patterns (k_pat_assign): the SpMV of both precisions reads a
+    // one-byte pattern id instead of the coefficients of rows that repeat a
+    // table row.  Measured on MI355X at 13.1M rows (95% of rows on
+    // patterns): FP32 SpMV 108 -> 65 us, FP64 SpMV (streamed) 198 -> 143 us;
+    // at 2.45M rows FP64 SpMV 36 -> 20 us (cached).  Results are
+    // bit-identical.  CLOWNS_PATTERNS = 0 switches them off.
+    bool patterns = true;
     // CLOWNS_VERBOSE = 1: report per-operator tuning decisions on stderr.
     bool verbose = false;
 };
@@ -223,6 +232,12 @@ struct clowns_solver {
     DevBuf<int4> base8;
     bool ll8_ok = false;
     int ll8_wide = 0;      // slots read from the 32-bit LL (LL8_WIDE)
+    DevBuf<uint8_t> pid;   // row pattern ids (k_pat_assign), valid if pat_ok
+    DevBuf<double> ptab;   // pattern table, PAT_MAX rows of PAT_W values
+    DevBuf<double> pat_keys;  // sampled rows for a new table
+    int npat = 0;          // rows in ptab (kept for the next operator)
+    int pat_rows = 0;      // rows of the resident operator on a pattern
+    bool pat_ok = false;
     DevBuf<float> AL32, D32;
     DevBuf<float> ALr32;   // row-major FP32 coefficients of the FP32 SpMV (Knobs::rows32)
     DevBuf<short> Lr16;    // row-major 16-bit codes of the same
@@ -356,6 +371,8 @@ template <typename T> struct Op {
     const short *Lr16 = nullptr;
     const uint8_t *Lr8 = nullptr;   // ... 8-bit codes (with LL8), or null
     const int *nonunit = nullptr;   // ... device count of rows whose FP32 diagonal is not 1
+    const uint8_t *pid = nullptr;   // row pattern ids and table (NL = 6), or null
+    const double *ptab = nullptr;
     KProf *prof = nullptr;          // armed kernel profile, or null
 };
 
@@ -581,10 +598,31 @@ template <typename T, int NL, typename IX, bool ST>
 void launch_spmv_v2_st(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int mode, const T *x, const T *v1,
                        T *y) {
     const int G2 = geo_of(A.N, 2).G;
+    if constexpr (NL == 6) {
+        if (A.pid) {
+            if (mode == 0)
+                k_spmv_v2<T, 6, 0, IX, ST, true>
+                    <<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order, A.pid, A.ptab);
+            else
+                k_spmv_v2<T, 6, 3, IX, ST, true>
+                    <<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order, A.pid, A.ptab);
+            return;
+        }
+    }
     if (mode == 0)
         k_spmv_v2<T, NL, 0, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order);
     else
         k_spmv_v2<T, NL, 3, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order);
+}
+
+template <typename IX, typename CT, bool ST>
+void launch_rows32_st(hipStream_t s, int G2, const Scal *sc, const Op<float> &A, IX ix, const CT *Lr,
+                      const float *x, float *y) {
+    if (A.pid)
+        k_spmv_rows32<IX, CT, ST, true>
+            <<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y, A.pid, A.ptab);
+    else
+        k_spmv_rows32<IX, CT, ST, false><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y);
 }
 
 template <typename IX, typename CT>
@@ -592,9 +630,9 @@ void launch_rows32(hipStream_t s, const Scal *sc, const Op<float> &A, IX ix, con
                    float *y) {
     const int G2 = geo_of(A.N, 2).G;
     if (A.st_spmv)
-        k_spmv_rows32<IX, CT, true><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y);
+        launch_rows32_st<IX, CT, true>(s, G2, sc, A, ix, Lr, x, y);
     else
-        k_spmv_rows32<IX, CT, false><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y);
+        launch_rows32_st<IX, CT, false>(s, G2, sc, A, ix, Lr, x, y);
 }
 
 template <typename T, int NL, typename IX>
@@ -893,6 +931,7 @@ int check_nl(int NL) { return (NL == 6 || NL == 4) ? CLOWNS_OK : CLOWNS_EBADNL; 
 
 int build_ll16(clowns_solver *h);
 int build_ll8(clowns_solver *h);
+int build_patterns(clowns_solver *h);
 int ensure_block_order(clowns_solver *h);
 
 // Load AL/LL/D (reference layout, stride lda) from `kind` memory and scale +
@@ -953,7 +992,8 @@ int load_matrix(clowns_solver *h, const double *AL, int NL, int lda, const int *
     h->will_rev_valid = false;
     h->order_G = 0;  // a caller's operator: natural order unless set_block_order
     h->order_pending = false;
-    return build_ll16(h);
+    int rc2 = build_ll16(h);
+    return rc2 ? rc2 : build_patterns(h);
 }
 
 int ensure_f32(clowns_solver *h) {
@@ -1037,6 +1077,81 @@ int build_ll8(clowns_solver *h) {
     return CLOWNS_OK;
 }
 
+// Row patterns of the resident (scaled, sorted) operator: the table of the
+// last operator is kept, and a new operator first takes its ids from it (a
+// time-stepped run reassembles an operator of the same structure every
+// step); a new table is built when fewer than 4 in 5 rows match: the rows
+// of a 32768-row sample that repeat most often (at least 1 in 200 of the
+// sample), counted on the host.  Patterns are used while at least half the
+// rows are on one.
+constexpr int PAT_MIN_ROWS = 1 << 18;  // below this a solve is launch-bound
+constexpr int PAT_SAMPLE = 1 << 15;
+
+int assign_patterns(clowns_solver *h, int *nnone) {
+    hipStream_t s = h->stream;
+    const int N = h->N;
+    fill<int>(s, h->flag.p, 1, 0);
+    const int npad = n_pad(N) + 4;
+    k_pat_assign<<<cdiv(npad, BS), BS, 0, s>>>(N, npad, h->lda, h->AL.p, h->D.p, h->ptab.p, h->npat, h->pid.p,
+                                               h->flag.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(nnone, h->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return CLOWNS_OK;
+}
+
+int build_patterns(clowns_solver *h) {
+    h->pat_ok = false;
+    h->pat_rows = 0;
+    const int N = h->N;
+    if (!h->kn.patterns || h->NL != 6 || N < PAT_MIN_ROWS) return CLOWNS_OK;
+    int rc;
+    if ((rc = h->pid.ensure((size_t)n_pad(N) + 4)) || (rc = h->ptab.ensure(PAT_MAX * PAT_W)) ||
+        (rc = h->flag.ensure(1)))
+        return rc;
+    hipStream_t s = h->stream;
+    int nnone = N;
+    if (h->npat > 0 && (rc = assign_patterns(h, &nnone))) return rc;
+    if (h->npat == 0 || nnone > N / 5) {
+        const int S = std::min(N, PAT_SAMPLE), stride = N / S;
+        if ((rc = h->pat_keys.ensure((size_t)S * PAT_W))) return rc;
+        k_pat_sample<<<cdiv(S, BS), BS, 0, s>>>(N, h->lda, h->AL.p, h->D.p, S, stride, h->pat_keys.p);
+        HIPCHK(hipGetLastError());
+        try {  // host containers: an allocation failure is reported, not thrown across the C-ABI
+            std::vector<double> keys((size_t)S * PAT_W);
+            HIPCHK(hipMemcpyAsync(keys.data(), h->pat_keys.p, keys.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            typedef std::array<uint64_t, PAT_W> Key;
+            std::map<Key, int> count;
+            for (int r = 0; r < S; ++r) {
+                Key k;
+                std::memcpy(k.data(), &keys[(size_t)r * PAT_W], sizeof(Key));
+                ++count[k];
+            }
+            std::vector<std::pair<int, Key>> top;
+            for (const auto &kv : count)
+                if (kv.second >= std::max(2, S / 200)) top.push_back({kv.second, kv.first});
+            // most frequent first, ties by value: the same table on every run
+            std::sort(top.begin(), top.end(), [](const auto &a, const auto &b) {
+                return a.first != b.first ? a.first > b.first : a.second < b.second;
+            });
+            h->npat = (int)std::min<size_t>(top.size(), PAT_MAX);
+            std::vector<double> tab((size_t)PAT_MAX * PAT_W, 0.0);
+            for (int p = 0; p < h->npat; ++p) std::memcpy(&tab[(size_t)p * PAT_W], top[p].second.data(), sizeof(Key));
+            HIPCHK(hipMemcpyAsync(h->ptab.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s));
+            if ((rc = assign_patterns(h, &nnone))) return rc;
+        } catch (const std::bad_alloc &) {
+            return CLOWNS_ENOMEM;
+        }
+    }
+    h->pat_rows = N - nnone;
+    h->pat_ok = h->npat > 0 && nnone <= N / 2;
+    if (h->kn.verbose)
+        fprintf(stderr, "clowns_pcg: N=%d row patterns: %d in the table, %d rows on them%s\n", N, h->npat,
+                h->pat_rows, h->pat_ok ? "" : " (not used)");
+    return CLOWNS_OK;
+}
+
 // Views of the resident operator.
 Op<double> op64(const clowns_solver *h) {
     Op<double> A{h->N, h->NL, h->lda, h->AL.p, h->LL.p, h->D.p};
@@ -1049,6 +1164,10 @@ Op<double> op64(const clowns_solver *h) {
         A.base8 = h->base8.p;
     }
     if (h->order_G > 0 && h->order_G == geo_of(h->N, 2).G) A.order = h->order.p;
+    if (h->pat_ok) {
+        A.pid = h->pid.p;
+        A.ptab = h->ptab.p;
+    }
     return A;
 }
 Op<float> op32(const clowns_solver *h) {
@@ -1065,6 +1184,10 @@ Op<float> op32(const clowns_solver *h) {
         A.base8 = h->base8.p;
     }
     A.v4 = h->kn.v4_32;
+    if (h->pat_ok) {
+        A.pid = h->pid.p;
+        A.ptab = h->ptab.p;
+    }
     if (h->have_rows) {
         // the row-major SpMV: codes of the width the FP32 kernels run with
         A.ALr = h->ALr32.p;
@@ -1326,6 +1449,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
         flag("CLOWNS_PM_HALF", k.pm_half);
         flag("CLOWNS_BLOCK_ORDER", k.block_order);
         flag("CLOWNS_VERBOSE", k.verbose);
+        flag("CLOWNS_PATTERNS", k.patterns);
         flag("CLOWNS_CLASSIFY_HOST", h->cls_force_host);
         if (const char *e = getenv("CLOWNS_POLL")) h->poll = std::max(1, atoi(e));
         if (const char *e = getenv("CLOWNS_TAU_MIN")) h->tau_min = std::max(1e-8, atof(e));
@@ -1340,7 +1464,7 @@ extern "C" int clowns_solver_destroy(clowns_solver *h) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     h->AL.release(); h->D.release(); h->W.release(); h->LL.release(); h->LL16.release(); h->base16.release(); h->LL8.release(); h->base8.release(); h->order.release(); h->order_first.release();
-    h->AL32.release(); h->D32.release(); h->ALh.release(); h->ALr32.release(); h->Lr16.release(); h->Lr8.release(); h->nonunit.release();
+    h->AL32.release(); h->D32.release(); h->ALh.release(); h->ALr32.release(); h->Lr16.release(); h->Lr8.release(); h->nonunit.release(); h->pid.release(); h->ptab.release(); h->pat_keys.release();
     h->ALt.release(); h->Dt.release(); h->Wt.release(); h->LLt.release();
     h->LLt16.release(); h->baset16.release(); h->LLt8.release(); h->baset8.release();
     h->AL0.release(); h->D0.release(); h->LL0.release();
@@ -2001,6 +2125,7 @@ extern "C" int clowns_solver_assemble(clowns_solver *h, const clowns_grid_dev *g
     h->raw_valid = keep_raw != 0;
     h->will_rev_valid = false;
     if ((rc = build_ll16(h))) return rc;
+    if ((rc = build_patterns(h))) return rc;
     if ((rc = note_block_order(h, g))) return rc;
     HIPCHK(hipStreamSynchronize(s));
     return CLOWNS_OK;
@@ -2395,6 +2520,13 @@ extern "C" int clowns_solver_set_block_order(clowns_solver *h, const int32_t *d_
     if ((rc = h->order.ensure(G))) return rc;
     HIPCHK(hipMemcpy(h->order.p, ho.data(), (size_t)G * sizeof(int), hipMemcpyHostToDevice));
     h->order_G = G;
+    return CLOWNS_OK;
+}
+
+extern "C" int clowns_solver_patterns(clowns_solver *h, int32_t *npat, int32_t *rows) {
+    if (!h || h->N < 1) return CLOWNS_EBADARG;
+    if (npat) *npat = h->pat_ok ? h->npat : 0;
+    if (rows) *rows = h->pat_ok ? h->pat_rows : 0;
     return CLOWNS_OK;
 }
 

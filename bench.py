@@ -48,6 +48,7 @@ resident; a step is one update (clowns_solver_vof).
 """
 import argparse
 import ctypes
+import itertools
 import json
 import math
 import os
@@ -823,24 +824,23 @@ def main():
     if args.pmc_run:
         # an assembled FP64 solve and a mixed solve in each of the kernel
         # forms the solver may pick here (streamed or cached PM stages and
-        # SpMV, 8- or 16-bit index codes), so the traffic of whichever form a
-        # timed run picks was counted inside a solve; then the copy kernel
-        # the counters are scaled by
+        # SpMV, 8- or 16-bit index codes, single or fused PM stages), so the
+        # traffic of whichever form a timed run picks was counted inside a
+        # solve; then the copy kernel the counters are scaled by
         tunes = []
-        for st in ("0", "1", "2", "3"):
-            for ll8 in ("0", "1"):
-                os.environ["CLOWNS_STREAM"], os.environ["CLOWNS_LL8"] = st, ll8
-                try:
-                    s2 = Solver(0)
-                finally:
-                    del os.environ["CLOWNS_STREAM"], os.environ["CLOWNS_LL8"]
-                for mixed in (False, True):
-                    dg.assemble_into(s2, b=b)
-                    x.zero_()
-                    s2.solve(b, x, eps=args.eps, maxiter=10000, mixed=mixed)
-                tunes.append(s2.tuning())
-                s2.close()
-                progress(f"counted forms {tunes[-1]}")
+        for st, ll8, fuse in itertools.product(("0", "1", "2", "3"), ("0", "1"), ("0", "1")):
+            os.environ["CLOWNS_STREAM"], os.environ["CLOWNS_LL8"], os.environ["CLOWNS_FUSE_PM"] = st, ll8, fuse
+            try:
+                s2 = Solver(0)
+            finally:
+                del os.environ["CLOWNS_STREAM"], os.environ["CLOWNS_LL8"], os.environ["CLOWNS_FUSE_PM"]
+            for mixed in (False, True):
+                dg.assemble_into(s2, b=b)
+                x.zero_()
+                s2.solve(b, x, eps=args.eps, maxiter=10000, mixed=mixed)
+            tunes.append(s2.tuning())
+            s2.close()
+            progress(f"counted forms {tunes[-1]}")
         dg.assemble_into(sol, b=b)
         sol.time_kernel(4, 3)
         torch.cuda.synchronize()

@@ -140,6 +140,14 @@ int clowns_solver_time_kernel(clowns_solver *h, int32_t kernel, int32_t reps,
 int clowns_solver_kernel_profile(clowns_solver *h, int32_t enable, double *ms,
                                  int32_t *count);
 
+/* Row patterns of the resident operator: rows whose six scaled, sorted
+ * coefficients and diagonal repeat one of up to 16 table rows bit for bit,
+ * for which the SpMV reads a one-byte pattern id instead of the
+ * coefficients (results unchanged).  *npat table rows and
+ * *rows rows on them; both 0 when the operator does not use them.  No
+ * counterpart in the reference (a storage form of the operator). */
+int clowns_solver_patterns(clowns_solver *h, int32_t *npat, int32_t *rows);
+
 /* ---- grid geometry and assembly --------------------------------------
  * Box convention: cells i=0..nx+1, j=0..ny+1, k=0..nz+1 whose outer ring is
  * solid (closed walls); box index c = i + X*(j + Y*k), X = nx+2, Y = ny+2,
