@@ -70,6 +70,7 @@ SIGNATURES = {
     "clowns_solver_set_stream": (ctypes.c_int, [_P, _P]),
     "clowns_solver_set_matrix": (ctypes.c_int, [_P, _P, _I, _I, _P, _P, _I, _D]),
     "clowns_solver_solve": (ctypes.c_int, [_P, _P, _P, _D, _I, _I, ctypes.POINTER(Info)]),
+    "clowns_solver_solve_noalloc": (ctypes.c_int, [_P, _P, _P, _D, _I, _I, ctypes.POINTER(Info)]),
     "clowns_solver_get_scaled": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "clowns_solver_solve_transpose": (ctypes.c_int, [_P, _P, _P, _D, _I, _I, ctypes.POINTER(Info)]),
     "clowns_grad_coefficients": (ctypes.c_int, [_P, _P, _P, _P, _P]),

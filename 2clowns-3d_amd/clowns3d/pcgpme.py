@@ -129,6 +129,17 @@ class Solver:
                                             ctypes.byref(info)), "clowns_solver_solve")
         return x, info.as_dict()
 
+    def solve_noalloc(self, b, x, eps=1e-10, maxiter=10000, mixed=False, warm=False):
+        """solve() through clowns_solver_solve_noalloc: no allocation in the
+        call (work memory in the library's fixed array; at most 524288 rows;
+        mixed needs one solve(mixed=True) first).  Returns (x, info);
+        raises ClownsError on a rejected call."""
+        flags = (L.CLOWNS_MIXED if mixed else 0) | (L.CLOWNS_WARM_START if warm else 0)
+        info = L.Info()
+        L.check(L.lib().clowns_solver_solve_noalloc(self._h, _dptr(b), _dptr(x), float(eps), int(maxiter), flags,
+                                                    ctypes.byref(info)), "clowns_solver_solve_noalloc")
+        return x, info.as_dict()
+
     def index_bits(self):
         """8 or 16 if the resident operator's kernels read 8- or 16-bit coded
         neighbour indices, else 32."""

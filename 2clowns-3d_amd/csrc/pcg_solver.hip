@@ -46,8 +46,10 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 template <typename T> struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool borrowed = false;  // memory the buffer does not own (never freed or grown)
     int ensure(size_t want) {
         if (want <= n) return CLOWNS_OK;
+        if (borrowed) return CLOWNS_EBADARG;
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
@@ -60,9 +62,15 @@ template <typename T> struct DevBuf {
         return CLOWNS_OK;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        borrowed = false;
+    }
+    void borrow(T *q, size_t cap) {
+        p = q;
+        n = cap;
+        borrowed = true;
     }
 };
 
@@ -1515,6 +1523,106 @@ extern "C" int clowns_solver_solve(clowns_solver *h, const double *d_b, double *
         return solve_mixed(h, A, A32, h->W.p, d_b, d_x, eps, maxiter, warm, info);
     }
     return solve_fp64(h, A, h->W.p, d_b, d_x, eps, maxiter, warm, info);
+}
+
+// ----------------------------------------------------- allocation-free solve
+// The work memory of clowns_solver_solve_noalloc: a file-scope device array
+// (every GPU the library runs on holds its own copy until the process ends)
+// for the runs' vectors (nine FP64 and nine FP32 ones of N + 64 entries),
+// the scaled right-hand side, the dot partials and the run scalars, for up
+// to NOALLOC_MAX_ROWS rows: 58 MiB.
+constexpr int NOALLOC_MAX_ROWS = 1 << 19;
+constexpr size_t NA_VEC = (size_t)NOALLOC_MAX_ROWS + PAD;                  // entries per vector
+constexpr size_t NA_G1 = (((size_t)NOALLOC_MAX_ROWS + BS - 1) / BS + 7) / 8 * 8;  // largest grid
+constexpr size_t NA_V64 = 0;                                                 // 9 FP64 vectors + bs
+constexpr size_t NA_V32 = NA_V64 + 10 * NA_VEC * sizeof(double);             // 9 FP32 vectors
+constexpr size_t NA_PART = NA_V32 + 9 * NA_VEC * sizeof(float);              // 2 x NA_G1 partials
+constexpr size_t NA_SC = NA_PART + 2 * NA_G1 * sizeof(double);               // 2 run scalars
+constexpr size_t NA_BYTES = NA_SC + 2 * sizeof(Scal);
+static_assert(NA_VEC * sizeof(float) % 256 == 0 && NA_V32 % 256 == 0 && NA_PART % 256 == 0 && NA_SC % 256 == 0,
+              "vectors start 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[64] = {};  // device address of `scratch` on each GPU
+
+// The address of `scratch` on the solver's (current) device, looked up once
+// per device: in host code the symbol names no device memory.
+int scratch_of(int dev, unsigned char **out) {
+    if (dev < 0 || dev >= 64) return CLOWNS_EBADARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch[dev];
+    return CLOWNS_OK;
+}
+
+template <typename T> void borrow_vecs(VecSet<T> &V, unsigned char *base) {
+    DevBuf<T> *all[] = {&V.r, &V.rh, &V.p0, &V.q, &V.e, &V.v, &V.y, &V.t, &V.x};
+    for (int k = 0; k < 9; ++k) all[k]->borrow((T *)(base + (size_t)k * NA_VEC * sizeof(T)), NA_VEC);
+    V.padN = -1;  // the array keeps the last call's contents: ghost rows cleared again
+}
+} // namespace
+
+extern "C" int clowns_solver_solve_noalloc(clowns_solver *h, const double *d_b, double *d_x, double eps,
+                                           int32_t maxiter, int32_t flags, clowns_info *info) {
+    if (!h || !d_b || !d_x || h->N < 1) return CLOWNS_EBADARG;
+    if (flags & ~(CLOWNS_MIXED | CLOWNS_WARM_START)) return CLOWNS_EBADARG;
+    if (h->N > NOALLOC_MAX_ROWS) {
+        fprintf(stderr, "clowns_pcg: clowns_solver_solve_noalloc: %d rows exceed its maximum of %d\n", h->N,
+                NOALLOC_MAX_ROWS);
+        return CLOWNS_EBADARG;
+    }
+    if ((flags & CLOWNS_MIXED) && !h->have32) {
+        fprintf(stderr, "clowns_pcg: clowns_solver_solve_noalloc: the operator's FP32 copies are not built "
+                        "(one clowns_solver_solve with CLOWNS_MIXED builds them)\n");
+        return CLOWNS_EBADARG;
+    }
+    // the pinned status mirror and the events come with the operator
+    if (!h->h_sc || !h->ev0 || !h->evp[0]) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    unsigned char *base = nullptr;
+    if ((rc = scratch_of(h->device, &base))) return rc;
+    if (eps <= 1e-15) eps = 1e-9;                                                // :51
+    if (maxiter <= 0) maxiter = (int)(10.0 * (std::sqrt((double)h->N) + 1.0));  // :52
+    // the run's work buffers point into the array for this call
+    VecSet<double> v64;
+    VecSet<float> v32;
+    DevBuf<double> bs, part;
+    DevBuf<Scal> sc;
+    borrow_vecs(v64, base + NA_V64);
+    bs.borrow((double *)(base + NA_V64 + 9 * NA_VEC * sizeof(double)), NA_VEC);
+    borrow_vecs(v32, base + NA_V32);
+    part.borrow((double *)(base + NA_PART), 2 * NA_G1);
+    sc.borrow((Scal *)(base + NA_SC), 2);
+    std::swap(h->v64, v64);
+    std::swap(h->v32, v32);
+    std::swap(h->bs, bs);
+    std::swap(h->part, part);
+    std::swap(h->sc, sc);
+    const int bs_padN = h->bs_padN;
+    const bool prof_on = h->prof.on;
+    h->bs_padN = -1;
+    h->prof.on = false;  // its events would be created on demand
+    // the grid's block order is taken if it was built (its build allocates)
+    Op<double> A = op64(h);
+    if (flags & CLOWNS_MIXED)
+        rc = solve_mixed(h, A, op32(h), h->W.p, d_b, d_x, eps, maxiter, (flags & CLOWNS_WARM_START) != 0, info);
+    else
+        rc = solve_fp64(h, A, h->W.p, d_b, d_x, eps, maxiter, (flags & CLOWNS_WARM_START) != 0, info);
+    std::swap(h->v64, v64);
+    std::swap(h->v32, v32);
+    std::swap(h->bs, bs);
+    std::swap(h->part, part);
+    std::swap(h->sc, sc);
+    h->bs_padN = bs_padN;
+    h->prof.on = prof_on;
+    return rc;
 }
 
 extern "C" int clowns_solver_get_scaled(clowns_solver *h, double *d_AL, int32_t *d_LL,

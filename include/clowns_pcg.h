@@ -94,6 +94,23 @@ int clowns_solver_solve(clowns_solver *h, const double *d_b, double *d_x,
                         double eps, int32_t maxiter, int32_t flags,
                         clowns_info *info);
 
+/* clowns_solver_solve without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving or
+ * time-stepping loops, harnesses that forbid it).  No counterpart in the
+ * reference.  Same arguments and results as clowns_solver_solve (bit for
+ * bit); the run's work vectors, scaled right-hand side, dot partials and
+ * scalars live in a file-scope device array of the library, 58 MiB, which
+ * every GPU the library runs on holds until the process ends.  Limits: at
+ * most 524288 rows (CLOWNS_EBADARG beyond, with a message on stderr); the
+ * operator comes from clowns_solver_set_matrix / clowns_solver_assemble as
+ * usual (they allocate), and CLOWNS_MIXED needs the operator's FP32 copies,
+ * which one clowns_solver_solve with CLOWNS_MIXED builds.  All calls on one
+ * GPU share the array: calls that overlap on one GPU (from several threads
+ * or streams) race. */
+int clowns_solver_solve_noalloc(clowns_solver *h, const double *d_b, double *d_x,
+                                double eps, int32_t maxiter, int32_t flags,
+                                clowns_info *info);
+
 /* Copy of the resident scaled/sorted operator (reference post-call layout,
  * column stride lda of set_matrix), the row scales W, and the final scaled
  * residual of the last solve.  Any pointer may be NULL. */
