@@ -8,5 +8,5 @@ from ._lib import ClownsError, Info, LIB_PATH, lib  # noqa: F401
 from .pcgpme import Solver, pcgpme, pcgpme_will  # noqa: F401
 from . import configs  # noqa: F401,E402
 from .assembly import DeviceGrid, classify, set_dx  # noqa: F401,E402
-from .autograd import PCGSolve, pcg_solve  # noqa: F401,E402
+from .autograd import OpenRatioSolve, PCGSolve, pcg_solve, pcg_solve_open_ratios  # noqa: F401,E402
 from . import decks  # noqa: F401,E402

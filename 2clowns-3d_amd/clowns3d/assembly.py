@@ -46,6 +46,7 @@ L.SIGNATURES.update({
                                                                                  ctypes.POINTER(ctypes.c_double)]),
     "clowns_assemble": (ctypes.c_int, [ctypes.POINTER(GridDev), _I, _P, _P, _P, _P, _P]),
     "clowns_solver_assemble": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), ctypes.c_double, _P, _I]),
+    "clowns_grad_open_ratios": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), _P, _P, _P, _P]),
     "clowns_solver_pressure_correction": (ctypes.c_int, [_P, ctypes.POINTER(GridDev), _P, _P, ctypes.c_double, _I,
                                                          ctypes.c_double, _I, ctypes.POINTER(PCInfo)]),
     "clowns_solver_classify": (ctypes.c_int, [_P, ctypes.POINTER(GridDesc)] + [_P] * 11 +
@@ -292,6 +293,37 @@ class DeviceGrid:
                                                       ctypes.c_void_p(Fn.data_ptr()), ctypes.byref(ms)),
                 "clowns_solver_vof_time_kernel")
         return ms.value
+
+    def with_open_ratios(self, a=None, fb=None):
+        """A copy of this grid view whose face open ratios a (3, ncell) and
+        cell open ratios fb (ncell,) are the given float64 CUDA tensors (kept
+        alive by the view); the flags and numbering stay this grid's."""
+        out = DeviceGrid.__new__(DeviceGrid)
+        out.__dict__.update(self.__dict__)
+        out.t = dict(self.t)
+        fields = {k: getattr(self.dev, k) for k, _ in GridDev._fields_}
+        for name, v in (("a", a), ("fb", fb)):
+            if v is None:
+                continue
+            if v.dtype != self._torch.float64 or not v.is_cuda or not v.is_contiguous():
+                raise TypeError(f"{name} must be a contiguous CUDA float64 tensor")
+            if v.numel() != self.t[name].numel():
+                raise ValueError(f"{name} has {v.numel()} entries, expected {self.t[name].numel()}")
+            out.t[name] = v
+            fields[name] = ctypes.c_void_p(v.data_ptr())
+        out.dev = GridDev(**fields)
+        return out
+
+    def grad_open_ratios(self, solver, lam, x):
+        """dL/da (3, ncell) and dL/dfb (ncell,) through this grid's assembly,
+        from lambda = A^{-T} dL/dx and x (clowns_grad_open_ratios)."""
+        torch = self._torch
+        ga = torch.empty(self.t["a"].shape, dtype=torch.float64, device=self.device)
+        gfb = torch.empty(self.t["fb"].shape, dtype=torch.float64, device=self.device)
+        L.check(L.lib().clowns_grad_open_ratios(solver._h, ctypes.byref(self.dev), ctypes.c_void_p(lam.data_ptr()),
+                                                ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(ga.data_ptr()),
+                                                ctypes.c_void_p(gfb.data_ptr())), "clowns_grad_open_ratios")
+        return ga, gfb
 
     def assemble_into(self, solver, parm=5.0, keep_raw=False, b=None):
         """Fused assembly + scaling + sort into the solver's resident operator.

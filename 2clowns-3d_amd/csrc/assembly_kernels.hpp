@@ -254,6 +254,141 @@ k_rhs(GridDev g, double *__restrict__ B, double *__restrict__ part) {
     if (threadIdx.x == 0) part[blockIdx.x] = lds[0];
 }
 
+// ------------------------------------------- gradients of the open ratios --
+// dL/da and dL/dfb through the assembly above, given lambda = A^{-T} dL/dx
+// and x for the operator and right-hand side assembled from g (the adjoint
+// of init_matrix's cal_alu_f, 1sub-matrix.F90:50-61,82-87, of
+// cal_matrix_element's diagonal, 1sub-momentum.F90:486-509, and of the RHS,
+// 1sub-momentum.F90:315-346).  Only fluid rows depend on them: a surface
+// row's D is 1 and its coefficients come from sp or are the constant rate.
+// With gAL(r,s) = -lambda_r x(LL(r,s)), gD(r) = -lambda_r x_r and, per open
+// slot, ALU_C = -mp 2/h/(h+hm) a/fb_c:
+//   dL/dALU_C = gAL (numbered neighbour) + gD dD/dALU_C, where
+//   dD/dALU_C = -1, or 0 for the Neumann slots (air or unnumbered
+//   non-surface neighbour: D -= ALU_C, then D += ALU_C);
+//   dALU_C/da = -mp 2/h/(h+hm)/fb_c,  dALU_C/dfb_c = -ALU_C/fb_c;
+//   db/da = +-(mp/dt) u/dx/fb_c (high/low face), db/dfb_c = -(mp/dt) rn/fb_c
+//   (zero where the reference zeroes b, |DR1/dt| <= 1e-15).
+// The flags, the row numbering and which faces are open are fixed by the
+// classification: their dependence on a and fb is discrete.
+struct RhsRow {
+    double rn;   // renzoku_n (divergence / fb)
+    bool live;   // b = mp DR1/dt (not zeroed)
+};
+
+__device__ __forceinline__ RhsRow rhs_row(const GridDev &g, long c, const int (&p)[3]) {
+    const int X = g.nx + 2, Y = g.ny + 2;
+    const size_t n = (size_t)g.ncell;
+    const double fbc = g.fb[c];
+    double rn = -10000.0;
+    if (fbc != 0.0) {
+        auto A = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.a[l * n + cc]; };
+        auto U = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.un[l * n + cc]; };
+        const long cx = c + 1, cy = c + X, cz = c + (long)X * Y;
+        rn = ((U(0, cx) * A(0, cx) - g.un[c] * g.a[c]) / g.dx[p[0]] +
+              (U(1, cy) * A(1, cy) - g.un[n + c] * g.a[n + c]) / g.dx[(size_t)g.stride + p[1]] +
+              (U(2, cz) * A(2, cz) - g.un[2 * n + c] * g.a[2 * n + c]) / g.dx[2 * (size_t)g.stride + p[2]]) /
+             fbc;
+    }
+    double ff = 1.0 - g.F[c];
+    ff = ff > 0.001 ? 0.001 : (ff < -0.001 ? -0.001 : ff);
+    const double dr1 = rn + ff / g.dt;
+    const double q = fabs(dr1 / g.dt);
+    return {rn, fbc != 0.0 && !(q <= 1e-15 && q != 0.0)};
+}
+
+// dL/dALU_C of cell q's slot toward neighbour nb (0 for a closed slot) and
+// dALU_C/da; q a fluid row.
+__device__ __forceinline__ void slot_grad(const GridDev &g, long q, long nb, long pc, int l, int pq, int pn,
+                                          double lam, double xq, const double *__restrict__ x, double &galu,
+                                          double &dalu_da, double &alu) {
+    galu = dalu_da = alu = 0.0;
+    const int fn = g.nf[nb];
+    const double af = g.nf[pc] == -1 ? 0.0 : g.a[(size_t)l * g.ncell + pc];
+    const double fbn = fn == -1 ? 0.0 : g.fb[nb];
+    if (fn == -1 || af == 0.0 || fbn == 0.0) return;  // LL_C = 0
+    const double h = g.dx[(size_t)l * g.stride + pq];
+    const double hm = g.dx[(size_t)l * g.stride + pn];
+    const double fbq = g.fb[q];
+    dalu_da = -g.mp * (2.0 / h / (h + hm) / fbq);
+    alu = -g.mp * (2.0 / h / (h + hm) * af / fbq);
+    const int np = g.mno[nb];
+    const bool numbered = !(fn == 0 || np == 0);
+    const double gD = -lam * xq;
+    if (numbered)
+        galu = -lam * x[np - 1] + gD * -1.0;
+    else
+        galu = gD * (fn != 2 ? 0.0 : -1.0);
+}
+
+// dL/da per face: face (l, pc) is the low face of cell pc; the cells on its
+// two sides are cm = pc - step[l] (its high face) and pc (its low face).
+__global__ void __launch_bounds__(256)
+k_grad_open_faces(GridDev g, const double *__restrict__ lam, const double *__restrict__ x, double *__restrict__ ga) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * g.ncell) return;
+    const int l = (int)(t / g.ncell);
+    const long pc = t - (long)l * g.ncell;
+    const int X = g.nx + 2, Y = g.ny + 2;
+    const int p[3] = {(int)(pc % X), (int)((pc / X) % Y), (int)(pc / ((long)X * Y))};
+    const long step[3] = {1, X, (long)X * Y};
+    double d = 0.0;
+    if (p[l] >= 1 && g.nf[pc] != -1) {
+        const long cm = pc - step[l];
+        for (int side = 0; side < 2; ++side) {
+            const long q = side == 0 ? cm : pc;      // the row's cell
+            const long nb = side == 0 ? pc : cm;     // its neighbour across the face
+            if (g.nf[q] != 1) continue;
+            const int r = g.mno[q];
+            if (r == 0) continue;
+            const double lr = lam[r - 1], xr = x[r - 1];
+            const int pq = side == 0 ? p[l] - 1 : p[l];
+            const int pn = side == 0 ? p[l] : p[l] - 1;
+            double galu, dalu_da, alu;
+            slot_grad(g, q, nb, pc, l, pq, pn, lr, xr, x, galu, dalu_da, alu);
+            d += galu * dalu_da;
+            // the right-hand side of row q
+            int pqv[3] = {p[0], p[1], p[2]};
+            pqv[l] = pq;
+            const RhsRow rr = rhs_row(g, q, pqv);
+            if (rr.live) {
+                const double u = g.un[(size_t)l * g.ncell + pc];
+                const double dxq = g.dx[(size_t)l * g.stride + pq];
+                d += lr * (g.mp / g.dt) * ((side == 0 ? u : -u) / dxq / g.fb[q]);
+            }
+        }
+    }
+    ga[t] = d;
+}
+
+// dL/dfb per cell: the cell's own row only (a neighbour's fb decides only
+// whether a slot is open).
+__global__ void __launch_bounds__(256)
+k_grad_open_cells(GridDev g, const double *__restrict__ lam, const double *__restrict__ x,
+                  double *__restrict__ gfb) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.ncell) return;
+    double d = 0.0;
+    const int r = g.mno[c];
+    if (g.nf[c] == 1 && r != 0) {
+        const int X = g.nx + 2, Y = g.ny + 2;
+        const int p[3] = {(int)(c % X), (int)((c / X) % Y), (int)(c / ((long)X * Y))};
+        const long step[3] = {1, X, (long)X * Y};
+        const double lr = lam[r - 1], xr = x[r - 1], fbc = g.fb[c];
+        for (int slot = 0; slot < 6; ++slot) {
+            const int l = slot >> 1, side = (slot & 1) ? 1 : -1;
+            const long nb = c + side * step[l];
+            const long pc = side > 0 ? nb : c;
+            double galu, dalu_da, alu;
+            slot_grad(g, c, nb, pc, l, p[l], p[l] + side, lr, xr, x, galu, dalu_da, alu);
+            d += galu * (-alu / fbc);
+        }
+        const RhsRow rr = rhs_row(g, c, p);
+        if (rr.live) d += lr * (g.mp / g.dt) * (-rr.rn / fbc);
+    }
+    gfb[c] = d;
+}
+
 __global__ void __launch_bounds__(1024) k_max_fin(const double *__restrict__ part, int G, double *__restrict__ out) {
     __shared__ double lds[1024];
     double m = 0.0;

@@ -2144,6 +2144,20 @@ extern "C" int clowns_assemble(const clowns_grid_dev *g, int32_t lda, double *d_
     return CLOWNS_OK;
 }
 
+extern "C" int clowns_grad_open_ratios(clowns_solver *h, const clowns_grid_dev *g, const double *d_lambda,
+                                       const double *d_x, double *d_ga, double *d_gfb) {
+    if (!h || !grid_ok(g) || !d_lambda || !d_x || (!d_ga && !d_gfb)) return CLOWNS_EBADARG;
+    int rc = set_device(h);
+    if (rc) return rc;
+    const GridDev G = grid_of(g);
+    hipStream_t s = h->stream;
+    if (d_ga) k_grad_open_faces<<<cdiv(3 * G.ncell, 256), 256, 0, s>>>(G, d_lambda, d_x, d_ga);
+    if (d_gfb) k_grad_open_cells<<<cdiv(G.ncell, 256), 256, 0, s>>>(G, d_lambda, d_x, d_gfb);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return CLOWNS_OK;
+}
+
 namespace {
 __global__ void __launch_bounds__(BS)
 k_block_first(int nb, const int *__restrict__ row2cell, int *__restrict__ out) {

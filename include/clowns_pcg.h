@@ -235,6 +235,21 @@ int clowns_assemble(const clowns_grid_dev *g, int32_t lda, double *d_AL,
 int clowns_solver_assemble(clowns_solver *h, const clowns_grid_dev *g,
                            double parm, double *d_B, int32_t keep_raw);
 
+/* Adjoint of the assembly: gradients of a loss L(x), x = A^{-1} b, with
+ * respect to the face open ratios a and the cell open ratios fb from which
+ * clowns_assemble / clowns_solver_assemble build A and b (cal_alu_f and the
+ * diagonal of init_matrix, src2LOWNS/1sub-matrix.F90:50-61,82-87;
+ * cal_matrix_element, 1sub-momentum.F90:481-548; the RHS,
+ * 1sub-momentum.F90:315-346), given lambda = A^{-T} dL/dx (from
+ * clowns_solver_solve_transpose) and x, both per row.  d_ga: 3*ncell entries
+ * in the layout of g->a (face = the low face of a cell); d_gfb: ncell
+ * entries; every entry is written, 0 where A and b do not depend on it
+ * (solid cells, closed faces, surface rows).  The flags, numbering and
+ * which faces are open stay as classified.  Either output may be NULL. */
+int clowns_grad_open_ratios(clowns_solver *h, const clowns_grid_dev *g,
+                            const double *d_lambda, const double *d_x,
+                            double *d_ga, double *d_gfb);
+
 /* The SMAC pressure-correction loop of one time step on the device
  * (src2LOWNS/1sub-momentum.F90:41-60 with check_dmax :419-463): the operator
  * is assembled once (fused with PCGPME's scaling and sort) and stays

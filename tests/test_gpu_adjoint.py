@@ -10,12 +10,17 @@
    differences with eps = 1e-6, to 1e-6 relative.  Each perturbed solve is
    done as a correction x0 + A'^{-1}(b' - A' x0) with the residual formed in
    FP64, so solver tolerance does not swamp the O(eps) difference.
+3. The same loss differentiated with respect to the face open ratios a and
+   the cell open ratios fb through the assembly (pcg_solve_open_ratios:
+   transpose solve + clowns_grad_open_ratios), against central differences
+   of the reassembled system along 20 seeded directions in (a, fb), at
+   configuration 4 and on a small bathymetry grid with partial cells.
 """
 import numpy as np
 import pytest
 import torch
 
-from clowns3d import DeviceGrid, Solver, configs, pcg_solve
+from clowns3d import DeviceGrid, Solver, configs, pcg_solve, pcg_solve_open_ratios
 from conftest import dense_matrix
 from _systems import box_system
 
@@ -120,3 +125,56 @@ def test_solver_orders_with_torch_default_stream():
     for _ in range(20):
         gAL, gD = sol.grad_coefficients(lam, x)  # allocates with torch.zeros, then the kernel writes
         assert torch.equal(gD, -lam * x)
+
+
+def _open_ratio_fd(g, nprobe, ndir, seed):
+    dg = DeviceGrid(g)
+    N = dg.N
+    rng = np.random.default_rng(seed)
+    probes = torch.tensor(rng.choice(N, min(nprobe, N), replace=False), device=dev)
+    w = torch.tensor(rng.uniform(0.5, 1.5, probes.numel()), device=dev)
+    sol = Solver(0)
+    a0, fb0 = dg.t["a"].clone(), dg.t["fb"].clone()
+    ar, fbr = a0.clone().requires_grad_(True), fb0.clone().requires_grad_(True)
+    x0 = pcg_solve_open_ratios(dg, ar, fbr, solver=sol, eps=1e-12)
+    (w * x0[probes] ** 2).sum().backward()
+    ga, gfb = ar.grad, fbr.grad
+    x0 = x0.detach()
+    assert ga.abs().sum() > 0 and gfb.abs().sum() > 0
+
+    def loss_at(a, fb):
+        AL, LL, D, b = dg.with_open_ratios(a.contiguous(), fb.contiguous()).assemble()
+        r = b - _apply(AL, LL, D, x0)
+        sol.set_matrix(AL.contiguous(), LL, D.contiguous())
+        d, info = sol.solve(r.contiguous(), eps=1e-12)
+        assert info["ier"] == 0, info
+        x = x0 + d
+        return (w * x[probes] ** 2).sum().item()
+
+    eps = 1e-6
+    errs = []
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    for _ in range(ndir):
+        # closed faces (a = 0) and solid cells stay as they are
+        va = torch.randn(a0.shape, generator=gen, dtype=torch.float64).to(dev) * a0
+        vfb = torch.randn(fb0.shape, generator=gen, dtype=torch.float64).to(dev) * fb0
+        pred = ((ga * va).sum() + (gfb * vfb).sum()).item()
+        fd = (loss_at(a0 + eps * va, fb0 + eps * vfb) - loss_at(a0 - eps * va, fb0 - eps * vfb)) / (2 * eps)
+        errs.append(abs(fd - pred) / max(abs(fd), 1e-300))
+    sol.close()
+    return N, errs
+
+
+def test_open_ratio_gradient_partial_cells():
+    """Bathymetry grid with partial cells (fb < 1 on 1832 cells, a < 1 on
+    3580 faces)."""
+    g = configs.config_bathymetry(48, 40, 16, nbumps=6, seed=7, width=(3, 10))
+    N, errs = _open_ratio_fd(g, 500, 20, 31)
+    print("open ratios, partial cells: N", N, "max rel err", max(errs))
+    assert max(errs) <= 1e-6
+
+
+def test_config4_open_ratio_gradient_vs_central_differences():
+    N, errs = _open_ratio_fd(configs.config2(), 1000, 20, 32)
+    print("open ratios, config 4: N", N, "max rel err", max(errs), "median", float(np.median(errs)))
+    assert max(errs) <= 1e-6
