@@ -316,6 +316,12 @@ struct Ix32 {
         return make_int4(j.x - 1, j.y - 1, j.z - 1, j.w - 1);
     }
     __device__ __forceinline__ int one(int i) const { return L[i] - 1; }
+    // Phased form (k_spmv_v2): the pair's raw codes, then 1-based indices.
+    typedef NV2<int>::t Raw;
+    typedef int Base;
+    __device__ __forceinline__ Base blk(int) const { return 0; }
+    template <bool ST = false> __device__ __forceinline__ Raw raw(int i) const { return ldn2<ST>(L + i); }
+    __device__ __forceinline__ int2 dec1(Raw v, int, Base) const { return make_int2(v.x, v.y); }
 };
 
 constexpr int LL16_ROWS = 512;  // rows per base block (two-row kernels: one block)
@@ -324,6 +330,7 @@ constexpr int LL16_WIDE = -32767;
 
 struct Ix16 {
     static constexpr int ROWS = LL16_ROWS;
+    static constexpr int WIDE = LL16_WIDE;
     const short *L;
     const int *L32;  // the same column, 32-bit (slots coded LL16_WIDE)
     const int2 *base;
@@ -354,6 +361,29 @@ struct Ix16 {
         return make_int4(dec(v.x, i, b), dec(v.y, i + 1, b), dec(v.z, i + 2, b), dec(v.w, i + 3, b));
     }
     __device__ __forceinline__ int one(int i) const { return dec(L[i], i, base[i / LL16_ROWS]); }
+    // Phased form for kernels that load several slots' codes before using
+    // any (k_spmv_v2): raw() the pair's codes, dec1() their indices 1-based
+    // (clamped to [1, N + 1]).  A wide slot's 32-bit index is loaded into
+    // the result without being touched, so the wait for it falls at the
+    // gather, after every slot's wide loads are in flight, not one slot at
+    // a time.
+    typedef NV2<short>::t Raw;
+    typedef int2 Base;
+    __device__ __forceinline__ Base blk(int i) const { return base[i / LL16_ROWS]; }
+    template <bool ST = false> __device__ __forceinline__ Raw raw(int i) const { return ldn2<ST>(L + i); }
+    __device__ __forceinline__ int near1(int v, int i, int2 b) const {
+        int j = i + v;
+        j = v >= 16384 ? i + b.x + (v - 16384) : j;
+        j = v < -16384 ? i + b.y + (v + 16385) : j;
+        j = v == LL16_GHOST ? N : j;
+        return min(max(j, 0), N) + 1;
+    }
+    __device__ __forceinline__ int2 dec1(Raw v, int i, Base b) const {
+        int2 j = make_int2(near1(v.x, i, b), near1(v.y, i + 1, b));
+        if (v.x == LL16_WIDE) j.x = L32[i];
+        if (v.y == LL16_WIDE) j.y = L32[i + 1];
+        return j;
+    }
 };
 
 // Build the 16-bit form of the sorted LL: one block per LL16_ROWS rows finds
@@ -438,6 +468,7 @@ constexpr int LL8_XM = 252, LL8_XP = 253, LL8_GHOST = 254, LL8_WIDE = 255;
 
 struct Ix8 {
     static constexpr int ROWS = LL8_ROWS;
+    static constexpr int WIDE = LL8_WIDE;
     const uint8_t *L;
     const int *L32;   // the same column, 32-bit (slots coded LL8_WIDE)
     const int4 *base; // per block: window starts of near-, near+, far-, far+
@@ -468,6 +499,29 @@ struct Ix8 {
                          dec(v >> 24, i + 3, b));
     }
     __device__ __forceinline__ int one(int i) const { return dec(L[i], i, base[i / LL8_ROWS]); }
+    // Phased form, as Ix16's.
+    typedef uint16_t Raw;
+    typedef int4 Base;
+    __device__ __forceinline__ Base blk(int i) const { return base[i / LL8_ROWS]; }
+    template <bool ST = false> __device__ __forceinline__ Raw raw(int i) const {
+        return ldv<ST>(reinterpret_cast<const uint16_t *>(L + i));
+    }
+    __device__ __forceinline__ int near1(int c, int i, int4 b) const {
+        const int cls = c >> 6;
+        const int bs = cls == 0 ? b.x : (cls == 1 ? b.y : (cls == 2 ? b.z : b.w));
+        int j = i + bs + (c & 63);
+        j = c == LL8_XM ? i - 1 : j;
+        j = c == LL8_XP ? i + 1 : j;
+        j = c == LL8_GHOST ? N : j;
+        return min(max(j, 0), N) + 1;
+    }
+    __device__ __forceinline__ int2 dec1(Raw v, int i, Base b) const {
+        const int c0 = v & 255, c1 = v >> 8;
+        int2 j = make_int2(near1(c0, i, b), near1(c1, i + 1, b));
+        if (c0 == LL8_WIDE) j.x = L32[i];
+        if (c1 == LL8_WIDE) j.y = L32[i + 1];
+        return j;
+    }
 };
 
 // Build the 8-bit form of the sorted LL: one block per LL8_ROWS rows finds
@@ -572,8 +626,9 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
         const C *__restrict__ a1, IX l1, const T *__restrict__ in,
         T *__restrict__ out, const T *__restrict__ w, double *__restrict__ part,
         const int *__restrict__ order) {
-    if (sc->done) return;
+    const int done = sc->done;  // read together with the block order
     const int lb = row_block(order);
+    if (done) return;
     const int i = (lb * BS + threadIdx.x) * 2;
     double s0 = 0.0, s1 = 0.0;
     if (i + 1 < N) {
@@ -693,20 +748,6 @@ k_pat_assign(int N, int npad, int lda, const double *__restrict__ AL, const doub
     if (threadIdx.x == 0 && cnt) atomicAdd(nnone, cnt);
 }
 
-// The pattern table in LDS as the kernel's type: TC = T for the SpMV,
-// half-rounded FP32 (the inner preconditioner's coefficients) for H.
-template <typename TC, bool HALF = false>
-__device__ __forceinline__ void pat_load(TC *t, const double *__restrict__ tab, int n) {
-    for (int k = threadIdx.x; k < n; k += BS) {
-        const double v = tab[k];
-        if constexpr (HALF)
-            t[k] = (TC)(float)(_Float16)(float)v;
-        else
-            t[k] = (TC)v;
-    }
-    __syncthreads();
-}
-
 // Four consecutive PM coefficients as FP32: one 8-byte (half precision) or
 // 16-byte load.
 template <bool ST, typename C>
@@ -791,8 +832,9 @@ __global__ void __launch_bounds__(BS)
 k_pm2_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
          const C *__restrict__ a1, IX l1, const T *__restrict__ in, T *__restrict__ out,
          const int *__restrict__ order) {
-    if (sc->done) return;
+    const int done = sc->done;  // read together with the block order
     const int i = (row_block(order) * BS + threadIdx.x) * 2;
+    if (done) return;
     // stage 1 at row r, and at its stage-2 neighbour m (a ghost m reads 0)
     auto one = [&](T c0, int jj0, T c1, int m, T xr) -> T {
         const T o1i = xr - c0 * in[jj0];
@@ -820,6 +862,11 @@ k_pm2_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
 // (MODE 3, the mixed solve's FP64 refinement).
 // PAT (NL = 6): the coefficients and D of pattern rows come from the table
 // ptab (pid the rows' pattern ids).
+// A wave's dependent memory round trips set this kernel's pace more than its
+// bytes (its row streams are a third of a PM stage's): the done flag, the
+// block order and the table are read together, the row streams (codes,
+// pattern ids, x) are in flight before the table is staged in LDS, and only
+// the gathers wait for the codes.
 template <typename T, int NL, int MODE, typename IX, bool ST, bool PAT = false>
 __global__ void __launch_bounds__(BS)
 k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL, IX LL,
@@ -828,25 +875,39 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
           const double *__restrict__ ptab = nullptr) {
     static_assert(MODE == 0 || MODE == 3, "y = A x or y = v1 - A x");
     static_assert(!PAT || NL == 6, "patterns of six-slot rows");
-    if (sc->done) return;
-    __shared__ T tab[PAT ? PAT_MAX * PAT_W : 1];
-    if (PAT) pat_load<T>(tab, ptab, PAT_MAX * PAT_W);
+    const int done = sc->done;
     const int i = (row_block(order) * BS + threadIdx.x) * 2;
-    if (i + 1 < N) {
-        typename V2<T>::t a[NL];
-        int2 l[NL];
+    __shared__ T tab[PAT ? PAT_MAX * PAT_W : 1];
+    double tv = 0.0;  // converted when staged, so nothing waits for it here
+    if (PAT && threadIdx.x < PAT_MAX * PAT_W) tv = ptab[threadIdx.x];
+    if (done) return;
+    const bool full = i + 1 < N;
+    typename IX::Raw raw[NL];
+    typename IX::Base bb;
+    typename V2<T>::t xv;
+    int p0 = PAT_NONE, p1 = PAT_NONE;
+    if (full) {
+        bb = LL.blk(i);
 #pragma unroll
-        for (int k = 0; k < NL; ++k) l[k] = LL.col((size_t)k * lda).template pair<ST>(i);
-        typename V2<T>::t d;
-        bool both = false;
-        int p0 = PAT_NONE, p1 = PAT_NONE;
+        for (int k = 0; k < NL; ++k) raw[k] = LL.col((size_t)k * lda).template raw<ST>(i);
         if (PAT) {
             const uint16_t pp = ldv<ST>(reinterpret_cast<const uint16_t *>(pid + i));
             p0 = pp & 255;
             p1 = pp >> 8;
-            both = p0 < PAT_MAX && p1 < PAT_MAX;
         }
-        if (both) {
+        xv = ld2(x + i);
+    }
+    if (PAT) {
+        if (threadIdx.x < PAT_MAX * PAT_W) tab[threadIdx.x] = (T)tv;
+        __syncthreads();
+    }
+    if (full) {
+        int2 l[NL];  // 1-based
+#pragma unroll
+        for (int k = 0; k < NL; ++k) l[k] = LL.col((size_t)k * lda).dec1(raw[k], i, bb);
+        typename V2<T>::t a[NL];
+        typename V2<T>::t d;
+        if (PAT && p0 < PAT_MAX && p1 < PAT_MAX) {
 #pragma unroll
             for (int k = 0; k < NL; ++k) {
                 a[k].x = tab[p0 * PAT_W + k];
@@ -871,12 +932,11 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
                 }
             }
         }
-        const auto xv = ld2(x + i);
         T acc0 = xv.x * d.x, acc1 = xv.y * d.y;
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
-            acc0 = acc0 + a[k].x * x[l[k].x];
-            acc1 = acc1 + a[k].y * x[l[k].y];
+            acc0 = acc0 + a[k].x * x[l[k].x - 1];
+            acc1 = acc1 + a[k].y * x[l[k].y - 1];
         }
         typename V2<T>::t o;
         if (MODE == 3) {
@@ -950,19 +1010,34 @@ k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restri
               const CT *__restrict__ Lr, IX LL, const float *__restrict__ D, const int *__restrict__ nonunit,
               const float *__restrict__ x, float *__restrict__ y, const uint8_t *__restrict__ pid = nullptr,
               const double *__restrict__ ptab = nullptr) {
-    if (sc->done) return;
-    __shared__ float tab[PAT ? PAT_MAX * PAT_W : 1];
-    if (PAT) pat_load<float>(tab, ptab, PAT_MAX * PAT_W);
+    const int done = sc->done;
     const bool unit = *nonunit == 0;
+    __shared__ float tab[PAT ? PAT_MAX * PAT_W : 1];
+    double tv = 0.0;  // converted when staged, so nothing waits for it here
+    if (PAT && threadIdx.x < PAT_MAX * PAT_W) tv = ptab[threadIdx.x];
+    if (done) return;
     const int i = (logical_block() * BS + threadIdx.x) * 2;
-    if (i + 1 < N) {
-        float ar[2][6];
-        int p[2] = {PAT_NONE, PAT_NONE};
+    const bool full = i + 1 < N;
+    int cr[2][6];
+    int p[2] = {PAT_NONE, PAT_NONE};
+    float2 xv;
+    std::remove_cv_t<std::remove_reference_t<decltype(*LL.base)>> b;
+    if (full) {
         if (PAT) {
             const uint16_t pp = ldv<ST>(reinterpret_cast<const uint16_t *>(pid + i));
             p[0] = pp & 255;
             p[1] = pp >> 8;
         }
+        row_codes<ST>(Lr, i, cr);
+        b = LL.base[i / IX::ROWS];
+        xv = ld2(x + i);
+    }
+    if (PAT) {
+        if (threadIdx.x < PAT_MAX * PAT_W) tab[threadIdx.x] = (float)tv;
+        __syncthreads();
+    }
+    if (full) {
+        float ar[2][6];
         if (PAT && p[0] < PAT_MAX && p[1] < PAT_MAX) {
 #pragma unroll
             for (int r = 0; r < 2; ++r)
@@ -977,21 +1052,25 @@ k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restri
 #pragma unroll
                 for (int k = 0; k < 6; ++k) ar[r][k] = (PAT && p[r] < PAT_MAX) ? tab[p[r] * PAT_W + k] : av[6 * r + k];
         }
-        int cr[2][6];
-        row_codes<ST>(Lr, i, cr);
-        const auto b = LL.base[i / IX::ROWS];
-        const float2 xv = ld2(x + i);
         float acc0 = xv.x, acc1 = xv.y;
         if (!unit) {
             const float2 d = lds2<ST>(D + i);
             acc0 = xv.x * d.x;
             acc1 = xv.y * d.y;
         }
+        int j[2][6];  // 1-based, as k_spmv_v2's
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             const IX c = LL.col((size_t)k * lda);
-            acc0 = acc0 + ar[0][k] * x[c.dec(cr[0][k], i, b)];
-            acc1 = acc1 + ar[1][k] * x[c.dec(cr[1][k], i + 1, b)];
+            j[0][k] = c.near1(cr[0][k], i, b);
+            j[1][k] = c.near1(cr[1][k], i + 1, b);
+            if (cr[0][k] == IX::WIDE) j[0][k] = c.L32[i];
+            if (cr[1][k] == IX::WIDE) j[1][k] = c.L32[i + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            acc0 = acc0 + ar[0][k] * x[j[0][k] - 1];
+            acc1 = acc1 + ar[1][k] * x[j[1][k] - 1];
         }
         float2 o;
         o.x = acc0;

@@ -9,6 +9,7 @@
 // device (k_fin_beta); the host only polls a pinned copy of the scalars every
 // few iterations, so iterations run back to back without host round trips.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <array>
@@ -95,15 +96,17 @@ template <typename T> struct VecSet {
 
 } // namespace
 
-// In-solve kernel durations (clowns_solver_kernel_profile): while armed, HIP
-// events on the solver's stream bracket every launch of the solve's kernels,
-// tagged with the kernel class, the Bi-CGSTAB run and its iteration; the
-// launches a run queued past its last iteration (they return at once) are
-// left out of the sums.  Disarmed (the default) it records nothing.
+// In-solve kernel durations (clowns_solver_kernel_profile): while armed,
+// every launch of the solve's kernels goes through hipExtLaunchKernelGGL
+// with a pair of HIP events that the runtime stamps with that dispatch's own
+// start and end (the interval a kernel trace reports, without the event
+// packets' own time), tagged with the kernel class, the Bi-CGSTAB run and
+// its iteration; the launches a run queued past its last iteration (they
+// return at once) are left out of the sums.  Disarmed (the default) it
+// records nothing and the launches are plain ones.
 enum KClass { KC_SPMV, KC_PM, KC_PM2, KC_PMDOT, KC_PUPD, KC_EUPD, KC_UPDATE, KC_FIN, KC_RESID, KC_N };
 struct KProf {
     bool on = false;
-    hipStream_t s = nullptr;
     std::vector<hipEvent_t> pool;  // two per record
     struct Rec {
         int cls, run, K;
@@ -119,12 +122,8 @@ struct KProf {
             if (hipEventCreate(&e) != hipSuccess) return -1;
             pool.push_back(e);
         }
-        if (hipEventRecord(pool[2 * i], s) != hipSuccess) return -1;
         rec.push_back({cls, run, K});
         return (int)i;
-    }
-    void end(int i) {
-        if (i >= 0) (void)hipEventRecord(pool[2 * i + 1], s);
     }
     void release() {
         for (hipEvent_t e : pool) (void)hipEventDestroy(e);
@@ -133,16 +132,38 @@ struct KProf {
         kout.clear();
     }
 };
+// The event pair a KScope armed for the next klaunch on this thread.
+struct KSlot {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+thread_local KSlot t_kslot;
+
 struct KScope {
     KProf *p;
     int i = -1;
     KScope(KProf *prof, int cls) : p(prof) {
-        if (p) i = p->begin(cls);
+        if (p && (i = p->begin(cls)) >= 0) t_kslot = {p->pool[2 * i], p->pool[2 * i + 1]};
     }
     ~KScope() {
-        if (p) p->end(i);
+        if (p && t_kslot.start) {  // no launch took the events: drop the record
+            if (i >= 0) p->rec[i].cls = -1;
+            t_kslot = {};
+        }
     }
 };
+
+// Launch f<<<g, b, 0, s>>>(a...); when a KScope armed an event pair, through
+// hipExtLaunchKernelGGL so the events carry the dispatch's start and end.
+template <typename... P, typename... A>
+inline void klaunch(void (*f)(P...), dim3 g, dim3 b, hipStream_t s, A... a) {
+    if (t_kslot.start) {
+        const KSlot k = t_kslot;
+        t_kslot = {};
+        hipExtLaunchKernelGGL(f, g, b, 0, s, k.start, k.stop, 0, a...);
+    } else {
+        f<<<g, b, 0, s>>>(a...);
+    }
+}
 
 // Run-time switches (environment variables), read once when a solver is
 // created and kept in that solver: creating another solver never changes the
@@ -411,9 +432,9 @@ template <typename T, typename C, typename IX>
 void launch_pm2_ix(hipStream_t s, int G2, const Scal *sc, const Op<T> &A, const C *a0, IX x0, const C *a1, IX x1,
                    const T *in, T *out) {
     if (A.st_pm)
-        k_pm2_v2<T, IX, true, C><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out, A.order);
+        klaunch(k_pm2_v2<T, IX, true, C>, G2, BS, s, sc, A.N, a0, x0, a1, x1, in, out, A.order);
     else
-        k_pm2_v2<T, IX, false, C><<<G2, BS, 0, s>>>(sc, A.N, a0, x0, a1, x1, in, out, A.order);
+        klaunch(k_pm2_v2<T, IX, false, C>, G2, BS, s, sc, A.N, a0, x0, a1, x1, in, out, A.order);
 }
 
 template <typename T, typename C>
@@ -445,10 +466,10 @@ template <typename T, int NT, typename IX, bool ST, typename C>
 void launch_pm_v2(hipStream_t s, int G2, const Scal *sc, int N, const C *a0, IX x0, const C *a1, IX x1,
                   const T *in, T *out, int dot, const T *w, double *part, const int *order) {
     switch (dot) {
-    case 0: k_pm_v2<T, NT, 0, IX, ST, C><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part, order); break;
-    case 1: k_pm_v2<T, NT, 1, IX, ST, C><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part, order); break;
-    case 2: k_pm_v2<T, NT, 2, IX, ST, C><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part, order); break;
-    default: k_pm_v2<T, NT, 3, IX, ST, C><<<G2, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part, order); break;
+    case 0: klaunch(k_pm_v2<T, NT, 0, IX, ST, C>, G2, BS, s, sc, N, a0, x0, a1, x1, in, out, w, part, order); break;
+    case 1: klaunch(k_pm_v2<T, NT, 1, IX, ST, C>, G2, BS, s, sc, N, a0, x0, a1, x1, in, out, w, part, order); break;
+    case 2: klaunch(k_pm_v2<T, NT, 2, IX, ST, C>, G2, BS, s, sc, N, a0, x0, a1, x1, in, out, w, part, order); break;
+    default: klaunch(k_pm_v2<T, NT, 3, IX, ST, C>, G2, BS, s, sc, N, a0, x0, a1, x1, in, out, w, part, order);
     }
 }
 
@@ -456,10 +477,10 @@ template <int NT, typename IX, bool ST, typename C>
 void launch_pm_v4(hipStream_t s, int G4, const Scal *sc, int N, const C *a0, IX x0, const C *a1, IX x1,
                   const float *in, float *out, int dot, const float *w, double *part) {
     switch (dot) {
-    case 0: k_pm_v4<NT, 0, IX, ST, C><<<G4, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
-    case 1: k_pm_v4<NT, 1, IX, ST, C><<<G4, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
-    case 2: k_pm_v4<NT, 2, IX, ST, C><<<G4, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
-    default: k_pm_v4<NT, 3, IX, ST, C><<<G4, BS, 0, s>>>(sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    case 0: klaunch(k_pm_v4<NT, 0, IX, ST, C>, G4, BS, s, sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    case 1: klaunch(k_pm_v4<NT, 1, IX, ST, C>, G4, BS, s, sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    case 2: klaunch(k_pm_v4<NT, 2, IX, ST, C>, G4, BS, s, sc, N, a0, x0, a1, x1, in, out, w, part); break;
+    default: klaunch(k_pm_v4<NT, 3, IX, ST, C>, G4, BS, s, sc, N, a0, x0, a1, x1, in, out, w, part); break;
     }
 }
 
@@ -609,28 +630,31 @@ void launch_spmv_v2_st(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int
     if constexpr (NL == 6) {
         if (A.pid) {
             if (mode == 0)
-                k_spmv_v2<T, 6, 0, IX, ST, true>
-                    <<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order, A.pid, A.ptab);
+                klaunch(k_spmv_v2<T, 6, 0, IX, ST, true>, G2, BS, s, sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order,
+                        A.pid, A.ptab);
             else
-                k_spmv_v2<T, 6, 3, IX, ST, true>
-                    <<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order, A.pid, A.ptab);
+                klaunch(k_spmv_v2<T, 6, 3, IX, ST, true>, G2, BS, s, sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order,
+                        A.pid, A.ptab);
             return;
         }
     }
     if (mode == 0)
-        k_spmv_v2<T, NL, 0, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order);
+        klaunch(k_spmv_v2<T, NL, 0, IX, ST>, G2, BS, s, sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order, nullptr,
+                nullptr);
     else
-        k_spmv_v2<T, NL, 3, IX, ST><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order);
+        klaunch(k_spmv_v2<T, NL, 3, IX, ST>, G2, BS, s, sc, A.N, A.lda, A.AL, ix, A.D, x, v1, y, A.order, nullptr,
+                nullptr);
 }
 
 template <typename IX, typename CT, bool ST>
 void launch_rows32_st(hipStream_t s, int G2, const Scal *sc, const Op<float> &A, IX ix, const CT *Lr,
                       const float *x, float *y) {
     if (A.pid)
-        k_spmv_rows32<IX, CT, ST, true>
-            <<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y, A.pid, A.ptab);
+        klaunch(k_spmv_rows32<IX, CT, ST, true>, G2, BS, s, sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y, A.pid,
+                A.ptab);
     else
-        k_spmv_rows32<IX, CT, ST, false><<<G2, BS, 0, s>>>(sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y);
+        klaunch(k_spmv_rows32<IX, CT, ST, false>, G2, BS, s, sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y, nullptr,
+                nullptr);
 }
 
 template <typename IX, typename CT>
@@ -865,31 +889,31 @@ int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc, int i
         // Q = PM*A*P (:149-151), after P = R + BETA*(P - AMU*Q) (:244) for K > 1.
         if (K > 1) {
             KScope ks(pf, kcls<T>(KC_PUPD));
-            k_pupd_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.p0.p, V.r.p, V.q.p);
+            klaunch(k_pupd_vec<T>, Gv, BS, s, sc, A.N, V.p0.p, V.r.p, V.q.p);
         }
         spmv<T>(s, sc, A, 0, V.p0.p, nullptr, V.y.p);
         const int Gq = apply_pm<T>(s, sc, A, V.y.p, V.q.p, V.y.p, V.t.p, 1, V.rh.p, part);
         {
             KScope ks(pf, kcls<T>(KC_FIN));
-            k_fin_alpha<<<1, FIN_BS, 0, s>>>(sc, part, Gq);
+            klaunch(k_fin_alpha, 1, FIN_BS, s, sc, part, Gq);
         }
         // V = PM*A*E with E = R - ALP*Q (:173-181).
         {
             KScope ks(pf, kcls<T>(KC_EUPD));
-            k_eupd_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.r.p, V.q.p, V.e.p);
+            klaunch(k_eupd_vec<T>, Gv, BS, s, sc, A.N, V.r.p, V.q.p, V.e.p);
         }
         spmv<T>(s, sc, A, 0, V.e.p, nullptr, V.y.p);
         const int Gw = apply_pm<T>(s, sc, A, V.y.p, V.v.p, V.y.p, V.t.p, 2, V.e.p, part);
         {
             KScope ks(pf, kcls<T>(KC_FIN));
-            k_fin_omega<<<1, FIN_BS, 0, s>>>(sc, part, Gw);
+            klaunch(k_fin_omega, 1, FIN_BS, s, sc, part, Gw);
         }
         {
             KScope ks(pf, kcls<T>(KC_UPDATE));
-            k_update_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
+            klaunch(k_update_vec<T>, Gv, BS, s, sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
         }
         KScope ks(pf, kcls<T>(KC_FIN));
-        k_fin_beta<<<1, FIN_BS, 0, s>>>(sc, part, Gv, K);
+        klaunch(k_fin_beta, 1, FIN_BS, s, sc, part, Gv, K);
     };
     Scal *slot[2] = {h->h_sc + 2, h->h_sc + 3};
     PollPlan plan;
@@ -1937,7 +1961,6 @@ extern "C" int clowns_solver_kernel_profile(clowns_solver *h, int32_t enable, do
         p.kout.clear();
         p.run = -1;
         p.K = 0;
-        p.s = h->stream;
         p.on = true;
         return CLOWNS_OK;
     }
@@ -1950,6 +1973,7 @@ extern "C" int clowns_solver_kernel_profile(clowns_solver *h, int32_t enable, do
     }
     for (size_t i = 0; i < p.rec.size(); ++i) {
         const KProf::Rec &r = p.rec[i];
+        if (r.cls < 0) continue;
         if (r.run >= 0 && r.run < (int)p.kout.size() && r.K > p.kout[r.run]) continue;  // returned at once
         float e = 0.f;
         HIPCHK(hipEventElapsedTime(&e, p.pool[2 * i], p.pool[2 * i + 1]));
