@@ -411,7 +411,14 @@ int clowns_solver_vof(clowns_solver *h, const clowns_grid_dev *g,
  * (ll = 0) or high (ll = 1) at [6 m + 2 l + ll].  On each face the cell
  * drains through, qf = sign(u) V / area / dt / a (0 where |u| dt < 1e-9 or
  * V = 0); the rest as clowns_solver_vof.  The listed cells must be flow
- * cells (CLOWNS_EBADARG otherwise). */
+ * cells (CLOWNS_EBADARG otherwise).
+ * PARITY UNPINNED: this surface stage is restated from quvwf_normal's
+ * source and checked only against the same formula written out by hand
+ * (and, downstream of the fluxes, against the pinned given-flux mode); the
+ * reference's PLIC state (the segment volumes it clips from its polyhedra,
+ * src2LOWNS/1sub-normal*.F90) is not built here, so no reference output
+ * exists to compare it with.  clowns_solver_vof (kaidan surface stage, or
+ * caller-given PLIC fluxes) is the pinned path. */
 int clowns_solver_vof_plic(clowns_solver *h, const clowns_grid_dev *g,
                            const double *d_Fp, int32_t n_plic,
                            const int32_t *d_plic_cells,
