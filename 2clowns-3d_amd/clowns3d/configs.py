@@ -4,7 +4,12 @@ A grid is a box of cells i=0..nx+1, j=0..ny+1, k=0..nz+1 whose outer ring is
 solid (closed walls), stored flat with index c = i + X*(j + Y*k), X = nx+2,
 Y = ny+2 -- the reference's k-major row order (src2LOWNS/nfd2015.F90:69-78).
 Fields, as the reference holds them per cell (src2LOWNS/0mod-arrays.F90):
-  solid  int8   1 = wall/object cell (nf = -1, nfb = 0, so mn = 0)
+  solid  int8   0 = flow cell, 1 = wall cell outside the case (nf = -1,
+                nfb = 0, so mn = 0), other values an object cell of the case
+                (nf = -1, numbered, its own F, fb, a, un) with that object
+                code: for a boundary object with nfb = b (1..199) b itself
+                for 2..126 (4: a pressure boundary), -2 for 1, 124 - b for
+                127..199; -1 / -128 the cells appended with nfb 300 / 200
   F      f64    VOF fill
   fb     f64    cell open ratio
   a      f64    (3, ncell) open ratio of the LOW face of the cell in x, y, z
@@ -118,6 +123,53 @@ def config_wavy_box(nx, ny, nz, box, seed=2024):
     g.set_water(wavy_surface(g))
     g.set_velocity(seed)
     g.name = f"wavy{nx}x{ny}x{nz}"
+    return g
+
+
+def open_object_faces(g, codes=(4, 5)):
+    """Open (a = 1) every face between a flow cell and an object cell whose
+    code is in `codes` (inflow and outflow boundaries are open to the flow;
+    set_faces closes every face against a solid or object cell)."""
+    for l, ax in enumerate((2, 1, 0)):  # a[l] is indexed (k, j, i): axis of l
+        obj = np.isin(g.solid, codes)
+        flow = g.solid == 0
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax] = slice(None, -1)
+        hi[ax] = slice(1, None)
+        lo, hi = tuple(lo), tuple(hi)
+        # face between cell m (low side) and cell c = m + e_l is a[l][c]
+        pair = (obj[lo] & flow[hi]) | (flow[lo] & obj[hi])
+        g.a[l][hi] = np.where(pair, 1.0, g.a[l][hi])
+
+
+def config_objects(nx=40, ny=24, nz=16, seed=2030):
+    """Wavy free surface with object cells of the case (boundary objects,
+    src2LOWNS/data_io.F90:171-199): the high-x wall a pressure-boundary
+    column (code 4, open to the flow), the low-x wall inflow boundary
+    objects (codes 5, 7, -2, -30: nfb 5, 7, 1, 154; open), a 2x2x2 pressure-boundary block in the water and an
+    obstacle whose cells are appended object cells (codes -1 and -128,
+    nfb 300 and 200) carrying seeded fill and open ratios; every object
+    cell keeps its own F, fb, a and un, as the case reader gives them."""
+    g = config_wavy_box(nx, ny, nz, (nx // 3, nx // 3 + 4, ny // 3, ny // 3 + 4, 0, 3), seed)
+    rng = np.random.default_rng(seed + 1)
+    i0, j0 = nx // 3 + 1, ny // 3 + 1
+    obst = g.solid[1:4, j0:j0 + 4, i0:i0 + 4]
+    obst[:] = np.where(rng.random(obst.shape) < 0.5, -1, -128).astype(np.int8)
+    g.F[1:4, j0:j0 + 4, i0:i0 + 4] = rng.uniform(0.0, 1.0, obst.shape)
+    g.fb[1:4, j0:j0 + 4, i0:i0 + 4] = rng.uniform(0.2, 1.0, obst.shape)
+    g.solid[1:-1, 1:-1, -1] = 4
+    g.solid[1:-1, 1:-1, 0] = 5
+    g.solid[1:-1, 2:-1:4, 0] = 7     # other boundary objects: nfb 7, 1 and 154
+    g.solid[1:-1, 3:-1:4, 0] = -2
+    g.solid[1:-1, 4:-1:4, 0] = -30
+    g.F[:, :, -1] = g.F[:, :, -2]
+    g.F[:, :, 0] = g.F[:, :, 1]
+    ib, jb = 2 * nx // 3, 2 * ny // 3
+    g.solid[2:4, jb:jb + 2, ib:ib + 2] = 4
+    g.F[2:4, jb:jb + 2, ib:ib + 2] = 1.0
+    open_object_faces(g, codes=(4, 5, 7, -2, -30))
+    g.name = f"objects{nx}x{ny}x{nz}"
     return g
 
 

@@ -20,7 +20,7 @@ class DeckDesc(ctypes.Structure):
                 ("ie", ctypes.c_int32), ("je", ctypes.c_int32), ("ke", ctypes.c_int32),
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
                 ("stride", ctypes.c_int32), ("sparse", ctypes.c_int32), ("big_endian", ctypes.c_int32),
-                ("inne", ctypes.c_int32), ("has_dt0", ctypes.c_int32),
+                ("inne", ctypes.c_int32), ("inne2", ctypes.c_int32), ("has_dt0", ctypes.c_int32),
                 ("t0", ctypes.c_double), ("g", ctypes.c_double), ("nu", ctypes.c_double),
                 ("dt0", ctypes.c_double)]
 

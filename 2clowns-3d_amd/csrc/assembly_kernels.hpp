@@ -29,6 +29,12 @@ struct GridDev {
     const double *sp_row, *F, *fb, *a, *un, *dx, *xf;
 };
 
+// A cell of the case (the reference's mn != 0): a flow cell, or an object
+// cell, whose nb is its nonzero object code (4: a pressure-boundary cell).
+// A cell outside the case reads as the reference's sentinel cell 0: its
+// fb, a and un are 0.
+__device__ __forceinline__ bool live(const GridDev &g, long c) { return g.nf[c] != -1 || g.nb[c] != 0; }
+
 template <bool FUSED>
 __global__ void __launch_bounds__(256)
 k_assemble(GridDev g, int lda, double *__restrict__ AL, int *__restrict__ LL,
@@ -58,12 +64,14 @@ k_assemble(GridDev g, int lda, double *__restrict__ AL, int *__restrict__ LL,
         const long nc = c + side * step[l];
         const long pc = side > 0 ? nc : c;  // cell whose low face is shared
         const int fn = g.nf[nc];
-        const double af = g.nf[pc] == -1 ? 0.0 : g.a[(size_t)l * g.ncell + pc];
-        const double fbn = fn == -1 ? 0.0 : g.fb[nc];
+        const double af = live(g, pc) ? g.a[(size_t)l * g.ncell + pc] : 0.0;
+        const double fbn = live(g, nc) ? g.fb[nc] : 0.0;
         alu[slot] = 0.0;
         llu[slot] = 0;
         de_adj[slot] = 0.0;
-        if (fn == -1 || af == 0.0 || fbn == 0.0) continue;  // LL_C = 0
+        // LL_C = 0: an object neighbour other than a pressure-boundary cell,
+        // a closed face or an empty neighbour (1sub-matrix.F90:41-46)
+        if ((fn == -1 && g.nb[nc] != 4) || af == 0.0 || fbn == 0.0) continue;
         const double h = g.dx[(size_t)l * g.stride + p[l]];
         const double hm = g.dx[(size_t)l * g.stride + p[l] + side];
         const double alu_c = -mp * (2.0 / h / (h + hm) * af / fbc);
@@ -103,8 +111,8 @@ k_assemble(GridDev g, int lda, double *__restrict__ AL, int *__restrict__ LL,
         double rn;
         if (fbc != 0.0) {
             const size_t n = (size_t)g.ncell;
-            auto A = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.a[l * n + cc]; };
-            auto U = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.un[l * n + cc]; };
+            auto A = [&](int l, long cc) { return live(g, cc) ? g.a[l * n + cc] : 0.0; };
+            auto U = [&](int l, long cc) { return live(g, cc) ? g.un[l * n + cc] : 0.0; };
             const long cx = c + 1, cy = c + X, cz = c + (long)X * Y;
             rn = ((U(0, cx) * A(0, cx) - g.un[c] * g.a[c]) / g.dx[p[0]] +
                   (U(1, cy) * A(1, cy) - g.un[n + c] * g.a[n + c]) / g.dx[(size_t)g.stride + p[1]] +
@@ -177,18 +185,26 @@ k_pressure_update(int N, const int32_t *__restrict__ row2cell, const double *__r
     if (r < N) p[row2cell[r]] -= xe[r];
 }
 
-// estimate_velocity2 (src2LOWNS/1sub-momentum.F90:590-651) for grids without
-// pressure-boundary objects: un(l,c) = un + dt*(pm - pc)/dxc on every open low
-// face of a non-air cell whose low neighbour is fluid or surface, with
-// pc = -xe(row of c), pm = -xe(row of the neighbour) (0 for an unnumbered
-// cell: the reference's never-written xe(0)), dxc = (dx_i + dx_{i-1})/2
-// (momentum_d, lib/src/math_func.F90:251-255); |un| < 1e-15 -> 0.
+// estimate_velocity2 (src2LOWNS/1sub-momentum.F90:590-651):
+// un(l,c) = un + dt*(pm - pc)/dxc with pc = -xe(row of c), pm = -xe(row of
+// the low neighbour) (0 for an unnumbered cell: the reference's
+// never-written xe(0)), dxc = (dx_i + dx_{i-1})/2 (momentum_d,
+// lib/src/math_func.F90:251-255), over the cells that are neither air nor
+// objects other than pressure-boundary cells (:600-602):
+//   a pressure-boundary cell (object code 4): every axis whose low neighbour
+//     is fluid or surface (:611-623);
+//   a fluid or surface cell: every open low face (a != 0) whose neighbour is
+//     fluid or surface (then |un| < 1e-15 -> 0) or a pressure-boundary cell
+//     (:624-646).
+// A low neighbour below the box is the sentinel (flag -1, code 0).
 __global__ void __launch_bounds__(256)
 k_velocity_correction(GridDev g, const double *__restrict__ xe, double *__restrict__ un) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.ncell) return;
     const int fc = g.nf[c];
-    if (fc == -1 || fc == 0) return;
+    if (fc == 0) return;
+    const bool pbc = fc == -1 && g.nb[c] == 4;
+    if (fc == -1 && !pbc) return;
     const int X = g.nx + 2, Y = g.ny + 2;
     const int p[3] = {(int)(c % X), (int)((c / X) % Y), (int)(c / ((long)X * Y))};
     const long step[3] = {1, X, (long)X * Y};
@@ -196,14 +212,17 @@ k_velocity_correction(GridDev g, const double *__restrict__ xe, double *__restri
     const double xec = -(nc ? xe[nc - 1] : 0.0);
     for (int l = 0; l < 3; ++l) {
         const size_t o = (size_t)l * g.ncell + c;
-        if (g.a[o] == 0.0) continue;
+        if (!pbc && g.a[o] == 0.0) continue;
+        if (p[l] < 1) continue;
         const long m = c - step[l];
-        if (g.nf[m] < 1) continue;
+        const int fm = g.nf[m];
+        const bool flow = fm >= 1;
+        if (!flow && (pbc || !(fm == -1 && g.nb[m] == 4))) continue;
         const double dxc = (g.dx[(size_t)l * g.stride + p[l]] + g.dx[(size_t)l * g.stride + p[l] - 1]) * 0.5;
         const int nm = g.mno[m];
         const double pm = -(nm ? xe[nm - 1] : 0.0);
         double v = un[o] + g.dt * (pm - xec) / dxc;
-        if (fabs(v) < 1e-15) v = 0.0;
+        if (!pbc && flow && fabs(v) < 1e-15) v = 0.0;
         un[o] = v;
     }
 }
@@ -225,8 +244,8 @@ k_rhs(GridDev g, double *__restrict__ B, double *__restrict__ part) {
             double rn;
             if (fbc != 0.0) {
                 const size_t n = (size_t)g.ncell;
-                auto A = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.a[l * n + cc]; };
-                auto U = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.un[l * n + cc]; };
+                auto A = [&](int l, long cc) { return live(g, cc) ? g.a[l * n + cc] : 0.0; };
+                auto U = [&](int l, long cc) { return live(g, cc) ? g.un[l * n + cc] : 0.0; };
                 const long cx = c + 1, cy = c + X, cz = c + (long)X * Y;
                 rn = ((U(0, cx) * A(0, cx) - g.un[c] * g.a[c]) / g.dx[p[0]] +
                       (U(1, cy) * A(1, cy) - g.un[n + c] * g.a[n + c]) / g.dx[(size_t)g.stride + p[1]] +
@@ -282,8 +301,8 @@ __device__ __forceinline__ RhsRow rhs_row(const GridDev &g, long c, const int (&
     const double fbc = g.fb[c];
     double rn = -10000.0;
     if (fbc != 0.0) {
-        auto A = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.a[l * n + cc]; };
-        auto U = [&](int l, long cc) { return g.nf[cc] == -1 ? 0.0 : g.un[l * n + cc]; };
+        auto A = [&](int l, long cc) { return live(g, cc) ? g.a[l * n + cc] : 0.0; };
+        auto U = [&](int l, long cc) { return live(g, cc) ? g.un[l * n + cc] : 0.0; };
         const long cx = c + 1, cy = c + X, cz = c + (long)X * Y;
         rn = ((U(0, cx) * A(0, cx) - g.un[c] * g.a[c]) / g.dx[p[0]] +
               (U(1, cy) * A(1, cy) - g.un[n + c] * g.a[n + c]) / g.dx[(size_t)g.stride + p[1]] +
@@ -304,9 +323,9 @@ __device__ __forceinline__ void slot_grad(const GridDev &g, long q, long nb, lon
                                           double &dalu_da, double &alu) {
     galu = dalu_da = alu = 0.0;
     const int fn = g.nf[nb];
-    const double af = g.nf[pc] == -1 ? 0.0 : g.a[(size_t)l * g.ncell + pc];
-    const double fbn = fn == -1 ? 0.0 : g.fb[nb];
-    if (fn == -1 || af == 0.0 || fbn == 0.0) return;  // LL_C = 0
+    const double af = live(g, pc) ? g.a[(size_t)l * g.ncell + pc] : 0.0;
+    const double fbn = live(g, nb) ? g.fb[nb] : 0.0;
+    if ((fn == -1 && g.nb[nb] != 4) || af == 0.0 || fbn == 0.0) return;  // LL_C = 0
     const double h = g.dx[(size_t)l * g.stride + pq];
     const double hm = g.dx[(size_t)l * g.stride + pn];
     const double fbq = g.fb[q];
@@ -333,7 +352,7 @@ k_grad_open_faces(GridDev g, const double *__restrict__ lam, const double *__res
     const int p[3] = {(int)(pc % X), (int)((pc / X) % Y), (int)(pc / ((long)X * Y))};
     const long step[3] = {1, X, (long)X * Y};
     double d = 0.0;
-    if (p[l] >= 1 && g.nf[pc] != -1) {
+    if (p[l] >= 1 && live(g, pc)) {
         const long cm = pc - step[l];
         for (int side = 0; side < 2; ++side) {
             const long q = side == 0 ? cm : pc;      // the row's cell

@@ -42,8 +42,10 @@ struct ClsGrid {
 __device__ __forceinline__ int cls_flag(const ClsGrid &g, long c) {
     return g.solid[c] ? -1 : g.nf[c];
 }
+// solid[c] = 1: outside the case (reads as the sentinel: 0); other nonzero
+// values: an object cell of the case, read with its own values.
 __device__ __forceinline__ double cls_face(const ClsGrid &g, int l, long c) {
-    return g.solid[c] ? 0.0 : g.a[(size_t)l * g.ncell + c];
+    return g.solid[c] == 1 ? 0.0 : g.a[(size_t)l * g.ncell + c];
 }
 __device__ __forceinline__ void cls_coords(const ClsGrid &g, long c, int p[3]) {
     p[0] = (int)(c % g.X);
@@ -62,14 +64,14 @@ __global__ void __launch_bounds__(256) k_cls_init(ClsGrid g, int *__restrict__ e
     if (c < g.ncell) {
         if (g.solid[c]) {
             g.nf[c] = -1;
-            g.nb[c] = 0;
+            g.nb[c] = g.solid[c] == 1 ? 0 : g.solid[c];  // object code
         } else {
             int p[3];
             cls_coords(g, c, p);
             const long up = c + (long)g.X * g.Y, dn = c - (long)g.X * g.Y;
             const double f = g.F[c];
             const bool air =
-                f <= 1e-12 || (f * g.fb[c] < 1.0e-3 && g.solid[dn] && (g.solid[up] ? 0.0 : g.F[up]) == 0.0);
+                f <= 1e-12 || (f * g.fb[c] < 1.0e-3 && g.solid[dn] && (g.solid[up] == 1 ? 0.0 : g.F[up]) == 0.0);
             if (f < 0.0) atomicOr(ext + 4, 1);  // clipping order matters: exact host path
             g.nb[c] = 0;
             if (air) {
@@ -158,8 +160,8 @@ __device__ double cls_fxx1(const ClsGrid &g, int ix, int ll, const int lg[3]) {
         const long base = (lg[0] + (long)g.X * (lg[1] + (long)g.Y * lg[2])) + ix * step[ll];
         for (int ii = -1; ii <= 1; ++ii) {
             const long n = base + ii * step[l];
-            const double f = g.solid[n] ? 0.0 : g.F[n];
-            const double fb = g.solid[n] ? 0.0 : g.fb[n];
+            const double f = g.solid[n] == 1 ? 0.0 : g.F[n];
+            const double fb = g.solid[n] == 1 ? 0.0 : g.fb[n];
             s = s + h(l, lg[l] + ii) * f * fb;
         }
         for (int m = 0; m < 3; ++m)

@@ -18,10 +18,13 @@
 // takes either byte order (from the first record's length), the writer
 // writes the one asked for.  Default INTEGER is 4 bytes and REAL 8 (-r8).
 //
-// Supported cases: every cell with nf = -1 has nfb = 0 (walls and solid
-// objects; the box model's solid cells).  Cells with other nfb values
-// (boundary objects, nfb >= 200 appended cells, nfb = -1 open cells) are
-// reported as CLOWNS_EDECK.
+// Cells with nf = -1 (data_io.F90:168-199): nfb = 0 cells are not numbered
+// (mn = 0: the box model's solid value 1); boundary objects (1 <= nfb <
+// 200) are numbered with the flow cells and appended objects (nfb >= 200)
+// after them, and become solid cells with their object code (see
+// object_code); nfb = -1 cells map to the reference's second sentinel cell
+// inne2+1 (flag -1, b = -1; code -127, read with zero fields like a cell
+// outside the case).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -152,18 +155,57 @@ class RecordWriter {
     std::vector<char> buf_;
 };
 
-// The cell numbering of read_data (data_io.F90:160-176): j outer, k, i
-// inner over cells with nf >= 0 (the box model has no numbered objects).
-// Returns the numbered box cells in order.
-std::vector<long> numbering(const clowns_deck_desc *d, const std::vector<int32_t> &nf) {
+// The object code of an nf = -1 cell from its nfb (the box grids' solid
+// value; 1: a cell the case does not number).  A boundary object with
+// nfb = b (1 <= b < 200) keeps b for 2..126, takes -2 for b = 1 and 124 - b
+// (-3..-75) for b >= 127; appended cells (nfb >= 200) take -1 when
+// nfb = 300 (their b becomes -1) and -128 otherwise (b = 0); nfb = -1 cells
+// take -127.
+int8_t object_code(int32_t nfb) {
+    if (nfb == -1) return -127;
+    if (nfb <= 0) return 1;
+    if (nfb >= 200) return nfb == 300 ? -1 : -128;
+    if (nfb == 1) return -2;
+    if (nfb >= 127) return (int8_t)(124 - nfb);
+    return (int8_t)nfb;
+}
+// The nfb a file holds for an object code (its inverse).
+int32_t file_nfb(int8_t code) {
+    if (code == 1 || code == 0) return 0;
+    if (code == -127) return -1;
+    if (code == -1) return 300;
+    if (code == -128) return 200;
+    if (code == -2) return 1;
+    if (code < 0) return 124 - code;
+    return code;
+}
+// nff%b the reference keeps for a cell of that code after read_data
+// (appended cells' nfb is reset, data_io.F90:190-194).
+int32_t kept_b(int8_t code) {
+    if (code == -1) return -1;
+    if (code == -128) return 0;
+    return file_nfb(code);
+}
+bool in_main(int32_t nf, int32_t nfb) { return nf >= 0 || (nfb >= 1 && nfb < 200); }
+bool appended(int32_t nf, int32_t nfb) { return nf < 0 && nfb >= 200; }
+
+// The cell numbering of read_data (data_io.F90:168-199): j outer, k, i
+// inner over flow cells and boundary objects (1..inne), then the same sweep
+// over appended object cells (inne+1..inne2).  Returns the numbered box
+// cells in order; *inne the first part's count.
+std::vector<long> numbering(const clowns_deck_desc *d, const std::vector<int32_t> &nf,
+                            const std::vector<int32_t> &nfb, long *inne) {
     const long X = d->nx + 2, Y = d->ny + 2, Z = d->nz + 2;
     std::vector<long> cells;
-    for (long j = 0; j < Y; ++j)
-        for (long k = 0; k < Z; ++k)
-            for (long i = 0; i < X; ++i) {
-                const long c = i + X * (j + Y * k);
-                if (nf[c] >= 0) cells.push_back(c);
-            }
+    for (int pass = 0; pass < 2; ++pass) {
+        for (long j = 0; j < Y; ++j)
+            for (long k = 0; k < Z; ++k)
+                for (long i = 0; i < X; ++i) {
+                    const long c = i + X * (j + Y * k);
+                    if (pass == 0 ? in_main(nf[c], nfb[c]) : appended(nf[c], nfb[c])) cells.push_back(c);
+                }
+        if (pass == 0) *inne = (long)cells.size();
+    }
     return cells;
 }
 
@@ -249,9 +291,9 @@ static int clowns_deck_open_impl(const char *grid_path, const char *data_path, c
     std::vector<int32_t> nf, nfb;
     int rc = read_flags(d, grid, data, nf, nfb);
     if (rc) return rc;
-    for (size_t c = 0; c < nf.size(); ++c)
-        if (nf[c] < 0 && nfb[c] != 0) return CLOWNS_EDECK;  // objects outside the box model
-    d->inne = (int32_t)numbering(d, nf).size();
+    long inne = 0;
+    d->inne2 = (int32_t)numbering(d, nf, nfb, &inne).size();
+    d->inne = (int32_t)inne;
     return CLOWNS_OK;
 }
 
@@ -288,21 +330,25 @@ static int clowns_deck_read_impl(const char *grid_path, const char *data_path, c
     std::vector<int32_t> nf, nfb;
     int rc = read_flags(d, grid2, data, nf, nfb);
     if (rc) return rc;
-    const std::vector<long> cells = numbering(d, nf);
-    const long inne = (long)cells.size();
+    long inne = 0;
+    const std::vector<long> cells = numbering(d, nf, nfb, &inne);
+    const long inne2 = (long)cells.size();
+    // without an open-ratio file the cells 1..inne are fully open and the
+    // appended ones closed (read_fdata, data_io.F90:359-362 and :314-316)
     for (long c = 0; c < nc; ++c) {
-        if (nf[c] < 0 && nfb[c] != 0) return CLOWNS_EDECK;
-        solid[c] = nf[c] < 0 ? 1 : 0;
+        solid[c] = nf[c] < 0 ? object_code(nfb[c]) : 0;
+        const bool open = in_main(nf[c], nfb[c]);
         F[c] = 0.0;
         p[c] = 0.0;
-        fb[c] = solid[c] ? 0.0 : 1.0;
+        fb[c] = open ? 1.0 : 0.0;
         for (int l = 0; l < 3; ++l) {
             un[l * nc + c] = 0.0;
-            a[l * nc + c] = solid[c] ? 0.0 : 1.0;
+            a[l * nc + c] = open ? 1.0 : 0.0;
         }
     }
+    // U of every numbered cell; F and P of the cells 1..inne
     auto put = [&](long nn, const double *u, const double *f, const double *pp) -> bool {
-        if (nn < 1 || nn > inne) return false;
+        if (nn < 1 || nn > inne2 || (nn > inne && (f || pp))) return false;
         const long c = cells[nn - 1];
         for (int l = 0; l < 3; ++l) un[l * nc + c] = u[l];
         if (f) F[c] = *f;
@@ -311,20 +357,33 @@ static int clowns_deck_read_impl(const char *grid_path, const char *data_path, c
     };
     // fields (data_io.F90:219-253); t0 = 0 with a dense grid still carries them
     if (!d->sparse) {
-        for (long nn = 1; nn <= inne; ++nn) {
+        for (long nn = 1; nn <= inne2; ++nn) {
             double u[3], f, pp;
-            if (!data.next() || !data.f64(u[0]) || !data.f64(u[1]) || !data.f64(u[2]) || !data.f64(f) ||
-                !data.f64(pp) || !put(nn, u, &f, &pp))
+            if (nn <= inne) {
+                if (!data.next() || !data.f64(u[0]) || !data.f64(u[1]) || !data.f64(u[2]) || !data.f64(f) ||
+                    !data.f64(pp) || !put(nn, u, &f, &pp))
+                    return CLOWNS_EDECK;
+            } else if (!data.next() || !data.f64(u[0]) || !data.f64(u[1]) || !data.f64(u[2]) ||
+                       !put(nn, u, nullptr, nullptr)) {
                 return CLOWNS_EDECK;
+            }
         }
     } else {
         int32_t icc, icc2;
-        if (!data.next() || !data.i32(icc) || !data.i32(icc2) || icc < 0 || icc2 != 0) return CLOWNS_EDECK;
+        if (!data.next() || !data.i32(icc) || !data.i32(icc2) || icc < 0 || icc2 < 0) return CLOWNS_EDECK;
         for (int32_t q = 0; q < icc; ++q) {
             int32_t nn, a1, b1;
             double u[3], f, pp;
             if (!data.next() || !data.i32(nn) || !data.f64(u[0]) || !data.f64(u[1]) || !data.f64(u[2]) ||
-                !data.f64(f) || !data.f64(pp) || !data.i32(a1) || !data.i32(b1) || !put(nn, u, &f, &pp))
+                !data.f64(f) || !data.f64(pp) || !data.i32(a1) || !data.i32(b1) || nn > inne ||
+                !put(nn, u, &f, &pp))
+                return CLOWNS_EDECK;
+        }
+        for (int32_t q = 0; q < icc2; ++q) {
+            int32_t nn;
+            double u[3];
+            if (!data.next() || !data.i32(nn) || !data.f64(u[0]) || !data.f64(u[1]) || !data.f64(u[2]) ||
+                !put(nn, u, nullptr, nullptr))
                 return CLOWNS_EDECK;
         }
     }
@@ -336,8 +395,9 @@ static int clowns_deck_read_impl(const char *grid_path, const char *data_path, c
     if (fdata_path && fdata_path[0]) {
         RecordReader fd(fdata_path);
         if (!fd.ok()) return CLOWNS_EDECK;
+        // a0 of every numbered cell, fb0 of the cells 1..inne
         auto put_a = [&](long nn, const double *av, const double *fbv) -> bool {
-            if (nn < 1 || nn > inne) return false;
+            if (nn < 1 || nn > inne2 || (nn > inne && fbv)) return false;
             const long c = cells[nn - 1];
             for (int l = 0; l < 3; ++l) a[l * nc + c] = av[l];
             if (fbv) fb[c] = *fbv;
@@ -345,24 +405,37 @@ static int clowns_deck_read_impl(const char *grid_path, const char *data_path, c
         };
         if (d->sparse) {
             int32_t icc, icc1;
-            if (!fd.next() || !fd.i32(icc) || !fd.i32(icc1) || icc < 0 || icc1 != 0) return CLOWNS_EDECK;
+            if (!fd.next() || !fd.i32(icc) || !fd.i32(icc1) || icc < 0 || icc1 < 0) return CLOWNS_EDECK;
             for (int32_t q = 0; q < icc; ++q) {
                 int32_t nn;
                 double av[3], fbv;
                 if (!fd.next() || !fd.i32(nn) || !fd.f64(av[0]) || !fd.f64(av[1]) || !fd.f64(av[2]) ||
-                    !fd.f64(fbv) || !put_a(nn, av, &fbv))
+                    !fd.f64(fbv) || nn > inne || !put_a(nn, av, &fbv))
+                    return CLOWNS_EDECK;
+            }
+            for (int32_t q = 0; q < icc1; ++q) {
+                int32_t nn;
+                double av[3];
+                if (!fd.next() || !fd.i32(nn) || !fd.f64(av[0]) || !fd.f64(av[1]) || !fd.f64(av[2]) ||
+                    !put_a(nn, av, nullptr))
                     return CLOWNS_EDECK;
             }
         } else {
-            for (long nn = 1; nn <= inne; ++nn) {
+            for (long nn = 1; nn <= inne2; ++nn) {
                 double av[3], fbv;
-                if (!fd.next() || !fd.f64(av[0]) || !fd.f64(av[1]) || !fd.f64(av[2]) || !fd.f64(fbv) ||
-                    !put_a(nn, av, &fbv))
+                if (nn <= inne) {
+                    if (!fd.next() || !fd.f64(av[0]) || !fd.f64(av[1]) || !fd.f64(av[2]) || !fd.f64(fbv) ||
+                        !put_a(nn, av, &fbv))
+                        return CLOWNS_EDECK;
+                } else if (!fd.next() || !fd.f64(av[0]) || !fd.f64(av[1]) || !fd.f64(av[2]) ||
+                           !put_a(nn, av, nullptr)) {
                     return CLOWNS_EDECK;
+                }
             }
         }
     }
     d->inne = (int32_t)inne;
+    d->inne2 = (int32_t)inne2;
     return CLOWNS_OK;
 }
 
@@ -409,12 +482,13 @@ static int clowns_deck_write_grid_impl(const char *path, const clowns_deck_desc 
     for (long i = 0; i < X; ++i)
         for (long j = 0; j < Y; ++j)
             for (long k = 0; k < Z; ++k) {
-                if (!solid[i + X * (j + Y * k)]) continue;
+                const int8_t sv = solid[i + X * (j + Y * k)];
+                if (!sv) continue;
                 w.i32((int32_t)(i + d->is - 1));
                 w.i32((int32_t)(k + d->ks - 1));
                 w.i32((int32_t)(j + d->js - 1));
                 w.i32(-1);
-                w.i32(0);
+                w.i32(file_nfb(sv));
                 w.end();
             }
     return w.ok() ? CLOWNS_OK : CLOWNS_EDECK;
@@ -426,9 +500,14 @@ static int clowns_deck_write_data_impl(const char *path, const clowns_deck_desc 
     if (!path || !d || !nf || !nb || !un || !F || !p) return CLOWNS_EBADARG;
     if (!desc_ok(d)) return CLOWNS_EBADARG;
     const long X = d->nx + 2, Y = d->ny + 2, Z = d->nz + 2, nc = X * Y * Z;
-    std::vector<int32_t> nf32(nc);
-    for (long c = 0; c < nc; ++c) nf32[c] = nf[c];
-    const std::vector<long> cells = numbering(d, nf32);
+    // the numbering from the flags and, for nf = -1 cells, the object codes
+    std::vector<int32_t> nf32(nc), nfb32(nc);
+    for (long c = 0; c < nc; ++c) {
+        nf32[c] = nf[c];
+        nfb32[c] = nf[c] < 0 ? file_nfb(nb[c]) : 0;
+    }
+    long inne = 0;
+    const std::vector<long> cells = numbering(d, nf32, nfb32, &inne);
     RecordWriter w(path, big_endian != 0);
     if (!w.ok()) return CLOWNS_EDECK;
     w.f64(t);  // output_data, data_io.F90:717
@@ -436,14 +515,17 @@ static int clowns_deck_write_data_impl(const char *path, const clowns_deck_desc 
     w.f64(d->nu);
     w.end();
     if (!d->sparse) {
-        // (((NFF(mn(I,k,j))%f,K=ks-1,KE),J=js-1,JE),I=is-1,IE), then %b;
-        // a solid cell reads as the sentinel cell 0 (f = -1, b = 0)
+        // (((NFF(mn(I,k,j))%f,K=ks-1,KE),J=js-1,JE),I=is-1,IE), then %b with
+        // the appended cells' 300 / 200 put back (data_io.F90:719-737); a
+        // cell outside the case reads as the sentinel cell 0 (f = -1, b = 0)
         for (int which = 0; which < 2; ++which) {
             for (long i = 0; i < X; ++i)
                 for (long j = 0; j < Y; ++j)
                     for (long k = 0; k < Z; ++k) {
                         const long c = i + X * (j + Y * k);
-                        w.i32(which == 0 ? (int32_t)nf[c] : (nf[c] < 0 ? 0 : (int32_t)nb[c]));
+                        int32_t v = nf[c];
+                        if (which == 1) v = nf[c] < 0 ? file_nfb(nb[c]) : nb[c];
+                        w.i32(v);
                     }
             w.end();
         }
@@ -452,8 +534,10 @@ static int clowns_deck_write_data_impl(const char *path, const clowns_deck_desc 
             w.f64(un[c]);
             w.f64(un[nc + c]);
             w.f64(un[2 * nc + c]);
-            w.f64(F[c]);
-            w.f64(p[c]);
+            if (q < inne) {
+                w.f64(F[c]);
+                w.f64(p[c]);
+            }
             w.end();
         }
     } else {
@@ -462,10 +546,11 @@ static int clowns_deck_write_data_impl(const char *path, const clowns_deck_desc 
             const double s = un[c] * un[c] + un[nc + c] * un[nc + c] + un[2 * nc + c] * un[2 * nc + c];
             return !(nf[c] == 0 && s == 0.0 && F[c] < 1.0e-13);
         };
-        int32_t icc = 0;
-        for (long c : cells) icc += keep(c) ? 1 : 0;
+        int32_t icc = 0, icc2 = 0;
+        for (long q = 0; q < (long)cells.size(); ++q)
+            if (keep(cells[q])) (q < inne ? icc : icc2) += 1;
         w.i32(icc);
-        w.i32(0);
+        w.i32(icc2);
         w.end();
         for (long q = 0; q < (long)cells.size(); ++q) {
             const long c = cells[q];
@@ -474,10 +559,12 @@ static int clowns_deck_write_data_impl(const char *path, const clowns_deck_desc 
             w.f64(un[c]);
             w.f64(un[nc + c]);
             w.f64(un[2 * nc + c]);
-            w.f64(F[c]);
-            w.f64(p[c]);
-            w.i32(nf[c]);
-            w.i32(nb[c]);
+            if (q < inne) {
+                w.f64(F[c]);
+                w.f64(p[c]);
+                w.i32(nf[c]);
+                w.i32(nf[c] < 0 ? kept_b(nb[c]) : nb[c]);
+            }
             w.end();
         }
     }

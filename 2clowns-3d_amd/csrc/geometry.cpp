@@ -15,8 +15,12 @@
 //
 // Box convention: cells i=0..nx+1, j=0..ny+1, k=0..nz+1 with a solid outer
 // ring, index c = i + X*(j + Y*k), X = nx+2, Y = ny+2 (k-major: the row
-// order of the reference's fln list).  A solid cell reads as the reference's
-// sentinel cell 0: flag -1, F = fb = 0, closed faces.
+// order of the reference's fln list).  solid[c] != 0: a cell of flag -1.
+// solid[c] = 1: a cell outside the case, which reads as the reference's
+// sentinel cell 0 (mn = 0): flag -1, F = fb = 0, closed faces.  Any other
+// value: an object cell of the case (numbered by the case reader,
+// src2LOWNS/data_io.F90:171-199), read with its own F, fb and a, whose nb
+// becomes that value (its object code: 4 a pressure-boundary cell).
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -50,9 +54,9 @@ class Classifier {
     // Every cell touched below is a flow cell (i in 1..n) or its +-1
     // neighbour, so c +- step stays inside the box.
     int flag(long c) const { return solid_[c] ? -1 : nf_[c]; }
-    double fill(long c) const { return solid_[c] ? 0.0 : F_[c]; }
-    double open(long c) const { return solid_[c] ? 0.0 : fb_[c]; }
-    double face(int l, long c) const { return solid_[c] ? 0.0 : a_[l][c]; }
+    double fill(long c) const { return solid_[c] == 1 ? 0.0 : F_[c]; }
+    double open(long c) const { return solid_[c] == 1 ? 0.0 : fb_[c]; }
+    double face(int l, long c) const { return solid_[c] == 1 ? 0.0 : a_[l][c]; }
     double h(int l, int i) const { return dx_[(size_t)l * stride_ + i]; }
     double xlo(int l, int i) const { return xf_[(size_t)l * stride_ + i]; }
 
@@ -77,7 +81,7 @@ class Classifier {
         for (long c = 0; c < ncell_; ++c) {
             if (solid_[c]) {
                 nf_[c] = -1;
-                nb_[c] = 0;
+                nb_[c] = solid_[c] == 1 ? 0 : solid_[c];
                 continue;
             }
             int p[3];

@@ -213,7 +213,10 @@ __global__ void __launch_bounds__(256) k_vof_check(VofDev v, int *__restrict__ c
     vof_pos(v, c, p);
     const int f = v.nf[c], b = v.nb[c];
     const bool ring = p[0] == 0 || p[0] == v.nx + 1 || p[1] == 0 || p[1] == v.ny + 1 || p[2] == 0 || p[2] == v.nz + 1;
-    const bool bad = (ring && f != -1) || f < -1 || f > 2 || (f == 2 && (b == 0 || b < -3 || b > 3));
+    // object cells of the case (flag -1 with a nonzero object code) are not
+    // covered: QFF's object branches (1sub-vof.F90:77-97) are not restated
+    const bool bad = (ring && f != -1) || f < -1 || f > 2 || (f == 2 && (b == 0 || b < -3 || b > 3)) ||
+                     (f == -1 && b != 0);
     if (bad) atomicOr(&cnt[VC_BAD], 1);
 }
 

@@ -169,11 +169,19 @@ int clowns_solver_patterns(clowns_solver *h, int32_t *npat, int32_t *rows);
  * Box convention: cells i=0..nx+1, j=0..ny+1, k=0..nz+1 whose outer ring is
  * solid (closed walls); box index c = i + X*(j + Y*k), X = nx+2, Y = ny+2,
  * which is the reference's k-major row order (src2LOWNS/nfd2015.F90:69-78).
- * Per-cell fields: solid (int8, 1 = wall/object cell, nf = -1), F (VOF
- * fill), fb (cell open ratio), a[3][ncell] (open ratio of the cell's LOW face
- * per axis), un[3][ncell] (velocity on that face); per-axis dx[3][stride]
- * (cell sizes) and xf[3][stride] (low-face coordinates), index = cell index
- * along the axis. */
+ * Per-cell fields: solid (int8: 0 a flow cell; otherwise a cell of flag -1:
+ * 1 a cell outside the case, which reads as the reference's sentinel cell 0
+ * with zero fields; any other value an object cell of the case, numbered by
+ * the case reader (src2LOWNS/data_io.F90:171-199) and read with its own
+ * fields, whose object code it is: for a boundary object with nfb = b
+ * (1..199) b for 2..126 (4: a pressure-boundary cell), -2 for b = 1,
+ * 124 - b for b >= 127; -1 / -128 the object cells appended to the numbering
+ * with nfb 300 / 200; -127 a cell with nfb = -1), F (VOF fill), fb (cell
+ * open ratio), a[3][ncell] (open ratio of the cell's LOW face per axis),
+ * un[3][ncell] (velocity on that face); per-axis dx[3][stride] (cell sizes)
+ * and xf[3][stride] (low-face coordinates), index = cell index along the
+ * axis.  The classification gives a solid cell nf = -1 and nb = its object
+ * code (0 outside the case). */
 typedef struct clowns_grid_desc {
     int32_t nx, ny, nz;
     int32_t stride;     /* >= max(nx,ny,nz) + 2 */
@@ -185,7 +193,8 @@ typedef struct clowns_grid_desc {
  * and the surface position of set_initial_surface_position
  * (1sub-normal-ini.F90:23-40).  Host arrays.  Outputs per cell nf (-1 solid,
  * 0 air, 1 fluid, 2 surface), nb (surface orientation +-1..3), mno (row,
- * 1-based, or 0); per row row2cell (box index) and sp_row (surface position
+ * 1-based, or 0) (nb of a solid cell: its object code, 0 outside the case);
+ * per row row2cell (box index) and sp_row (surface position
  * along the normal for surface rows, else 0); nrows = mnoe; mp = (min dx)^2
  * (1sub-matrix.F90:17).  F is modified as the reference modifies it (air
  * cells clipped to >= 0).  cap = capacity of row2cell/sp_row. */
@@ -256,8 +265,8 @@ int clowns_grad_open_ratios(clowns_solver *h, const clowns_grid_dev *g,
  * resident; each pass computes the RHS and DMAX from d_un
  * (cal_err_of_continuity), stops when DMAX < d0, solves for xe (x0 = 0,
  * stopping rule eps; CLOWNS_MIXED selects mixed precision), updates
- * d_p(cell) -= xe and corrects d_un (estimate_velocity2, :590-651; grids
- * without pressure-boundary objects).  d_un is 3*ncell low-face velocities
+ * d_p(cell) -= xe and corrects d_un (estimate_velocity2, :590-651, with
+ * pressure-boundary cells, object code 4).  d_un is 3*ncell low-face velocities
  * (used instead of g->un), d_p per cell.  More than max_passes solves ->
  * info->ier = 3000 (the reference stops at ik > 1000). */
 typedef struct clowns_pc_info {
@@ -326,16 +335,21 @@ int clowns_set_dx(double *x, int32_t m, double *dx);
  * nf/nfb dense in the data file) or ".xyzn" (list of nf = -1 cells,
  * output_xyzn :1531-1560).  Data file: read_data :130-156, :219-300 /
  * output_data :708-783 (T, g, nu; flags when dense; U, F, P of the numbered
- * cells; dt).  Open-ratio file: read_fdata :304-330 (a0, fb0).  The box model
- * covers cells with nf = -1 and nfb = 0 (walls, solid objects); other cells
- * give CLOWNS_EDECK. */
+ * cells; dt).  Open-ratio file: read_fdata :304-330 (a0, fb0).  Cells with
+ * nf = -1 become solid cells (data_io.F90:168-199): nfb = 0 ones outside the
+ * case (solid value 1); boundary objects (1 <= nfb < 200, numbered with the
+ * flow cells 1..inne) and appended objects (nfb >= 200, numbered
+ * inne+1..inne2: U and a0 only, fb 0) with their object code (the grid
+ * descriptor's convention); nfb = -1 ones, which the reference maps to its
+ * second sentinel cell (flag -1, b = -1), code -127 with zero fields. */
 typedef struct clowns_deck_desc {
     int32_t is, js, ks, ie, je, ke;  /* the grid file's extents */
     int32_t nx, ny, nz;              /* box interior: ie-is, je-js, ke-ks */
     int32_t stride;                  /* max(nx,ny,nz)+2: axis array stride */
     int32_t sparse;                  /* 1: .xyzn grid file, 0: .xyz */
     int32_t big_endian;              /* byte order of the grid file */
-    int32_t inne;                    /* numbered (non-solid) cells */
+    int32_t inne;                    /* flow cells and boundary objects (1..inne) */
+    int32_t inne2;                   /* with the appended object cells */
     int32_t has_dt0;                 /* t0 > 0 and the data file ends with dt0 */
     double t0, g, nu, dt0;           /* the data file's header (with t0 = 0 the
                                         reference takes g, nu from its own
@@ -390,7 +404,10 @@ int clowns_deck_write_data(const char *path, const clowns_deck_desc *d,
  *   d_Fnew   out [ncell]: the new F (may be g->F)
  * info->dt is dt after cal_new_f's reductions (dt/1.1 per retry; the
  * reference also moves t by the same amount).  CLOWNS_EVOF where the
- * reference STOPs.  Synchronous on return. */
+ * reference STOPs.  Grids with object cells of the case (a flag -1 cell
+ * with a nonzero object code) give CLOWNS_EBADARG: QFF's branches for
+ * object donors (1sub-vof.F90:77-97) are not restated.  Synchronous on
+ * return. */
 typedef struct clowns_vof_info {
     int32_t retries;   /* cal_new_f dt reductions */
     int32_t modified;  /* surface cells corrected by quvwf_modify */

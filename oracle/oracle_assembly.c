@@ -18,11 +18,17 @@
  * Grid: cells i=1..nx, j=1..ny, k=1..nz (is=js=ks=1, ie=nx+1, je=ny+1,
  * ke=nz+1) plus the boundary ring i=0, nx+1 (etc.), stored on the box
  * (nx+2)*(ny+2)*(nz+2) with index c = i + X*(j + Y*k), X=nx+2, Y=ny+2.
- * A ring or object cell is "solid" (nf = -1, nfb = 0): the reference gives
- * such cells mn = 0 (src2LOWNS/data_io.F90:168-186), i.e. the sentinel
- * nff(0) = (f=-1, b=0) with fb(0) = 0, a(:,0) = 0 and un(:,0) = 0
- * (data_io.F90:274-277, main_2CLOWNS.F90:73); that is what reading a solid
- * neighbour yields here.  a[l][c] is the open ratio of the low face of c in
+ * solid[c] != 0 marks a cell of flag -1 (nff%f = iob).  solid[c] = 1: a cell
+ * outside the case, which the reference gives mn = 0 (src2LOWNS/
+ * data_io.F90:168-186), i.e. the sentinel nff(0) = (f=-1, b=0) with
+ * fb(0) = 0, a(:,0) = 0 and un(:,0) = 0 (data_io.F90:274-277,
+ * main_2CLOWNS.F90:73); that is what reading such a neighbour yields here.
+ * Any other value is an object cell of the case (numbered by the case
+ * reader, data_io.F90:171-199, so its f, fb, a and un are read as they are)
+ * and is its object code, which becomes its nb: for a boundary object with
+ * nfb = b (1 <= b < 200) b itself when 2 <= b <= 126 (4: a pressure-boundary
+ * cell), -2 when b = 1, 124 - b when b >= 127; -1 and -128 for the object
+ * cells appended to the numbering with nfb 300 and 200 (b = -1 and 0).  a[l][c] is the open ratio of the low face of c in
  * direction l (between c - e_l and c); un[l][c] the velocity on that face.
  * dx[l][i] (i = 0..n_l+1) the cell sizes after set_dx rounding, xf[l][i]
  * (i = 0..n_l+1) the low-face coordinates x(l,i).
@@ -59,26 +65,33 @@ static inline int fflag(const G *g, int i, int j, int k) {
     long c = cid(g, i, j, k);
     return g->solid[c] ? -1 : g->nf[c];
 }
+/* nff(mn(...))%b of a solid cell: its object code (0 outside the case) */
+static inline int bflag(const G *g, int i, int j, int k) {
+    if (!inside(g, i, j, k)) return 0;
+    long c = cid(g, i, j, k);
+    return g->solid[c] ? (g->solid[c] == 1 ? 0 : g->solid[c]) : g->nb[c];
+}
+/* f, fb, a, un of mn(...): a cell outside the case reads the sentinel's 0 */
 static inline double fval(const G *g, int i, int j, int k) {
     if (!inside(g, i, j, k)) return 0.0;
     long c = cid(g, i, j, k);
-    return g->solid[c] ? 0.0 : g->F[c];
+    return g->solid[c] == 1 ? 0.0 : g->F[c];
 }
 static inline double fbval(const G *g, int i, int j, int k) {
     if (!inside(g, i, j, k)) return 0.0;
     long c = cid(g, i, j, k);
-    return g->solid[c] ? 0.0 : g->fb[c];
+    return g->solid[c] == 1 ? 0.0 : g->fb[c];
 }
 /* a(l, mn(i,j,k)) */
 static inline double aval(const G *g, int l, int i, int j, int k) {
     if (!inside(g, i, j, k)) return 0.0;
     long c = cid(g, i, j, k);
-    return g->solid[c] ? 0.0 : g->a[l][c];
+    return g->solid[c] == 1 ? 0.0 : g->a[l][c];
 }
 static inline double unval(const G *g, int l, int i, int j, int k) {
     if (!inside(g, i, j, k)) return 0.0;
     long c = cid(g, i, j, k);
-    return g->solid[c] ? 0.0 : g->un[l][c];
+    return g->solid[c] == 1 ? 0.0 : g->un[l][c];
 }
 
 static const int E[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}; /* lf (main_2CLOWNS.F90:33) */
@@ -112,7 +125,7 @@ static void classify(G *g) {
         for (int j = 0; j < g->Y; ++j)
             for (int i = 0; i < g->X; ++i) {
                 long c = cid(g, i, j, k);
-                if (g->solid[c]) { g->nf[c] = -1; g->nb[c] = 0; continue; }
+                if (g->solid[c]) { g->nf[c] = -1; g->nb[c] = g->solid[c] == 1 ? 0 : g->solid[c]; continue; }
                 double f = g->F[c];
                 if (f <= VSMALL || (f * g->fb[c] < 1.0e-3 && fflag(g, i, j, k - 1) == -1 &&
                                     fval(g, i, j, k + 1) == 0.0)) {
@@ -336,8 +349,9 @@ int oracle_build_system(int nx, int ny, int nz, const int8_t *solid, const doubl
                 int ni = i + ii * E[l][0], nj = j + ii * E[l][1], nk = k + ii * E[l][2];
                 int pi = i + (ii > 0) * E[l][0], pj = j + (ii > 0) * E[l][1], pk = k + (ii > 0) * E[l][2];
                 int fnb = fflag(&g, ni, nj, nk);
-                /* nff(np)%b of a solid neighbour is 0 (no pressure-boundary cells here) */
-                if (fnb == -1 || aval(&g, l, pi, pj, pk) == 0.0 || fbval(&g, ni, nj, nk) == 0.0) {
+                /* an object neighbour closes the slot unless it is a pressure-boundary cell */
+                if ((fnb == -1 && bflag(&g, ni, nj, nk) != 4) || aval(&g, l, pi, pj, pk) == 0.0 ||
+                    fbval(&g, ni, nj, nk) == 0.0) {
                     ll_c[slot] = -1;
                     alu_c[slot] = 0.0;
                 } else {
@@ -416,9 +430,15 @@ int oracle_build_system(int nx, int ny, int nz, const int8_t *solid, const doubl
  *   p(mno2mn(r)) -= xe(r)            src2LOWNS/1sub-momentum.F90:53-58
  *   estimate_velocity2               src2LOWNS/1sub-momentum.F90:590-651
  *     un(l,c) = momentum_d(un, -xe(nno(l)), -xe(nnn), dxc, dt)
- *             = un + dt*(pm - pc)/dxc, dxc = (dx(l,i) + dx(l,i-1))/2,
- *     for every non-air, non-solid cell c and axis l with a(l,c) != 0 and a
- *     low neighbour flagged >= 1; |un| < 1e-15 -> 0.  (lib/src/math_func.F90:251-255)
+ *             = un + dt*(pm - pc)/dxc, dxc = (dx(l,i) + dx(l,i-1))/2
+ *     (lib/src/math_func.F90:251-255), over the cells of the case's main
+ *     numbering that are neither air nor objects other than pressure-boundary
+ *     cells:
+ *       a pressure-boundary cell (nb = 4): on every axis whose low neighbour
+ *         is flagged >= 1 (:611-623);
+ *       any other cell: on every axis with a(l,c) != 0 whose low neighbour is
+ *         flagged >= 1 (then |un| < 1e-15 -> 0) or is a pressure-boundary
+ *         cell (:624-646).
  *   The reference reads xe(0) for an unnumbered cell; PCGPME never writes
  *   it and the caller allocates it without a value, so here it is 0.
  * and the RHS + DMAX of the next pass (1sub-momentum.F90:315-359) for the
@@ -436,20 +456,28 @@ void oracle_velocity_correction(int nx, int ny, int nz, const int8_t *solid, con
         for (int j = 0; j < Y; ++j)
             for (int i = 0; i < X; ++i) {
                 long c = i + (long)X * (j + (long)Y * k);
-                if (solid[c] || nf[c] == 0) continue;
+                if (solid[c] == 1 || (!solid[c] && nf[c] == 0)) continue;
+                const int pbc = solid[c] == 4;  /* a pressure-boundary cell */
+                if (solid[c] && !pbc) continue;
                 int idx[3] = {i, j, k};
                 double xec = -(mno[c] ? xe[mno[c] - 1] : 0.0);
                 for (int l = 0; l < 3; ++l) {
-                    if (a[(size_t)l * ncell + c] == 0.0) continue;
-                    long m = c - step[l];
-                    int fm = solid[m] ? -1 : nf[m];
-                    if (fm >= 1) {
-                        double dxc = (dx[(size_t)l * dxstride + idx[l]] + dx[(size_t)l * dxstride + idx[l] - 1]) * 0.5;
-                        double pm = -(mno[m] ? xe[mno[m] - 1] : 0.0);
-                        double v = un[(size_t)l * ncell + c] + dt * (pm - xec) / dxc;
-                        if (fabs(v) < 1e-15) v = 0.0;
-                        un[(size_t)l * ncell + c] = v;
+                    if (!pbc && a[(size_t)l * ncell + c] == 0.0) continue;
+                    /* the low neighbour; below the box it is the sentinel (f = -1, b = 0) */
+                    int fm = -1, bm = 0;
+                    long m = -1;
+                    if (idx[l] >= 1) {
+                        m = c - step[l];
+                        fm = solid[m] ? -1 : nf[m];
+                        bm = solid[m] == 1 ? 0 : solid[m];
                     }
+                    const int corr = fm >= 1 || (!pbc && fm == -1 && bm == 4);
+                    if (!corr) continue;
+                    double dxc = (dx[(size_t)l * dxstride + idx[l]] + dx[(size_t)l * dxstride + idx[l] - 1]) * 0.5;
+                    double pm = -(m >= 0 && mno[m] ? xe[mno[m] - 1] : 0.0);
+                    double v = un[(size_t)l * ncell + c] + dt * (pm - xec) / dxc;
+                    if (!pbc && fm >= 1 && fabs(v) < 1e-15) v = 0.0;
+                    un[(size_t)l * ncell + c] = v;
                 }
             }
 }
@@ -469,10 +497,11 @@ double oracle_rhs(int nx, int ny, int nz, const int8_t *solid, const int8_t *nf,
         double rn;
         if (fb[c] != 0.0) {
             long cx = c + 1, cy = c + X, cz = c + (long)X * Y;
-            double axp = solid[cx] ? 0.0 : a[cx], ayp = solid[cy] ? 0.0 : a[ncell + cy];
-            double azp = solid[cz] ? 0.0 : a[2 * ncell + cz];
-            double up = solid[cx] ? 0.0 : un[cx], vp = solid[cy] ? 0.0 : un[ncell + cy];
-            double wp = solid[cz] ? 0.0 : un[2 * ncell + cz];
+            /* a cell outside the case reads the sentinel's 0 */
+            double axp = solid[cx] == 1 ? 0.0 : a[cx], ayp = solid[cy] == 1 ? 0.0 : a[ncell + cy];
+            double azp = solid[cz] == 1 ? 0.0 : a[2 * ncell + cz];
+            double up = solid[cx] == 1 ? 0.0 : un[cx], vp = solid[cy] == 1 ? 0.0 : un[ncell + cy];
+            double wp = solid[cz] == 1 ? 0.0 : un[2 * ncell + cz];
             rn = ((up * axp - un[c] * a[c]) / dx[i] + (vp * ayp - un[ncell + c] * a[ncell + c]) / dx[dxstride + j] +
                   (wp * azp - un[2 * ncell + c] * a[2 * ncell + c]) / dx[2 * dxstride + k]) / fb[c];
         } else {

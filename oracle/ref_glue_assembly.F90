@@ -10,9 +10,12 @@
 !   cal_err_of_continuity, ik = 0 (src2LOWNS/1sub-momentum.F90:235)
 !   cal_matrix_element            (src2LOWNS/1sub-momentum.F90:465)
 ! and copies the resulting flags, row numbering and PCGPME inputs
-! (alu, llu, de, b) back.  The cell numbering nn restates the loop of
-! read_data (src2LOWNS/data_io.F90:139-158) for grids whose solid cells
-! (walls, objects) all have nfb = 0, i.e. mn = 0.  dx and x are allocated
+! (alu, llu, de, b) back.  The cell numbering nn restates the loops of
+! read_data (src2LOWNS/data_io.F90:168-199): solid value 0 (flow cells) and
+! the boundary-object codes are numbered in one (j, k, i) sweep, the
+! appended object codes (-1, -128) after it; solid value 1 (outside the
+! case) gets mn = 0.  Object cells keep f = -1 and b from their code
+! (b_of below).  dx and x are allocated
 ! with bounds that cover every index the routines touch (the reference's
 ! own allocation, 0mod-arrays.F90:205-206, starts at 1 while set_dx writes
 ! from ls-2).
@@ -95,21 +98,40 @@ subroutine ref_build_system(nx, ny, nz, solid, fvv, fbv, av, unv, dxv, xfv, stri
     ! cell numbering nn (data_io.F90:139-158): j, k, i order, flow cells only
     allocate(mn(is-2:ie+2, ks-2:ke+2, js-2:je+2)) ; mn = 0
     allocate(in(ncell), jn(ncell), kn(ncell))
+    ! numbering as read_data's (data_io.F90:168-199): flow cells and boundary
+    ! objects in one (j, k, i) sweep, then the appended object cells
     nn = 0
     do j = js-1, je
         do k = ks-1, ke
             do i = is-1, ie
                 c = i + XX*(j + YY*k)
-                if (solid(c) == 0) then
+                if (solid(c) == 0 .or. in_main(solid(c))) then
                     nn = nn + 1 ; mn(i,k,j) = nn ; in(nn) = i ; jn(nn) = j ; kn(nn) = k
                 endif
             enddo
         enddo
     enddo
-    inns = 1 ; inne = nn ; inne2 = nn
+    inns = 1 ; inne = nn
+    do j = js-1, je
+        do k = ks-1, ke
+            do i = is-1, ie
+                c = i + XX*(j + YY*k)
+                if (solid(c) == -1 .or. solid(c) == -128) then
+                    nn = nn + 1 ; mn(i,k,j) = nn ; in(nn) = i ; jn(nn) = j ; kn(nn) = k
+                endif
+            enddo
+        enddo
+    enddo
+    inne2 = nn
     allocate(nff(0:inne2+1))
     nff(:)%f = 0 ; nff(:)%b = 0 ; nff(:)%fp = 0 ; nff(:)%bp = 0
     nff(0)%f = -1 ; nff(0)%b = 0 ; nff(inne2+1)%f = -1 ; nff(inne2+1)%b = -1
+    do nn = 1, inne2
+        c = in(nn) + XX*(jn(nn) + YY*kn(nn))
+        if (solid(c) /= 0) then
+            nff(nn)%f = -1 ; nff(nn)%b = b_of(solid(c))
+        endif
+    enddo
     call alloc_uvwfp(inne2)
     call alloc_afb(inne2, inne2)
     call alloc_a0fb0(inne2, inne2)
@@ -141,7 +163,11 @@ subroutine ref_build_system(nx, ny, nz, solid, fvv, fbv, av, unv, dxv, xfv, stri
     do nn = 1, inne2
         c = in(nn) + XX*(jn(nn) + YY*kn(nn))
         nf_out(c) = int(nff(nn)%f, c_int8_t)
-        nb_out(c) = int(nff(nn)%b, c_int8_t)
+        if (nff(nn)%f == -1) then
+            nb_out(c) = code_of(nff(nn)%b, nn > inne)
+        else
+            nb_out(c) = int(nff(nn)%b, c_int8_t)
+        endif
     enddo
     do r = 1, mnoe
         nn = mno2mn(r)
@@ -151,6 +177,46 @@ subroutine ref_build_system(nx, ny, nz, solid, fvv, fbv, av, unv, dxv, xfv, stri
         Dd(r) = de(r)
         Bb(r) = b(r)
     enddo
+contains
+    ! object codes (the box grids' solid values other than 0 and 1): a
+    ! boundary object with nfb = b (1 <= b < 200) has code b for 2..126, -2
+    ! for b = 1 and 124 - b (-3..-75) for b >= 127; the cells appended with
+    ! nfb 300 / 200 have -1 / -128 (their b becomes -1 / 0)
+    logical function in_main(s)
+        integer(c_int8_t), intent(in) :: s
+        in_main = s /= 0 .and. s /= 1 .and. s /= -1 .and. s /= -128
+    end function
+    integer function b_of(s)
+        integer(c_int8_t), intent(in) :: s
+        if (s == -128) then
+            b_of = 0
+        elseif (s == -1) then
+            b_of = -1
+        elseif (s == -2) then
+            b_of = 1
+        elseif (s < 0) then
+            b_of = 124 - s
+        else
+            b_of = s
+        endif
+    end function
+    integer(c_int8_t) function code_of(b, appended)
+        integer, intent(in) :: b
+        logical, intent(in) :: appended
+        if (appended) then
+            if (b == -1) then
+                code_of = -1
+            else
+                code_of = -128
+            endif
+        elseif (b == 1) then
+            code_of = -2
+        elseif (b >= 127) then
+            code_of = int(124 - b, c_int8_t)
+        else
+            code_of = int(b, c_int8_t)
+        endif
+    end function
 end subroutine ref_build_system
 
 ! One pressure-correction pass of the reference after ref_build_system on

@@ -22,8 +22,23 @@ def _rec(f, fmt, vals, be):
     f.write(m + payload + m)
 
 
+# object codes of nf = -1 cells (the box model's solid values), from nfb
+def object_code(nfb):
+    if nfb == -1:
+        return -127
+    if nfb <= 0:
+        return 1
+    if nfb >= 200:
+        return -1 if nfb == 300 else -128
+    if nfb == 1:
+        return -2
+    if nfb >= 127:
+        return 124 - nfb
+    return nfb
+
+
 def make_case(tmp, nx, ny, nz, seed, sparse, be=False, lo=(3, 3, 3), t0=12.5, with_dt0=True, obstacle=True,
-              object_cell=False, fdata=True, sparse_frac=0.6):
+              objects=False, fdata=True, sparse_frac=0.6):
     rng = np.random.default_rng(seed)
     X, Y, Z = nx + 2, ny + 2, nz + 2
     nc = X * Y * Z
@@ -38,13 +53,25 @@ def make_case(tmp, nx, ny, nz, seed, sparse, be=False, lo=(3, 3, 3), t0=12.5, wi
     solid = solid.ravel()
     nf = np.where(solid == 1, -1, rng.integers(0, 3, nc)).astype(np.int32)
     nfb = np.where(solid == 1, 0, rng.integers(0, 2, nc)).astype(np.int32)
-    if object_cell:  # a boundary object the box model does not cover
-        c = 1 + X * (1 + Y * 1)
-        nf[c], nfb[c] = -1, 7
-    # numbering: j outer, k, i inner over nf >= 0 (data_io.F90:160-176)
-    cells = [i + X * (j + Y * k) for j in range(Y) for k in range(Z) for i in range(X) if nf[i + X * (j + Y * k)] >= 0]
-    cells = np.array(cells, np.int64)
-    inne = len(cells)
+    if objects:
+        # boundary objects (1 <= nfb < 200), appended objects (200, 300) and
+        # nfb = -1 cells (data_io.F90:168-199), inside and on the walls
+        free = np.nonzero(solid == 0)[0]
+        kinds = [4, 5, 1, 150, 7, 200, 300, 300, -1, 4, 200]
+        for c, b in zip(rng.choice(free, len(kinds), replace=False), kinds):
+            nf[c], nfb[c] = -1, b
+        wall = np.nonzero(solid == 1)[0]
+        for c, b in zip(rng.choice(wall, 4, replace=False), (4, 5, 300, 1)):
+            nfb[c] = b
+    solid = np.array([0 if nf[c] >= 0 else object_code(int(nfb[c])) for c in range(nc)], np.int8)
+    # numbering (data_io.F90:168-199): j outer, k, i inner over flow cells and
+    # boundary objects, then the same sweep over appended objects
+    sweep = [i + X * (j + Y * k) for j in range(Y) for k in range(Z) for i in range(X)]
+    main = [c for c in sweep if nf[c] >= 0 or 1 <= nfb[c] < 200]
+    app = [c for c in sweep if nf[c] < 0 and nfb[c] >= 200]
+    cells = np.array(main + app, np.int64)
+    inne = len(main)
+    inne2 = len(cells)
     # coordinates X(l, ls-2 .. le+1): n+4 faces, cell sizes on a 0.01 grid
     coords = []
     for n, base in zip((nx, ny, nz), (-3.0, 10.0, -1.5)):
@@ -53,32 +80,45 @@ def make_case(tmp, nx, ny, nz, seed, sparse, be=False, lo=(3, 3, 3), t0=12.5, wi
     U = np.zeros((3, nc))
     F = np.zeros(nc)
     P = np.zeros(nc)
-    A = np.where(solid == 1, 0.0, 1.0)[None].repeat(3, 0)
-    FB = np.where(solid == 1, 0.0, 1.0)
+    # without an open-ratio file: cells 1..inne open, appended ones closed
+    opened = np.zeros(nc, bool)
+    opened[main] = True
+    A = np.where(opened, 1.0, 0.0)[None].repeat(3, 0)
+    FB = np.where(opened, 1.0, 0.0)
     nf_keep = nf.copy()
     nb_keep = nfb.copy()
+    for c in app:  # read_data resets the appended cells' nfb (data_io.F90:190-194)
+        nb_keep[c] = -1 if nfb[c] == 300 else 0
     listed = np.ones(inne, bool)
     if sparse:
         listed = rng.random(inne) < sparse_frac
-    for q, c in enumerate(cells):
+    for q, c in enumerate(main):
         if listed[q]:
             U[:, c] = rng.standard_normal(3)
             F[c] = rng.random()
             P[c] = rng.standard_normal() * 100
+    for c in app:
+        U[:, c] = rng.standard_normal(3)
     if sparse:
-        # the data records carry the flags of the cells they list; the others
-        # keep the grid's nf = 0, nfb = 0
-        for q, c in enumerate(cells):
+        # the data records carry the flags of the cells they list (an object
+        # keeps its own); unlisted flow cells keep the grid's nf = 0, nfb = 0
+        for q, c in enumerate(main):
+            if nf[c] < 0:
+                continue
             if listed[q]:
                 nf_keep[c] = int(rng.integers(0, 3))
                 nb_keep[c] = int(rng.integers(0, 2))
             else:
                 nf_keep[c] = nb_keep[c] = 0
     a_listed = rng.random(inne) < 0.5
-    for q, c in enumerate(cells):
+    for q, c in enumerate(main):
         if not sparse or a_listed[q]:
             A[:, c] = rng.integers(0, 101, 3) / 100.0
             FB[c] = rng.integers(1, 101) / 100.0
+    app_listed = rng.random(len(app)) < 0.7
+    for q, c in enumerate(app):
+        if not sparse or app_listed[q]:
+            A[:, c] = rng.integers(0, 101, 3) / 100.0
     g, nu, dt0 = 9.8, 1.0e-6, 0.0025
     ext = (is_, js, ks, ie, je, ke)
     gname = tmp / ("case.xyzn" if sparse else "case.xyz")
@@ -101,30 +141,39 @@ def make_case(tmp, nx, ny, nz, seed, sparse, be=False, lo=(3, 3, 3), t0=12.5, wi
             order = [i + X * (j + Y * k) for i in range(X) for j in range(Y) for k in range(Z)]
             _rec(f, f"{nc}i", nf[order], be)
             _rec(f, f"{nc}i", nfb[order], be)
-            for c in cells:
+            for c in main:
                 _rec(f, "5d", [*U[:, c], F[c], P[c]], be)
+            for c in app:
+                _rec(f, "3d", [*U[:, c]], be)
         else:
             idx = np.nonzero(listed)[0]
-            _rec(f, "2i", [len(idx), 0], be)
+            _rec(f, "2i", [len(idx), len(app)], be)
             for q in idx:
-                c = cells[q]
+                c = main[q]
                 _rec(f, "i5d2i", [q + 1, *U[:, c], F[c], P[c], nf_keep[c], nb_keep[c]], be)
+            for q, c in enumerate(app):
+                _rec(f, "i3d", [inne + q + 1, *U[:, c]], be)
         if with_dt0:
             _rec(f, "d", [dt0], be)
     if fdata:
         with open(fname, "wb") as f:
             if not sparse:
-                for c in cells:
+                for c in main:
                     _rec(f, "4d", [*A[:, c], FB[c]], be)
+                for c in app:
+                    _rec(f, "3d", [*A[:, c]], be)
             else:
                 idx = np.nonzero(a_listed)[0]
-                _rec(f, "2i", [len(idx), 0], be)
+                aidx = np.nonzero(app_listed)[0]
+                _rec(f, "2i", [len(idx), len(aidx)], be)
                 for q in idx:
-                    c = cells[q]
+                    c = main[q]
                     _rec(f, "i4d", [q + 1, *A[:, c], FB[c]], be)
+                for q in aidx:
+                    _rec(f, "i3d", [inne + q + 1, *A[:, app[q]]], be)
     return dict(grid=gname, data=dname, fdata=fname if fdata else None, ext=ext, nx=nx, ny=ny, nz=nz,
                 X=X, Y=Y, Z=Z, nc=nc, solid=solid, U=U, F=F, P=P, A=A, FB=FB, nf=nf_keep, nb=nb_keep,
-                cells=cells, inne=inne, coords=coords, t0=t0, g=g, nu=nu, dt0=dt0)
+                cells=cells, inne=inne, inne2=inne2, coords=coords, t0=t0, g=g, nu=nu, dt0=dt0)
 
 
 def write_grid_case(tmp, g, nf, nb, p, sparse=False, be=False, lo=(3, 3, 3), t0=1.0, g_acc=9.8, nu=1.0e-6):

@@ -11,8 +11,11 @@ Bar, all bit-exact:
   the same values into them;
 - the writer's file is byte-identical to output_data's on the same state,
   and reading it back returns the same fields;
-- cells outside the box model (an nf = -1 cell with nfb != 0) and truncated
-  files give CLOWNS_EDECK.
+- cells with nf = -1 follow read_data's numbering (data_io.F90:168-199):
+  boundary objects numbered with the flow cells, appended objects after
+  them (U and a0 only), nfb = 0 and -1 cells not numbered; the reader gives
+  each its object code, and the writer puts nfb back as output_data does;
+- truncated files give CLOWNS_EDECK.
 The flang runtime of the reference build reads and writes the host byte
 order (no main program sets its conversion), so the comparisons with it
 use little-endian files; the big-endian form, the one the reference's own
@@ -57,11 +60,11 @@ def ref_read(case, data=None):
     flags = np.zeros((nmax, 2), np.int32)
     lib.ref_read_deck(gp, len(gp), dp, len(dp), fp, len(fp), ptr(ext), ptr(counts), ptr(hdr), ptr(xo), S, nmax,
                       ptr(cells), ptr(uo), ptr(fo), ptr(po), ptr(ao), ptr(fbo), ptr(flags))
-    n = int(counts[0])
+    n = int(counts[1])  # every numbered cell: 1..inne and the appended ones
     is_, js, ks = ext[:3]
     X, Y = case["X"], case["Y"]
     i, j, k = cells[:n, 0] - (is_ - 1), cells[:n, 1] - (js - 1), cells[:n, 2] - (ks - 1)
-    return dict(ext=ext, inne=n, inne2=int(counts[1]), hdr=hdr, xo=xo, box=i + X * (j + Y * k),
+    return dict(ext=ext, inne=int(counts[0]), inne2=n, hdr=hdr, xo=xo, box=i + X * (j + Y * k),
                 U=uo[:n].T, F=fo[:n], P=po[:n], A=ao[:n].T, FB=fbo[:n], flags=flags[:n])
 
 
@@ -81,7 +84,7 @@ def ours(case, data=None, fdata="fixture"):
 def _check_fields(case, g, p, d):
     assert (d.nx, d.ny, d.nz) == (case["nx"], case["ny"], case["nz"])
     assert (d.is_, d.js, d.ks, d.ie, d.je, d.ke) == case["ext"]
-    assert d.inne == case["inne"]
+    assert d.inne == case["inne"] and d.inne2 == case["inne2"]
     assert (d.t0, d.g, d.nu) == (case["t0"], case["g"], case["nu"])
     assert np.array_equal(g.solid.ravel(), case["solid"])
     assert np.array_equal(g.un.reshape(3, -1), case["U"])
@@ -119,13 +122,16 @@ def test_reader_without_fdata_defaults(tmp_path, name, sparse):
 
 
 @pytest.mark.parametrize("name,sparse", CASES)
-@pytest.mark.parametrize("shape,lo", [((7, 5, 4), (3, 3, 3)), ((12, 9, 6), (5, 3, 4))])
-def test_reader_matches_reference(tmp_path, name, sparse, shape, lo):
-    case = make_case(tmp_path, *shape, seed=sum(shape) + sparse, sparse=sparse, lo=lo)
+@pytest.mark.parametrize("shape,lo,objects", [((7, 5, 4), (3, 3, 3), False), ((12, 9, 6), (5, 3, 4), False),
+                                              ((9, 7, 5), (3, 2, 4), True)])
+def test_reader_matches_reference(tmp_path, name, sparse, shape, lo, objects):
+    case = make_case(tmp_path, *shape, seed=sum(shape) + sparse, sparse=sparse, lo=lo, objects=objects)
+    _check_fields(case, *ours(case))
     r = ref_read(case)
     g, p, d = ours(case)
     assert tuple(r["ext"]) == case["ext"]
-    assert r["inne"] == r["inne2"] == d.inne
+    assert r["inne"] == d.inne and r["inne2"] == d.inne2
+    assert (d.inne2 > d.inne) == objects
     assert np.array_equal(r["box"], case["cells"])  # same numbering, same order
     assert tuple(r["hdr"]) == (d.t0, d.g, d.nu, d.dt0)
     c = r["box"]
@@ -138,9 +144,13 @@ def test_reader_matches_reference(tmp_path, name, sparse, shape, lo):
     assert np.array_equal(r["flags"][:, 1], case["nb"][c])
     for l, n in enumerate(shape):
         assert np.array_equal(r["xo"][l, 1:n + 3], g.xf[l, :n + 2])
-    # every cell the reference does not number is solid in the box model
+    # every cell the reference does not number is outside the case in the box
+    # model (1, or -127 for its second sentinel); numbered object cells carry
+    # their codes
     rest = np.setdiff1d(np.arange(case["nc"]), c)
-    assert np.all(g.solid.ravel()[rest] == 1) and np.all(g.solid.ravel()[c] == 0)
+    sol = g.solid.ravel()
+    assert np.all((sol[rest] == 1) | (sol[rest] == -127))
+    assert np.all(sol[c] != 1) and np.all((sol[c] == 0) == (r["flags"][:, 0] >= 0))
 
 
 def test_dense_restart_time_zero(tmp_path):
@@ -155,13 +165,21 @@ def test_dense_restart_time_zero(tmp_path):
     assert np.array_equal(r["P"], p.ravel()[c]) and np.array_equal(r["U"], g.un.reshape(3, -1)[:, c])
 
 
+def writer_flags(case):
+    """nf/nb as the classification gives them: the reference's flags, with
+    each nf = -1 cell's object code as its nb."""
+    nf = np.where(case["solid"] != 0, -1, case["nf"]).astype(np.int8)
+    nb = np.where(case["solid"] != 0, np.where(case["solid"] == 1, 0, case["solid"]), case["nb"]).astype(np.int8)
+    return nf, nb
+
+
 @pytest.mark.parametrize("name,sparse", CASES)
-def test_writer_byte_identical_to_reference(tmp_path, name, sparse):
+@pytest.mark.parametrize("objects", [False, True])
+def test_writer_byte_identical_to_reference(tmp_path, name, sparse, objects):
     from clowns3d import decks
-    case = make_case(tmp_path, 8, 6, 5, seed=21 + sparse, sparse=sparse)
+    case = make_case(tmp_path, 8, 6, 5, seed=21 + sparse, sparse=sparse, objects=objects)
     g, p, d = ours(case)
-    nf = case["nf"].astype(np.int8)
-    nb = case["nb"].astype(np.int8)
+    nf, nb = writer_flags(case)
     rng = np.random.default_rng(4)
     # a new (solved) pressure on the open cells
     p2 = p.ravel() + rng.standard_normal(case["nc"]) * np.where(case["solid"] == 0, 1.0, 0.0)
@@ -174,7 +192,7 @@ def test_writer_byte_identical_to_reference(tmp_path, name, sparse):
     r = ref_read(case, data=mine)
     c = r["box"]
     U = g.un.reshape(3, -1)
-    expect = p2[c]
+    expect = np.where(np.arange(len(c)) < d.inne, p2[c], 0.0)  # appended cells carry no P
     if sparse:  # cells without state are not listed and read back as zero
         dropped = (nf[c] == 0) & ((U[:, c] ** 2).sum(0) == 0) & (g.F.ravel()[c] < 1e-13)
         assert dropped.any() and not dropped.all()
@@ -229,16 +247,23 @@ def test_sparse_writer_drops_stateless_cells(tmp_path):
     assert p2.ravel()[c[1]] == 0.0 and p2.ravel()[c[4]] == 0.0
 
 
-def test_object_cells_rejected(tmp_path):
-    from clowns3d import decks
-    from clowns3d._lib import CLOWNS_EDECK, ClownsError
+def test_object_codes(tmp_path):
+    """Object cells read into their codes: boundary objects (nfb 4, 5, 7 kept,
+    1 -> -2, 150 -> -26), appended cells (300 -> -1, 200 -> -128) with U and a
+    but no F, P or fb, nfb = -1 cells -> -127 with zero fields."""
     for sparse in (False, True):
         sub = tmp_path / str(sparse)
         sub.mkdir()
-        case = make_case(sub, 5, 4, 3, seed=9, sparse=sparse, object_cell=True)
-        with pytest.raises(ClownsError) as e:
-            decks.read_case(case["grid"], case["data"], case["fdata"])
-        assert e.value.code == CLOWNS_EDECK
+        case = make_case(sub, 6, 5, 4, seed=9, sparse=sparse, objects=True)
+        g, p, d = ours(case)
+        sol = g.solid.ravel()
+        assert set(np.unique(sol).tolist()) >= {0, 1, 4, 5, 7, -2, -26, -1, -128, -127}
+        app = case["cells"][d.inne:]
+        assert len(app) > 0 and set(np.unique(sol[app]).tolist()) <= {-1, -128}
+        assert np.all(g.F.ravel()[app] == 0) and np.all(p.ravel()[app] == 0) and np.all(g.fb.ravel()[app] == 0)
+        assert np.abs(g.un.reshape(3, -1)[:, app]).sum() > 0
+        s127 = sol == -127
+        assert s127.any() and np.all(g.fb.ravel()[s127] == 0) and np.all(g.a.reshape(3, -1)[:, s127] == 0)
 
 
 def test_truncated_and_missing_files(tmp_path):

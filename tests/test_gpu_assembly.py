@@ -46,7 +46,7 @@ def test_device_assembly_config1():
     assert np.array_equal(B.cpu().numpy(), o["B"][:N])
 
 
-@pytest.mark.parametrize("name", ["wavy32x24x16", "bathy28x24x16"])
+@pytest.mark.parametrize("name", ["wavy32x24x16", "bathy28x24x16", "objects40x24x16"])
 def test_fused_assembly_and_solve(name):
     g = GRIDS[name]()
     o = build_oracle(g)

@@ -116,6 +116,21 @@ def test_vof_rejects_open_ring():
     sol.close()
 
 
+def test_vof_rejects_object_cells():
+    """Object cells of the case (boundary objects, appended cells) are not
+    covered by the VOF update: the entry refuses the grid before any flux
+    kernel runs."""
+    from clowns3d import DeviceGrid, Solver, configs
+    from clowns3d._lib import CLOWNS_EBADARG, ClownsError
+    g = configs.config_objects(20, 12, 10, seed=3)
+    sol = Solver(0)
+    dg = DeviceGrid(g)
+    with pytest.raises(ClownsError) as e:
+        dg.vof(sol, dg.t["F"].clone(), mode=1, dt=g.dt)
+    assert e.value.code == CLOWNS_EBADARG
+    sol.close()
+
+
 @pytest.mark.parametrize("name,scenario", [("bathy28x24x16", "stress"), ("wavy32x24x16", "step"),
                                            ("solitary40x8x12", "stress")])
 def test_vof_plic_volumes_match_oracle(name, scenario):

@@ -67,6 +67,7 @@ GRIDS = {
     "wavy32x24x16": lambda: configs.config_wavy_box(32, 24, 16, (8, 20, 8, 16, 0, 6), seed=8),
     "bathy28x24x16": lambda: configs.config_bathymetry(28, 24, 16, nbumps=6, seed=9, width=(2, 5)),
     "solitary40x8x12": lambda: configs.config_solitary(40, 8, 12, t=0.7, seed=10),
+    "objects40x24x16": lambda: configs.config_objects(40, 24, 16, seed=11),
 }
 
 
@@ -153,10 +154,12 @@ def oracle_correction_pass(g, o, xe, p, un):
     return oracle_rhs_dmax(g, o, un)
 
 
-@pytest.mark.parametrize("name", ["wavy20x16x12", "bathy28x24x16"])
+@pytest.mark.parametrize("name", ["wavy20x16x12", "bathy28x24x16", "objects40x24x16"])
 def test_correction_pass_matches_reference(name):
     """p -= xe, estimate_velocity2 (1sub-momentum.F90:590-651) and the RHS of
-    the next pass, bit for bit against the reference routines."""
+    the next pass, bit for bit against the reference routines; on the grid
+    with object cells, pressure-boundary cells' own faces and the faces of
+    flow cells next to them are corrected too."""
     g = GRIDS[name]()
     r = build_ref(g)
     o = build_oracle(g)
@@ -175,6 +178,27 @@ def test_correction_pass_matches_reference(name):
     sol = A["solid"] == 0
     assert np.array_equal(p_o[sol], p_r[sol])
     un_r = un_r.reshape(3, -1)
-    assert np.array_equal(un_o[:, sol], un_r[:, sol])
+    case = A["solid"] != 1  # every cell the case numbers, object cells included
+    assert np.array_equal(un_o[:, case], un_r[:, case])
+    moved = (un_o != A["un"]).any(axis=0)
+    if (A["solid"] == 4).any():
+        assert moved[A["solid"] == 4].any()  # pressure-boundary cells' faces were corrected
     assert np.array_equal(Bo, Br)
     assert dmo == dm.value
+
+
+def test_object_cells_change_the_system():
+    """The object-cell grid pins what it should: read as cells outside the
+    case (the sentinel), its object cells would give another right-hand side
+    (their own face fluxes drop out), so the bit-exact match above checks
+    the object handling itself."""
+    g = GRIDS["objects40x24x16"]()
+    r = build_ref(g)
+    g2 = GRIDS["objects40x24x16"]()
+    g2.solid[g2.solid != 0] = 1
+    o2 = build_oracle(g2)
+    N = r["N"]
+    assert o2["N"] == N
+    assert not np.array_equal(o2["B"][:N], r["B"][:N])
+    codes = set(np.unique(g.solid).tolist())
+    assert {4, 5, -1, -128} <= codes
