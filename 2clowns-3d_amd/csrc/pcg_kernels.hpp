@@ -248,6 +248,8 @@ __device__ __forceinline__ void stv(V *p, V v) {
         *p = v;
 }
 typedef float f4v __attribute__((ext_vector_type(4)));
+// two floats at any 4-byte boundary (a dword-aligned 8-byte load)
+typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
 
 template <bool ST, typename T>
 __device__ __forceinline__ typename V2<T>::t lds2(const T *p) {
@@ -322,6 +324,7 @@ struct Ix32 {
     __device__ __forceinline__ Base blk(int) const { return 0; }
     template <bool ST = false> __device__ __forceinline__ Raw raw(int i) const { return ldn2<ST>(L + i); }
     __device__ __forceinline__ int2 dec1(Raw v, int, Base) const { return make_int2(v.x, v.y); }
+    static constexpr bool TABLE = false;
 };
 
 constexpr int LL16_ROWS = 512;  // rows per base block (two-row kernels: one block)
@@ -331,6 +334,7 @@ constexpr int LL16_WIDE = -32767;
 struct Ix16 {
     static constexpr int ROWS = LL16_ROWS;
     static constexpr int WIDE = LL16_WIDE;
+    static constexpr bool TABLE = false;
     const short *L;
     const int *L32;  // the same column, 32-bit (slots coded LL16_WIDE)
     const int2 *base;
@@ -522,6 +526,24 @@ struct Ix8 {
         if (c1 == LL8_WIDE) j.y = L32[i + 1];
         return j;
     }
+    // Decoding through a table in LDS, for kernels that stage one per code
+    // block: T[c] is code c's offset from its row (the ghost code's is large
+    // enough for the clamp to give N; the wide code's is not used).  One
+    // LDS read, an add and a clamp per slot, in place of the selects
+    // above; the kernels' pace is set by instruction issue about as much
+    // as by bytes.
+    static constexpr bool TABLE = true;
+    static __device__ __forceinline__ int tab_entry(int c, int4 b) {
+        const int cls = c >> 6;
+        const int bs = cls == 0 ? b.x : (cls == 1 ? b.y : (cls == 2 ? b.z : b.w));
+        int off = bs + (c & 63);
+        off = c == LL8_XM ? -1 : off;
+        off = c == LL8_XP ? 1 : off;
+        off = c == LL8_GHOST ? (1 << 30) : off;
+        return off;
+    }
+    // 0-based index of code c (not the wide code) at row i
+    __device__ __forceinline__ int near_t(int c, int i, const int *T) const { return min(max(i + T[c], 0), N); }
 };
 
 // Build the 8-bit form of the sorted LL: one block per LL8_ROWS rows finds
@@ -782,18 +804,25 @@ k_pm_v4(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0, con
         if (DOT == 2) s1 += (double)o * (double)o;
         if (DOT == 3) s0 += (double)o * (double)o;
     };
-    if (i + 3 < N) {
-        const f4v c0 = ldc4<ST>(a0 + i);
-        const int4 j0 = l0.template quad<ST>(i);
-        const f4v x = *reinterpret_cast<const f4v *>(in + i);
+    const bool full = i + 3 < N;
+    f4v c0, c1, x;
+    int4 j0, j1;
+    if (full) {
+        c0 = ldc4<ST>(a0 + i);
+        j0 = l0.template quad<ST>(i);
+        x = *reinterpret_cast<const f4v *>(in + i);
+        if (NT == 2) {
+            c1 = ldc4<ST>(a1 + i);
+            j1 = l1.template quad<ST>(i);
+        }
+    }
+    if (full) {
         f4v o;
         o.x = x.x - c0.x * in[j0.x];
         o.y = x.y - c0.y * in[j0.y];
         o.z = x.z - c0.z * in[j0.z];
         o.w = x.w - c0.w * in[j0.w];
         if (NT == 2) {
-            const f4v c1 = ldc4<ST>(a1 + i);
-            const int4 j1 = l1.template quad<ST>(i);
             o.x = o.x - c1.x * in[j1.x];
             o.y = o.y - c1.y * in[j1.y];
             o.z = o.z - c1.z * in[j1.z];
@@ -897,14 +926,31 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
         }
         xv = ld2(x + i);
     }
-    if (PAT) {
-        if (threadIdx.x < PAT_MAX * PAT_W) tab[threadIdx.x] = (T)tv;
+    // Ix8: the block's decode table (its 2 * BS rows are one code block)
+    __shared__ int DT[IX::TABLE ? 256 : 1];
+    if constexpr (IX::TABLE) {
+        static_assert(2 * BS == IX::ROWS, "a block is one code block");
+        const int blk = (int)(i / IX::ROWS);
+        DT[threadIdx.x] = (long)blk * IX::ROWS < N ? IX::tab_entry(threadIdx.x, LL.base[blk]) : 0;
+    }
+    if (PAT || IX::TABLE) {
+        if (PAT && threadIdx.x < PAT_MAX * PAT_W) tab[threadIdx.x] = (T)tv;
         __syncthreads();
     }
     if (full) {
         int2 l[NL];  // 1-based
 #pragma unroll
-        for (int k = 0; k < NL; ++k) l[k] = LL.col((size_t)k * lda).dec1(raw[k], i, bb);
+        for (int k = 0; k < NL; ++k) {
+            if constexpr (IX::TABLE) {
+                const IX c = LL.col((size_t)k * lda);
+                const int c0 = raw[k] & 255, c1 = raw[k] >> 8;
+                l[k] = make_int2(c.near_t(c0, i, DT) + 1, c.near_t(c1, i + 1, DT) + 1);
+                if (c0 == IX::WIDE) l[k].x = c.L32[i];
+                if (c1 == IX::WIDE) l[k].y = c.L32[i + 1];
+            } else {
+                l[k] = LL.col((size_t)k * lda).dec1(raw[k], i, bb);
+            }
+        }
         typename V2<T>::t a[NL];
         typename V2<T>::t d;
         if (PAT && p0 < PAT_MAX && p1 < PAT_MAX) {
@@ -1032,8 +1078,15 @@ k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restri
         b = LL.base[i / IX::ROWS];
         xv = ld2(x + i);
     }
-    if (PAT) {
-        if (threadIdx.x < PAT_MAX * PAT_W) tab[threadIdx.x] = (float)tv;
+    // Ix8: the block's decode table (its 2 * BS rows are one code block)
+    __shared__ int T[IX::TABLE ? 256 : 1];
+    if constexpr (IX::TABLE) {
+        static_assert(2 * BS == IX::ROWS, "a block is one code block");
+        const int blk = (int)(i / IX::ROWS);
+        T[threadIdx.x] = (long)blk * IX::ROWS < N ? IX::tab_entry(threadIdx.x, LL.base[blk]) : 0;
+    }
+    if (PAT || IX::TABLE) {
+        if (PAT && threadIdx.x < PAT_MAX * PAT_W) tab[threadIdx.x] = (float)tv;
         __syncthreads();
     }
     if (full) {
@@ -1062,15 +1115,32 @@ k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restri
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             const IX c = LL.col((size_t)k * lda);
-            j[0][k] = c.near1(cr[0][k], i, b);
-            j[1][k] = c.near1(cr[1][k], i + 1, b);
+            if constexpr (IX::TABLE) {
+                j[0][k] = c.near_t(cr[0][k], i, T) + 1;
+                j[1][k] = c.near_t(cr[1][k], i + 1, T) + 1;
+            } else {
+                j[0][k] = c.near1(cr[0][k], i, b);
+                j[1][k] = c.near1(cr[1][k], i + 1, b);
+            }
             if (cr[0][k] == IX::WIDE) j[0][k] = c.L32[i];
             if (cr[1][k] == IX::WIDE) j[1][k] = c.L32[i + 1];
         }
+        // the two rows' neighbours in a slot are usually adjacent (same
+        // offset): one 8-byte load takes both, a second load only where not
+        float g[2][6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            acc0 = acc0 + ar[0][k] * x[j[0][k] - 1];
-            acc1 = acc1 + ar[1][k] * x[j[1][k] - 1];
+            const f2u v = *reinterpret_cast<const f2u *>(x + (j[0][k] - 1));
+            g[0][k] = v.x;
+            g[1][k] = v.y;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (j[1][k] != j[0][k] + 1) g[1][k] = x[j[1][k] - 1];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            acc0 = acc0 + ar[0][k] * g[0][k];
+            acc1 = acc1 + ar[1][k] * g[1][k];
         }
         float2 o;
         o.x = acc0;
