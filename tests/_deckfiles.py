@@ -37,6 +37,21 @@ def object_code(nfb):
     return nfb
 
 
+def object_nfb(code):
+    """nfb of an object cell from its box code (the inverse of object_code)."""
+    if code == -127:
+        return -1
+    if code == -1:
+        return 300
+    if code == -128:
+        return 200
+    if code == -2:
+        return 1
+    if code <= -3:
+        return 124 - code
+    return code
+
+
 def make_case(tmp, nx, ny, nz, seed, sparse, be=False, lo=(3, 3, 3), t0=12.5, with_dt0=True, obstacle=True,
               objects=False, fdata=True, sparse_frac=0.6):
     rng = np.random.default_rng(seed)
@@ -178,13 +193,19 @@ def make_case(tmp, nx, ny, nz, seed, sparse, be=False, lo=(3, 3, 3), t0=12.5, wi
 
 def write_grid_case(tmp, g, nf, nb, p, sparse=False, be=False, lo=(3, 3, 3), t0=1.0, g_acc=9.8, nu=1.0e-6):
     """Case files holding a configs.Grid as it is: grid file (faces
-    X(l, ls-2 .. le+1) continuing g.xf), data file (nf/nb the flags per box
-    cell, -1 for solid ones; U, F, p; dt0 = g.dt) and open-ratio file
-    (g.a, g.fb).  Returns the three paths."""
+    X(l, ls-2 .. le+1) continuing g.xf), data file (nf/nb the flags of the
+    flow cells; cells with g.solid != 0 get nf = -1 and, for object cells,
+    the nfb of their code; U, F, p; dt0 = g.dt) and open-ratio file (g.a,
+    g.fb).  Object cells follow read_data's numbering (data_io.F90:168-199):
+    boundary objects among the flow cells, appended objects (nfb >= 200)
+    after them with U and a0 only.  Returns the three paths."""
     X, Y, Z = g.X, g.Y, g.Z
     nc = g.ncell
-    nf = np.asarray(nf).ravel().astype(np.int32)
-    nb = np.asarray(nb).ravel().astype(np.int32)
+    sol = np.asarray(g.solid).ravel()
+    nf = np.where(sol != 0, -1, np.asarray(nf).ravel()).astype(np.int32)
+    nfb = np.where(sol == 0, np.asarray(nb).ravel(), 0).astype(np.int32)
+    for c in np.nonzero((sol != 0) & (sol != 1))[0]:
+        nfb[c] = object_nfb(int(sol[c]))
     un = g.un.reshape(3, -1)
     F = g.F.ravel()
     a = g.a.reshape(3, -1)
@@ -192,7 +213,10 @@ def write_grid_case(tmp, g, nf, nb, p, sparse=False, be=False, lo=(3, 3, 3), t0=
     p = np.asarray(p).ravel()
     is_, js, ks = lo
     ext = (is_, js, ks, is_ + g.nx, js + g.ny, ks + g.nz)
-    cells = [i + X * (j + Y * k) for j in range(Y) for k in range(Z) for i in range(X) if nf[i + X * (j + Y * k)] >= 0]
+    sweep = [i + X * (j + Y * k) for j in range(Y) for k in range(Z) for i in range(X)]
+    cells = [c for c in sweep if nf[c] >= 0 or 1 <= nfb[c] < 200]
+    app = [c for c in sweep if nf[c] < 0 and nfb[c] >= 200]
+    inne = len(cells)
     gname = tmp / ("case.xyzn" if sparse else "case.xyz")
     dname, fname = tmp / "case.data", tmp / "case.fdata0"
     with open(gname, "wb") as f:
@@ -205,26 +229,35 @@ def write_grid_case(tmp, g, nf, nb, p, sparse=False, be=False, lo=(3, 3, 3), t0=
             sol = [(i, j, k) for i in range(X) for j in range(Y) for k in range(Z) if nf[i + X * (j + Y * k)] == -1]
             _rec(f, "i", [len(sol)], be)
             for i, j, k in sol:
-                _rec(f, "5i", [i + is_ - 1, k + ks - 1, j + js - 1, -1, 0], be)
+                c = i + X * (j + Y * k)
+                _rec(f, "5i", [i + is_ - 1, k + ks - 1, j + js - 1, -1, nfb[c]], be)
     with open(dname, "wb") as f:
         _rec(f, "3d", [t0, g_acc, nu], be)
         if not sparse:
             order = [i + X * (j + Y * k) for i in range(X) for j in range(Y) for k in range(Z)]
             _rec(f, f"{nc}i", nf[order], be)
-            _rec(f, f"{nc}i", np.where(nf < 0, 0, nb)[order], be)
+            _rec(f, f"{nc}i", nfb[order], be)
             for c in cells:
                 _rec(f, "5d", [*un[:, c], F[c], p[c]], be)
+            for c in app:
+                _rec(f, "3d", [*un[:, c]], be)
         else:
-            _rec(f, "2i", [len(cells), 0], be)
+            _rec(f, "2i", [len(cells), len(app)], be)
             for q, c in enumerate(cells):
-                _rec(f, "i5d2i", [q + 1, *un[:, c], F[c], p[c], nf[c], nb[c]], be)
+                _rec(f, "i5d2i", [q + 1, *un[:, c], F[c], p[c], nf[c], nfb[c]], be)
+            for q, c in enumerate(app):
+                _rec(f, "i3d", [inne + q + 1, *un[:, c]], be)
         _rec(f, "d", [g.dt], be)
     with open(fname, "wb") as f:
         if not sparse:
             for c in cells:
                 _rec(f, "4d", [*a[:, c], fb[c]], be)
+            for c in app:
+                _rec(f, "3d", [*a[:, c]], be)
         else:
-            _rec(f, "2i", [len(cells), 0], be)
+            _rec(f, "2i", [len(cells), len(app)], be)
             for q, c in enumerate(cells):
                 _rec(f, "i4d", [q + 1, *a[:, c], fb[c]], be)
+            for q, c in enumerate(app):
+                _rec(f, "i3d", [inne + q + 1, *a[:, c]], be)
     return gname, dname, fname

@@ -28,8 +28,8 @@ import shutil
 import numpy as np
 import pytest
 
-from _deckfiles import make_case
-from conftest import ROOT, ptr
+from _deckfiles import make_case, object_code, write_grid_case
+from conftest import REF_ASM_SO, ROOT, ptr
 
 REF_DECKS_SO = os.path.join(ROOT, "oracle", "_ref", "libref_decks.so")
 CASES = [("dense", False), ("sparse", True)]
@@ -245,6 +245,78 @@ def test_sparse_writer_drops_stateless_cells(tmp_path):
     kept = [c[0], c[2], c[3]]
     assert np.array_equal(p2.ravel()[kept], np.array(kept, float))
     assert p2.ravel()[c[1]] == 0.0 and p2.ravel()[c[4]] == 0.0
+
+
+def reference_grid(paths, like):
+    """The box grid the reference itself makes of case files: the flang
+    build of read_data + read_fdata places each numbered cell's values at
+    the box position of its own (i, j, k); a cell's code comes from the
+    reference's flags (flow cell 0; boundary object its nfb's code; appended
+    cell -1 or -128 from the nfb read_data resets it to); every cell the
+    reference does not number is outside the case (1); faces and dx come
+    from the flang build of set_dx (src2LOWNS/1sub-main.F90:53-74) on the
+    faces the reference read.  `like` gives the box size.  Returns (grid, p)."""
+    from clowns3d import configs
+    nx, ny, nz = like.nx, like.ny, like.nz
+    r = ref_read(dict(grid=paths[0], data=paths[1], fdata=paths[2], nc=like.ncell, nx=nx, ny=ny, nz=nz,
+                      X=like.X, Y=like.Y))
+    g = configs.Grid(nx, ny, nz)
+    nc = g.ncell
+    sol = np.ones(nc, np.int8)
+    F, P, fb = np.zeros(nc), np.zeros(nc), np.zeros(nc)
+    un, a = np.zeros((3, nc)), np.zeros((3, nc))
+    for q in range(r["inne2"]):
+        c = r["box"][q]
+        f, b = (int(v) for v in r["flags"][q])
+        sol[c] = 0 if f >= 0 else (object_code(b) if q < r["inne"] else (-1 if b == -1 else -128))
+        un[:, c], F[c], P[c], a[:, c], fb[c] = r["U"][:, q], r["F"][q], r["P"][q], r["A"][:, q], r["FB"][q]
+    shape = (g.Z, g.Y, g.X)
+    g.solid[:] = sol.reshape(shape)
+    g.F[:], g.fb[:] = F.reshape(shape), fb.reshape(shape)
+    g.un[:], g.a[:] = un.reshape((3,) + shape), a.reshape((3,) + shape)
+    if not os.path.exists(REF_ASM_SO):
+        pytest.skip("flang build of the reference (oracle/_ref) not present")
+    lib = ctypes.CDLL(REF_ASM_SO, mode=os.RTLD_LAZY)
+    g.xf[:] = 0.0
+    g.dx[:] = 0.0
+    for l, n in enumerate((nx, ny, nz)):
+        x = np.ascontiguousarray(r["xo"][l, :n + 4])
+        dx = np.zeros_like(x)
+        lib.ref_set_dx(len(x), ptr(x), ptr(dx))
+        m = min(n + 3, g.stride)  # box cells 0 .. n+1, then the last face where it fits
+        g.xf[l, :m], g.dx[l, :m] = x[1:m + 1], dx[1:m + 1]
+    g.h = tuple(float(g.dx[l, 1]) for l in range(3))
+    if r["hdr"][3] > 0:
+        g.dt = float(r["hdr"][3])
+    return g, P.reshape(shape)
+
+
+def assert_same_case(g, p, gr, pr):
+    """Every field the assembly and the pressure loop read, bit for bit."""
+    assert np.array_equal(g.solid, gr.solid)
+    for name in ("F", "fb", "a", "un", "xf", "dx"):
+        assert np.array_equal(getattr(g, name), getattr(gr, name)), name
+    assert np.array_equal(p, pr)
+    assert g.h == gr.h and g.dt == gr.dt
+
+
+@pytest.mark.parametrize("name,sparse", CASES)
+def test_object_grid_files_match_reference(tmp_path, name, sparse):
+    """A grid with object cells of every kind (configs.config_objects:
+    pressure-boundary cells, inflow boundary objects, appended obstacle
+    cells) written to case files: the product's reader and the reference's
+    reader + set_dx give the same box grid, pressure and spacing."""
+    from clowns3d import assembly, configs, decks
+    g0 = configs.config_objects(40, 24, 16, seed=11)
+    c = assembly.classify(g0)
+    p0 = np.where(g0.solid.ravel() == 1, 0.0, np.random.default_rng(7).standard_normal(g0.ncell) * 50)
+    paths = write_grid_case(tmp_path, g0, c["nf"], c["nb"], p0, sparse=sparse)
+    g1, p1, d = decks.read_case(*paths)
+    assert d.inne2 > d.inne
+    gr, pr = reference_grid(paths, g1)
+    assert_same_case(g1, p1, gr, pr)
+    codes = set(np.unique(gr.solid).tolist())
+    assert {4, 5, 7, -2, -30, -1, -128} <= codes
 
 
 def test_object_codes(tmp_path):
