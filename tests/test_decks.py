@@ -16,14 +16,17 @@ Bar, all bit-exact:
   them (U and a0 only), nfb = 0 and -1 cells not numbered; the reader gives
   each its object code, and the writer puts nfb back as output_data does;
 - truncated files give CLOWNS_EDECK.
-The flang runtime of the reference build reads and writes the host byte
-order (no main program sets its conversion), so the comparisons with it
-use little-endian files; the big-endian form, the one the reference's own
-build writes, is checked against the same fixture.
+The reference's readers and writer are compared in both byte orders: the
+little-endian form through oracle/_ref/libref_decks.so (flang's runtime in
+a library keeps the host order), the big-endian form -- the one the
+reference's own build writes -- through oracle/_ref/ref_decks_be, the same
+routines under a main program compiled with -fconvert=big-endian.
 """
 import ctypes
 import os
 import shutil
+import struct
+import subprocess
 
 import numpy as np
 import pytest
@@ -32,6 +35,7 @@ from _deckfiles import make_case, object_code, write_grid_case
 from conftest import REF_ASM_SO, ROOT, ptr
 
 REF_DECKS_SO = os.path.join(ROOT, "oracle", "_ref", "libref_decks.so")
+REF_DECKS_BE = os.path.join(ROOT, "oracle", "_ref", "ref_decks_be")
 CASES = [("dense", False), ("sparse", True)]
 
 
@@ -60,12 +64,70 @@ def ref_read(case, data=None):
     flags = np.zeros((nmax, 2), np.int32)
     lib.ref_read_deck(gp, len(gp), dp, len(dp), fp, len(fp), ptr(ext), ptr(counts), ptr(hdr), ptr(xo), S, nmax,
                       ptr(cells), ptr(uo), ptr(fo), ptr(po), ptr(ao), ptr(fbo), ptr(flags))
+    return _ref_dict(case, ext, counts, hdr, xo, cells, uo, fo, po, ao, fbo, flags)
+
+
+def _ref_dict(case, ext, counts, hdr, xo, cells, uo, fo, po, ao, fbo, flags):
     n = int(counts[1])  # every numbered cell: 1..inne and the appended ones
     is_, js, ks = ext[:3]
     X, Y = case["X"], case["Y"]
     i, j, k = cells[:n, 0] - (is_ - 1), cells[:n, 1] - (js - 1), cells[:n, 2] - (ks - 1)
     return dict(ext=ext, inne=int(counts[0]), inne2=n, hdr=hdr, xo=xo, box=i + X * (j + Y * k),
                 U=uo[:n].T, F=fo[:n], P=po[:n], A=ao[:n].T, FB=fbo[:n], flags=flags[:n])
+
+
+def _hex(v):
+    return struct.pack(">d", v).hex()
+
+
+def ref_read_be(case, data=None, write=None):
+    """read_data + read_fdata of the reference built with big-endian records
+    (oracle/_ref/ref_decks_be, a main program compiled with
+    -fconvert=big-endian) on big-endian case files; the same dict as
+    ref_read.  write = (path, t, dt): output_data of the state read, written
+    big-endian."""
+    if not os.path.exists(REF_DECKS_BE):
+        pytest.skip("flang build of the reference (oracle/_ref) not present")
+    nmax = case["nc"]
+    S = max(case["nx"], case["ny"], case["nz"]) + 8
+    out = str(case["grid"]) + ".ref_be.txt"
+    cmd = [REF_DECKS_BE, str(case["grid"]), str(data or case["data"]), str(case["fdata"]), str(nmax), str(S), out]
+    if write is not None:
+        cmd += [str(write[0]), _hex(write[1]), _hex(write[2])]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=60)
+    with open(out) as f:
+        vals = f.read().split()
+    pos = 0
+
+    def ints(n):
+        nonlocal pos
+        v = np.array([int(x) for x in vals[pos:pos + n]], np.int64)
+        pos += n
+        return v
+
+    def reals(n):
+        nonlocal pos
+        v = np.array([struct.unpack(">d", bytes.fromhex(x))[0] for x in vals[pos:pos + n]])
+        pos += n
+        return v
+
+    ext = ints(6)
+    counts = ints(2)
+    hdr = reals(4)
+    xo = reals(3 * S).reshape(3, S)
+    n = min(int(counts[1]), nmax)
+    cells, flags = np.zeros((n, 3), np.int64), np.zeros((n, 2), np.int64)
+    uo, ao = np.zeros((n, 3)), np.zeros((n, 3))
+    fo, po, fbo = np.zeros(n), np.zeros(n), np.zeros(n)
+    for q in range(n):
+        cells[q] = ints(3)
+        uo[q] = reals(3)
+        fo[q], po[q] = reals(2)
+        ao[q] = reals(3)
+        fbo[q] = reals(1)[0]
+        flags[q] = ints(2)
+    assert pos == len(vals)
+    return _ref_dict(case, ext, counts, hdr, xo, cells, uo, fo, po, ao, fbo, flags)
 
 
 def ref_write(path, t, dt):
@@ -124,10 +186,11 @@ def test_reader_without_fdata_defaults(tmp_path, name, sparse):
 @pytest.mark.parametrize("name,sparse", CASES)
 @pytest.mark.parametrize("shape,lo,objects", [((7, 5, 4), (3, 3, 3), False), ((12, 9, 6), (5, 3, 4), False),
                                               ((9, 7, 5), (3, 2, 4), True)])
-def test_reader_matches_reference(tmp_path, name, sparse, shape, lo, objects):
-    case = make_case(tmp_path, *shape, seed=sum(shape) + sparse, sparse=sparse, lo=lo, objects=objects)
+@pytest.mark.parametrize("be", [False, True], ids=["le", "be"])
+def test_reader_matches_reference(tmp_path, name, sparse, shape, lo, objects, be):
+    case = make_case(tmp_path, *shape, seed=sum(shape) + sparse, sparse=sparse, lo=lo, objects=objects, be=be)
     _check_fields(case, *ours(case))
-    r = ref_read(case)
+    r = ref_read_be(case) if be else ref_read(case)
     g, p, d = ours(case)
     assert tuple(r["ext"]) == case["ext"]
     assert r["inne"] == d.inne and r["inne2"] == d.inne2
@@ -175,21 +238,28 @@ def writer_flags(case):
 
 @pytest.mark.parametrize("name,sparse", CASES)
 @pytest.mark.parametrize("objects", [False, True])
-def test_writer_byte_identical_to_reference(tmp_path, name, sparse, objects):
+@pytest.mark.parametrize("be", [False, True], ids=["le", "be"])
+def test_writer_byte_identical_to_reference(tmp_path, name, sparse, objects, be):
     from clowns3d import decks
-    case = make_case(tmp_path, 8, 6, 5, seed=21 + sparse, sparse=sparse, objects=objects)
+    case = make_case(tmp_path, 8, 6, 5, seed=21 + sparse, sparse=sparse, objects=objects, be=be)
     g, p, d = ours(case)
     nf, nb = writer_flags(case)
     rng = np.random.default_rng(4)
     # a new (solved) pressure on the open cells
     p2 = p.ravel() + rng.standard_normal(case["nc"]) * np.where(case["solid"] == 0, 1.0, 0.0)
     t, dt = 37.25, 0.00125
-    ref_read(case)
     # put the new pressure into the reference's state through a data file
-    # it reads: ours, written once, read by the reference
+    # it reads: ours, written once, read by the reference, which then writes
+    # it out again with output_data
     mine = tmp_path / "mine.data"
-    decks.write_data(mine, d, nf, nb, g.un.reshape(3, -1), g.F.ravel(), p2, t, dt, big_endian=False)
-    r = ref_read(case, data=mine)
+    theirs = tmp_path / "theirs.data"
+    decks.write_data(mine, d, nf, nb, g.un.reshape(3, -1), g.F.ravel(), p2, t, dt, big_endian=be)
+    if be:
+        r = ref_read_be(case, data=mine, write=(theirs, t, dt))
+    else:
+        ref_read(case)
+        r = ref_read(case, data=mine)
+        ref_write(theirs, t, dt)
     c = r["box"]
     U = g.un.reshape(3, -1)
     expect = np.where(np.arange(len(c)) < d.inne, p2[c], 0.0)  # appended cells carry no P
@@ -199,9 +269,9 @@ def test_writer_byte_identical_to_reference(tmp_path, name, sparse, objects):
         expect = np.where(dropped, 0.0, expect)
     assert np.array_equal(r["P"], expect)
     assert np.array_equal(r["U"], g.un.reshape(3, -1)[:, r["box"]])
-    theirs = tmp_path / "theirs.data"
-    ref_write(theirs, t, dt)
-    assert mine.read_bytes() == theirs.read_bytes()
+    raw = mine.read_bytes()
+    assert raw[:4] == (24).to_bytes(4, "big" if be else "little")
+    assert raw == theirs.read_bytes()
 
 
 @pytest.mark.parametrize("name,sparse", CASES)
