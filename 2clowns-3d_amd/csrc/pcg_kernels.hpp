@@ -770,6 +770,156 @@ k_pat_assign(int N, int npad, int lda, const double *__restrict__ AL, const doub
     if (threadIdx.x == 0 && cnt) atomicAdd(nnone, cnt);
 }
 
+// FP64 PM stage for an operator with row patterns: k_pm_v2's stage (NT, DOT
+// as there) with two changes.  A wavefront takes two chunks of 128
+// consecutive rows, two rows a lane in each, and issues both chunks' loads
+// before either chunk's gathers -- twice the bytes in flight per wavefront;
+// and a pattern row takes its coefficient from the table (column `col` of
+// ptab, and col + 1 for NT = 2, staged in LDS), so the coefficient column
+// is read only for pairs holding a row of their own (at 512x512x96 with
+// bathymetry 95% of the rows are pattern rows and 9% of the column's cache
+// lines hold another).  Measured on MI355X at 13.1M rows, natural row order,
+// streamed, single stage: 50.7 us against 57.6 us for k_pm_v2; the two
+// chunks without the table 60.2 us, the table with one chunk 53.8 us.
+// A workgroup does the work of two k_pm_v2 workgroups -- row blocks 2g and
+// 2g+1 of the block order, wavefront pair h = 0, 1 taking block 2g + h --
+// and every row's arithmetic is k_pm_v2's.  Dot partials: chunk c of
+// wavefront q holds exactly the rows of k_pm_v2's wavefront 2(q&1) + c of
+// the same 512-row block, so each chunk's wave_sum equals that wavefront's
+// and the four are added in k_pm_v2's order: the partials, and the solve,
+// are bit-identical.  G: the row blocks (k_pm_v2's grid); the grid here is
+// geo_half(G) workgroups, those past G/2 doing nothing.
+template <int NT, int DOT, typename IX, bool ST>
+__global__ void __launch_bounds__(BS)
+k_pm_p2(const Scal *__restrict__ sc, int N, int G, const double *__restrict__ a0, IX l0,
+        const double *__restrict__ a1, IX l1, int col, const double *__restrict__ in,
+        double *__restrict__ out, const double *__restrict__ w, double *__restrict__ part,
+        const int *__restrict__ order, const uint8_t *__restrict__ pid, const double *__restrict__ ptab) {
+    __shared__ double t0[PAT_MAX], t1[PAT_MAX];
+    const int done = sc->done;
+    if (threadIdx.x < PAT_MAX) {
+        t0[threadIdx.x] = ptab[threadIdx.x * PAT_W + col];
+        if (NT == 2) t1[threadIdx.x] = ptab[threadIdx.x * PAT_W + col + 1];
+    }
+    __syncthreads();
+    if (done) return;
+    const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int blk = 2 * logical_block() + (q >> 1);
+    const bool live = blk < G;
+    const int rb = live ? (order ? order[blk] : blk) : 0;
+    const int i0 = rb * (2 * BS) + (q & 1) * (4 * 64) + 2 * lane;
+    bool full[2];
+    uint16_t pp[2];
+    typename IX::Raw r0[2], r1[2];
+    typename IX::Base b0[2];
+    double2 xv[2], wv[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int i = i0 + 128 * c;
+        full[c] = live && i + 1 < N;
+        if (full[c]) {
+            pp[c] = ldv<ST>(reinterpret_cast<const uint16_t *>(pid + i));
+            b0[c] = l0.blk(i);
+            r0[c] = l0.template raw<ST>(i);
+            if (NT == 2) r1[c] = l1.template raw<ST>(i);
+            xv[c] = ld2(in + i);
+            if (DOT == 1 || DOT == 2) wv[c] = lds2<ST>(w + i);
+        }
+    }
+    // coefficients of the pairs with a row of their own
+    double2 c0[2], c1[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        c0[c] = make_double2(0.0, 0.0);
+        c1[c] = make_double2(0.0, 0.0);
+        if (full[c] && ((pp[c] & 255) >= PAT_MAX || (pp[c] >> 8) >= PAT_MAX)) {
+            c0[c] = lds2<ST>(a0 + i0 + 128 * c);
+            if (NT == 2) c1[c] = lds2<ST>(a1 + i0 + 128 * c);
+        }
+    }
+    double g0[2][2], g1[2][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if (full[c]) {
+            const int i = i0 + 128 * c;
+            const int2 j0 = l0.dec1(r0[c], i, b0[c]);
+            g0[c][0] = in[j0.x - 1];
+            g0[c][1] = in[j0.y - 1];
+            if (NT == 2) {
+                const int2 j1 = l1.dec1(r1[c], i, b0[c]);  // one base array for all columns
+                g1[c][0] = in[j1.x - 1];
+                g1[c][1] = in[j1.y - 1];
+            }
+        }
+    }
+    double s0[2] = {0.0, 0.0}, s1[2] = {0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int i = i0 + 128 * c;
+        if (full[c]) {
+            const int p0 = pp[c] & 255, p1 = pp[c] >> 8;
+            const double a00 = p0 < PAT_MAX ? t0[p0] : c0[c].x;
+            const double a01 = p1 < PAT_MAX ? t0[p1] : c0[c].y;
+            double o0 = xv[c].x - a00 * g0[c][0];
+            double o1 = xv[c].y - a01 * g0[c][1];
+            if (NT == 2) {
+                const double a10 = p0 < PAT_MAX ? t1[p0] : c1[c].x;
+                const double a11 = p1 < PAT_MAX ? t1[p1] : c1[c].y;
+                o0 = o0 - a10 * g1[c][0];
+                o1 = o1 - a11 * g1[c][1];
+            }
+            double2 o;
+            o.x = o0;
+            o.y = o1;
+            sts2<ST>(out + i, o);
+            if (DOT == 1 || DOT == 2) {
+                s0[c] += wv[c].x * o0;
+                s0[c] += wv[c].y * o1;
+            }
+            if (DOT == 2) {
+                s1[c] += o0 * o0;
+                s1[c] += o1 * o1;
+            }
+            if (DOT == 3) {
+                s0[c] += o0 * o0;
+                s0[c] += o1 * o1;
+            }
+        } else if (live && i < N) {
+            double o = in[i] - a0[i] * in[l0.one(i)];
+            if (NT == 2) o = o - a1[i] * in[l1.one(i)];
+            out[i] = o;
+            if (DOT == 1 || DOT == 2) s0[c] += w[i] * o;
+            if (DOT == 2) s1[c] += o * o;
+            if (DOT == 3) s0[c] += o * o;
+        }
+    }
+    if (DOT != 0) {
+        // [k_pm_v2 wavefront of the row block][dot], per half of the workgroup
+        __shared__ double ws[2][4][2];
+        const int h = q >> 1;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double v0 = wave_sum(s0[c]);
+            const double v1 = DOT == 2 ? wave_sum(s1[c]) : 0.0;
+            if (lane == 0) {
+                ws[h][2 * (q & 1) + c][0] = v0;
+                ws[h][2 * (q & 1) + c][1] = v1;
+            }
+        }
+        __syncthreads();
+        if (live && (threadIdx.x & 127) == 0) {
+            double u0 = 0.0, u1 = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                u0 += ws[h][k][0];
+                u1 += ws[h][k][1];
+            }
+            part[rb] = u0;
+            if (DOT == 2) part[G + rb] = u1;
+        }
+    }
+}
+
 // Four consecutive PM coefficients as FP32: one 8-byte (half precision) or
 // 16-byte load.
 template <bool ST, typename C>

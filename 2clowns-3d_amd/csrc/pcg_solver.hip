@@ -231,6 +231,12 @@ struct Knobs {
     // at 2.45M rows FP64 SpMV 36 -> 20 us (cached).  Results are
     // bit-identical.  CLOWNS_PATTERNS = 0 switches them off.
     bool patterns = true;
+    // FP64 PM stages of an operator with row patterns in the two-chunk form
+    // with the pattern table (k_pm_p2): a wavefront's two chunks of loads
+    // in flight before its gathers, a pattern row's coefficient from LDS.
+    // Bit-identical to k_pm_v2.  tune_stream times it with the other forms
+    // (bit 3); CLOWNS_PM_PAT = 0 off, 1 on, -1 (default) timed.
+    int pm_pat = -1;
     // CLOWNS_VERBOSE = 1: report per-operator tuning decisions on stderr.
     bool verbose = false;
 };
@@ -329,6 +335,7 @@ struct clowns_solver {
     int st64_f = -1, st32_f = -1;      // ... under this forcing (force_key)
     int ix64 = -1, ix32 = -1;          // 8-bit (1) or 16-bit (0) index codes chosen by tune_stream ...
     int ix64_N = 0, ix32_N = 0;        // ... for operators of about this many rows
+    int pc64 = -1, pc64_N = 0;         // FP64 PM stages as k_pm_p2 (tune_stream), for about this many rows
 };
 
 namespace {
@@ -349,6 +356,8 @@ inline Geo geo_of(int N, int R) {
     G = (G + 7) & ~7;
     return {G, R};
 }
+// Workgroups of k_pm_p2 for G two-row blocks: each does two of them.
+inline int geo_half(int G) { return ((G + 1) / 2 + 7) & ~7; }
 
 int ensure_events(clowns_solver *h) {
     if (!h->ev0) {
@@ -402,6 +411,7 @@ template <typename T> struct Op {
     const int *nonunit = nullptr;   // ... device count of rows whose FP32 diagonal is not 1
     const uint8_t *pid = nullptr;   // row pattern ids and table (NL = 6), or null
     const double *ptab = nullptr;
+    bool pc2 = false;               // FP64 PM stages as k_pm_p2 (with pid; tune_stream)
     KProf *prof = nullptr;          // armed kernel profile, or null
 };
 
@@ -484,11 +494,25 @@ void launch_pm_v4(hipStream_t s, int G4, const Scal *sc, int N, const C *a0, IX 
     }
 }
 
+template <int NT, typename IX, bool ST>
+void launch_pm_p2(hipStream_t s, int G2, const Scal *sc, const Op<double> &A, const double *a0, IX x0,
+                  const double *a1, IX x1, int col, const double *in, double *out, int dot, const double *w,
+                  double *part) {
+    const int W = geo_half(G2);
+    switch (dot) {
+    case 0: klaunch(k_pm_p2<NT, 0, IX, ST>, W, BS, s, sc, A.N, G2, a0, x0, a1, x1, col, in, out, w, part, A.order, A.pid, A.ptab); break;
+    case 1: klaunch(k_pm_p2<NT, 1, IX, ST>, W, BS, s, sc, A.N, G2, a0, x0, a1, x1, col, in, out, w, part, A.order, A.pid, A.ptab); break;
+    case 2: klaunch(k_pm_p2<NT, 2, IX, ST>, W, BS, s, sc, A.N, G2, a0, x0, a1, x1, col, in, out, w, part, A.order, A.pid, A.ptab); break;
+    default: klaunch(k_pm_p2<NT, 3, IX, ST>, W, BS, s, sc, A.N, G2, a0, x0, a1, x1, col, in, out, w, part, A.order, A.pid, A.ptab);
+    }
+}
+
 // One PM stage in the form for T and the operator; returns the grid (the
-// number of dot partials a dot-carrying stage wrote).
+// number of dot partials a dot-carrying stage wrote).  col: the stage's
+// coefficient column.
 template <typename T, int NT, typename C, typename IX>
 int launch_pm_stage_ix(hipStream_t s, int G2, const Scal *sc, const Op<T> &A, const C *a0, IX x0, const C *a1,
-                        IX x1, const T *in, T *out, int dot, const T *w, double *part) {
+                        IX x1, int col, const T *in, T *out, int dot, const T *w, double *part) {
     if constexpr (sizeof(T) == 4) {
         // the dot-carrying stages always take four rows a thread; the single
         // stages unless CLOWNS_V4_32 = 0 (bit-identical either way)
@@ -499,6 +523,15 @@ int launch_pm_stage_ix(hipStream_t s, int G2, const Scal *sc, const Op<T> &A, co
             else
                 launch_pm_v4<NT, IX, false, C>(s, G4, sc, A.N, a0, x0, a1, x1, in, out, dot, w, part);
             return G4;
+        }
+    }
+    if constexpr (std::is_same<T, double>::value && std::is_same<C, double>::value) {
+        if (A.pc2 && A.pid) {
+            if (A.st_pm)
+                launch_pm_p2<NT, IX, true>(s, G2, sc, A, a0, x0, a1, x1, col, in, out, dot, w, part);
+            else
+                launch_pm_p2<NT, IX, false>(s, G2, sc, A, a0, x0, a1, x1, col, in, out, dot, w, part);
+            return G2;
         }
     }
     if (A.st_pm)
@@ -516,14 +549,14 @@ int launch_pm_stage_c(hipStream_t s, const Scal *sc, const Op<T> &A, const C *AL
     const int G2 = geo_of(A.N, 2).G;
     if (has8(A)) {
         const Ix8 x0 = ix8(A, o0), x1 = NT == 2 ? ix8(A, o1) : x0;
-        return launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
+        return launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, j, in, out, dot, w, part);
     }
     if (has16(A)) {
         const Ix16 x0 = ix16(A, o0), x1 = NT == 2 ? ix16(A, o1) : x0;
-        return launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
+        return launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, j, in, out, dot, w, part);
     }
     const Ix32 x0 = ix32(A, o0), x1 = NT == 2 ? ix32(A, o1) : x0;
-    return launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, in, out, dot, w, part);
+    return launch_pm_stage_ix<T, NT, C>(s, G2, sc, A, a0, x0, a1, x1, j, in, out, dot, w, part);
 }
 
 // One stage of LDUSUB (NT = 2: the two-term stage 5/6); returns the grid
@@ -707,20 +740,27 @@ void spmv(hipStream_t s, const Scal *sc, const Op<T> &A, int mode, const T *x, c
 }
 
 // Kernel forms as bits: bit 0 streamed PM stages, bit 1 streamed SpMV, bit 2
-// 8-bit index codes (only where the operator has them).
+// 8-bit index codes (only where the operator has them), bit 3 the FP64 PM
+// stages as k_pm_p2 (only FP64 operators with row patterns).
+constexpr int N_FORMS = 16;
 template <typename T> bool can_ix8(const Op<T> &A) { return A.LL8 != nullptr; }
+template <typename T> bool can_pc2(const Op<T> &A) { return sizeof(T) == 8 && A.pid != nullptr && A.NL == 6; }
 template <typename T> void set_forms(Op<T> &A, int f) {
     A.st_pm = (f & 1) != 0;
     A.st_spmv = (f & 2) != 0;
     A.ix8 = (f & 4) != 0 && can_ix8(A);
+    A.pc2 = (f & 8) != 0 && can_pc2(A);
 }
 // The environment's forcing, which the stored decisions are keyed on.
-inline int force_key(const clowns_solver *h) { return (h->kn.stream + 1) | ((h->kn.ll8 + 1) << 2); }
+inline int force_key(const clowns_solver *h) {
+    return (h->kn.stream + 1) | ((h->kn.ll8 + 1) << 2) | ((h->kn.pm_pat + 1) << 4);
+}
 // Bits fixed for operator A (mask) and their values.
 template <typename T> void fixed_forms(const clowns_solver *h, const Op<T> &A, int &mask, int &val) {
     mask = val = 0;
     if (h->kn.stream >= 0) mask |= 3, val |= h->kn.stream;
     if (!can_ix8(A) || h->kn.ll8 >= 0) mask |= 4, val |= (can_ix8(A) && h->kn.ll8 == 1) ? 4 : 0;
+    if (!can_pc2(A) || h->kn.pm_pat >= 0) mask |= 8, val |= (can_pc2(A) && h->kn.pm_pat == 1) ? 8 : 0;
 }
 // The stored decisions, if they hold for A: forms in *f.  The index-code
 // decision is kept apart from the streamed forms', so an operator without
@@ -736,8 +776,10 @@ template <typename T> bool stored_forms(const clowns_solver *h, const Op<T> &A, 
     const bool key = stF == force_key(h);
     const bool st_ok = (mask & 3) == 3 || (st >= 0 && tuned_size(A.N, stN) && key);
     const bool ix_ok = (mask & 4) != 0 || (ix >= 0 && tuned_size(A.N, ixN) && key);
-    if (!st_ok || !ix_ok) return false;
-    *f = ((mask & 3) == 3 ? (val & 3) : st) | ((mask & 4) ? (val & 4) : 4 * ix);
+    const bool pc_ok = (mask & 8) != 0 || (h->pc64 >= 0 && tuned_size(A.N, h->pc64_N) && key);
+    if (!st_ok || !ix_ok || !pc_ok) return false;
+    *f = ((mask & 3) == 3 ? (val & 3) : st) | ((mask & 4) ? (val & 4) : 4 * ix) |
+         ((mask & 8) ? (val & 8) : 8 * h->pc64);
     return true;
 }
 
@@ -783,11 +825,11 @@ template <typename T> void tune_stream(clowns_solver *h, Op<T> &A, VecSet<T> &V)
             k_update_vec<T><<<Gv, BS, 0, s>>>(sc, A.N, V.x.p, V.p0.p, V.e.p, V.v.p, V.r.p, V.rh.p, part);
         }
     };
-    float best[8];
+    float best[N_FORMS];
     for (float &b : best) b = 1e30f;
     bool ok = true;
     for (int pass = 0; pass < 2 && ok; ++pass)
-        for (int f = 0; f < 8 && ok; ++f) {
+        for (int f = 0; f < N_FORMS && ok; ++f) {
             if ((f & mask) != fval) continue;
             seq(f);
             seq(f);
@@ -804,7 +846,7 @@ template <typename T> void tune_stream(clowns_solver *h, Op<T> &A, VecSet<T> &V)
         return;
     }
     int f_best = fval;
-    for (int f = 0; f < 8; ++f)
+    for (int f = 0; f < N_FORMS; ++f)
         if (best[f] < best[f_best]) f_best = f;
     if ((mask & 3) != 3) {
         st = f_best & 3;
@@ -814,15 +856,20 @@ template <typename T> void tune_stream(clowns_solver *h, Op<T> &A, VecSet<T> &V)
         ix = (f_best >> 2) & 1;
         ixN = A.N;
     }
+    if (!(mask & 8)) {
+        h->pc64 = (f_best >> 3) & 1;
+        h->pc64_N = A.N;
+    }
     stF = force_key(h);
     if (h->kn.verbose) {
         fprintf(stderr, "clowns_pcg: N=%d %s two iterations (us):", A.N, f32 ? "FP32" : "FP64");
-        for (int f = 0; f < 8; ++f)
+        for (int f = 0; f < N_FORMS; ++f)
             if (best[f] < 1e29f)
-                fprintf(stderr, " [%s%s%s] %.1f", (f & 4) ? "Ix8 " : "", (f & 1) ? "PM-st " : "PM ",
-                        (f & 2) ? "SpMV-st" : "SpMV", 1e3f * best[f]);
-        fprintf(stderr, " -> %s, PM %s, SpMV %s\n", (f_best & 4) ? "8-bit" : "16-bit",
-                (f_best & 1) ? "streamed" : "cached", (f_best & 2) ? "streamed" : "cached");
+                fprintf(stderr, " [%s%s%s%s] %.1f", (f & 4) ? "Ix8 " : "", (f & 8) ? "pat " : "",
+                        (f & 1) ? "PM-st " : "PM ", (f & 2) ? "SpMV-st" : "SpMV", 1e3f * best[f]);
+        fprintf(stderr, " -> %s, PM %s%s, SpMV %s\n", (f_best & 4) ? "8-bit" : "16-bit",
+                (f_best & 1) ? "streamed" : "cached", (f_best & 8) ? " with the pattern table" : "",
+                (f_best & 2) ? "streamed" : "cached");
     }
     set_forms(A, f_best);
 }
@@ -1475,6 +1522,7 @@ extern "C" int clowns_solver_create(clowns_solver **out, int device, void *strea
             k.stream = v < 0 ? -1 : (v == 0 ? 0 : (v == 1 ? 3 : (v == 2 ? 1 : 2)));
         }
         tri("CLOWNS_LL8", k.ll8);
+        tri("CLOWNS_PM_PAT", k.pm_pat);
         flag("CLOWNS_LL16", k.ll16);
         flag("CLOWNS_V4_32", k.v4_32);
         flag("CLOWNS_SPMV_ROWS", k.rows32);
@@ -2629,7 +2677,7 @@ extern "C" int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid
     return CLOWNS_OK;
 }
 
-extern "C" int clowns_solver_tuning(clowns_solver *h, int32_t out[6]) {
+extern "C" int clowns_solver_tuning(clowns_solver *h, int32_t out[8]) {
     if (!h || !out || h->N < 1) return CLOWNS_EBADARG;
     const int N = h->N;
     const int fz = h->kn.fuse_pm;
@@ -2642,6 +2690,8 @@ extern "C" int clowns_solver_tuning(clowns_solver *h, int32_t out[6]) {
     out[3] = f[1] < 0 ? -1 : (f[1] & 3);
     out[4] = f[0] < 0 ? -1 : (f[0] & 4 ? 8 : (h->ll16_ok ? 16 : 32));
     out[5] = f[1] < 0 ? -1 : (f[1] & 4 ? 8 : (h->ll16_ok ? 16 : 32));
+    out[6] = f[0] < 0 ? -1 : (f[0] & 8 ? 1 : 0);
+    out[7] = 0;
     return CLOWNS_OK;
 }
 

@@ -595,10 +595,14 @@ def kernel_names(tune, precision):
         st = tune["stream64"]
         pm, sp = ("true" if st > 0 and st & 1 else "false"), ("true" if st > 0 and st & 2 else "false")
         ix = "Ix%d" % tune["index64"]
-        return {"spmv": f"k_spmv_v2<double,6,0,{ix},{sp}>", "pm_stage": f"k_pm_v2<double,1,0,{ix},{pm},double>",
-                "pm_pair": f"k_pm2_v2<double,{ix},{pm},double>", "pm_dot": f"k_pm_v2<double,2,DOT,{ix},{pm},double>",
-                "p_update": "k_pupd_vec<double>", "e_update": "k_eupd_vec<double>",
-                "xr_update": "k_update_vec<double>", "residual_spmv": f"k_spmv_v2<double,6,3,{ix},{sp}>"}
+        names = {"spmv": f"k_spmv_v2<double,6,0,{ix},{sp}>", "pm_stage": f"k_pm_v2<double,1,0,{ix},{pm},double>",
+                 "pm_pair": f"k_pm2_v2<double,{ix},{pm},double>", "pm_dot": f"k_pm_v2<double,2,DOT,{ix},{pm},double>",
+                 "p_update": "k_pupd_vec<double>", "e_update": "k_eupd_vec<double>",
+                 "xr_update": "k_update_vec<double>", "residual_spmv": f"k_spmv_v2<double,6,3,{ix},{sp}>"}
+        if tune.get("pmpat64") == 1:
+            names["pm_stage"] = f"k_pm_p2<1,0,{ix},{pm}>"
+            names["pm_dot"] = f"k_pm_p2<2,DOT,{ix},{pm}>"
+        return names
     st = tune["stream32"]
     pm, sp = ("true" if st > 0 and st & 1 else "false"), ("true" if st > 0 and st & 2 else "false")
     ix = "Ix%d" % tune["index32"]
@@ -716,11 +720,11 @@ def _short_name(k):
     if k.startswith("_ZN6clowns"):
         k = _demangle_args(k)
     k = k.split("(")[0].replace("void ", "").replace("clowns::", "").replace(" ", "").strip()
-    for pre in ("k_pm_v2<double,2,", "k_pm_v4<2,", "k_pm_v4<1,"):
+    for pre in ("k_pm_v2<double,2,", "k_pm_v4<2,", "k_pm_v4<1,", "k_pm_p2<2,", "k_pm_p2<1,"):
         if k.startswith(pre):
             rest = k[len(pre):]
             dot, _, tail = rest.partition(",")
-            if pre.startswith("k_pm_v4<1,") and dot == "0":
+            if pre.endswith("<1,") and dot == "0":
                 return k
             return pre + "DOT," + tail
     if k.startswith("k_pm_v2<double,1,") and not k.startswith("k_pm_v2<double,1,0,"):
@@ -824,16 +828,19 @@ def main():
     if args.pmc_run:
         # an assembled FP64 solve and a mixed solve in each of the kernel
         # forms the solver may pick here (streamed or cached PM stages and
-        # SpMV, 8- or 16-bit index codes, single or fused PM stages), so the
-        # traffic of whichever form a timed run picks was counted inside a
-        # solve; then the copy kernel the counters are scaled by
+        # SpMV, 8- or 16-bit index codes, single or fused PM stages, FP64 PM
+        # stages with or without the pattern table), so the traffic of
+        # whichever form a timed run picks was counted inside a solve; then
+        # the copy kernel the counters are scaled by
         tunes = []
-        for st, ll8, fuse in itertools.product(("0", "1", "2", "3"), ("0", "1"), ("0", "1")):
-            os.environ["CLOWNS_STREAM"], os.environ["CLOWNS_LL8"], os.environ["CLOWNS_FUSE_PM"] = st, ll8, fuse
+        for st, ll8, fuse, pat in itertools.product(("0", "1", "2", "3"), ("0", "1"), ("0", "1"), ("0", "1")):
+            env = {"CLOWNS_STREAM": st, "CLOWNS_LL8": ll8, "CLOWNS_FUSE_PM": fuse, "CLOWNS_PM_PAT": pat}
+            os.environ.update(env)
             try:
                 s2 = Solver(0)
             finally:
-                del os.environ["CLOWNS_STREAM"], os.environ["CLOWNS_LL8"], os.environ["CLOWNS_FUSE_PM"]
+                for k in env:
+                    del os.environ[k]
             for mixed in (False, True):
                 dg.assemble_into(s2, b=b)
                 x.zero_()

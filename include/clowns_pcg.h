@@ -457,9 +457,11 @@ int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid_dev *g,
  * stages (FP64, FP32: 1 fused, 0 single); out[2], out[3] streamed
  * (non-temporal) arrays (FP64, FP32: bit 0 the PM stages, bit 1 the SpMV);
  * out[4], out[5] the width of the neighbour-index codes (FP64, FP32: 8, 16
- * or 32).  -1 where not yet decided.  No counterpart in the reference
+ * or 32); out[6] 1 if the FP64 PM stages read pattern rows' coefficients
+ * from the row-pattern table (two 128-row chunks a wavefront), else 0;
+ * out[7] 0.  -1 where not yet decided.  No counterpart in the reference
  * (reporting). */
-int clowns_solver_tuning(clowns_solver *h, int32_t out[6]);
+int clowns_solver_tuning(clowns_solver *h, int32_t out[8]);
 
 /* Run the resident operator's gather kernels (SpMV, PM stages) over their
  * 512-row blocks in the given order: d_order[G] in device memory, a

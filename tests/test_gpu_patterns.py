@@ -84,3 +84,59 @@ def test_table_kept_for_the_next_operator():
     assert pats[1] > dg2.N // 2
     assert info["iters"] == ir["iters"] and im["iters"] == imr["iters"]
     assert torch.equal(x, xr) and torch.equal(xm, xmr)
+
+
+def _solve_env(dg, env):
+    import torch
+    from clowns3d import Solver
+    os.environ.update(env)
+    try:
+        sol = Solver(0)  # reads the switches at creation
+    finally:
+        for k in env:
+            del os.environ[k]
+    b = dg.assemble_into(sol, keep_raw=True)
+    x64, i64 = sol.solve(b, eps=1e-10)
+    x32, i32 = sol.solve(b, eps=1e-10, mixed=True)
+    tune = sol.tuning()
+    torch.cuda.synchronize()
+    out = (x64.cpu().numpy(), x32.cpu().numpy()), (i64, i32), tune
+    sol.close()
+    return out
+
+
+@pytest.mark.parametrize("env", [
+    {"CLOWNS_STREAM": "1", "CLOWNS_LL8": "1", "CLOWNS_FUSE_PM": "0"},
+    {"CLOWNS_STREAM": "0", "CLOWNS_LL8": "0", "CLOWNS_FUSE_PM": "0"},
+    {"CLOWNS_STREAM": "2", "CLOWNS_LL8": "1", "CLOWNS_FUSE_PM": "1"},
+    {"CLOWNS_STREAM": "0", "CLOWNS_LL8": "1", "CLOWNS_FUSE_PM": "0", "CLOWNS_BLOCK_ORDER": "0"},
+], ids=["streamed-8bit", "cached-16bit", "pairs-8bit", "natural-order"])
+def test_pattern_pm_stages_bit_identical(env):
+    """The FP64 PM stages with the pattern table, two 128-row chunks a
+    wavefront (k_pm_p2: single stages and the two-term stage with its dots),
+    compute what k_pm_v2 does: FP64 and mixed solves bit-identical with the
+    form forced on and off, in the grid's block order and in the natural one,
+    with 8- and 16-bit codes, streamed and cached; the solver reports it."""
+    from clowns3d import DeviceGrid
+    dg = DeviceGrid(_grid(bathymetry=True))
+    on = _solve_env(dg, dict(env, CLOWNS_PM_PAT="1"))
+    off = _solve_env(dg, dict(env, CLOWNS_PM_PAT="0"))
+    print(f"N={dg.N}: {on[2]} | {off[2]}")
+    assert on[2]["pmpat64"] == 1 and off[2]["pmpat64"] == 0
+    for a, b in zip(on[0], off[0]):
+        assert np.array_equal(a, b)
+    for a, b in zip(on[1], off[1]):
+        assert a["iters"] == b["iters"] and a["ier"] == b["ier"] == 0
+
+
+def test_pattern_pm_stages_timed_choice():
+    """By default the solver times the form with the others (tune_stream) and
+    reports its choice; the solve equals the forced forms'."""
+    from clowns3d import DeviceGrid
+    dg = DeviceGrid(_grid())
+    tm = _solve_env(dg, {})
+    ref = _solve_env(dg, {"CLOWNS_PM_PAT": "0"})
+    print(f"N={dg.N}: timed {tm[2]}")
+    assert tm[2]["pmpat64"] in (0, 1)
+    for a, b in zip(tm[0], ref[0]):
+        assert np.array_equal(a, b)
