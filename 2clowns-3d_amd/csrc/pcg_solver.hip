@@ -235,7 +235,10 @@ struct Knobs {
     // with the pattern table (k_pm_p2): a wavefront's two chunks of loads
     // in flight before its gathers, a pattern row's coefficient from LDS.
     // Bit-identical to k_pm_v2.  tune_stream times it with the other forms
-    // (bit 3); CLOWNS_PM_PAT = 0 off, 1 on, -1 (default) timed.
+    // (bit 3); CLOWNS_PM_PAT = 0 off, 1 on, -1 (default) timed.  (The FP32
+    // four-row stages with the table -- their four coefficients loaded only
+    // when a row has its own -- were slower: two inner iterations 1375
+    // against 1217 us at 13.1M rows; they keep the coefficient columns.)
     int pm_pat = -1;
     // CLOWNS_VERBOSE = 1: report per-operator tuning decisions on stderr.
     bool verbose = false;
