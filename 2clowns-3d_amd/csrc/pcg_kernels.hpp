@@ -9,7 +9,8 @@
 //   k_pm_v2        one stage of LDUSUB (:279-318); the last stage carries the
 //                  dot products that follow it ((r^,q) :157-160,
 //                  (e,v),(v,v) :186-191); k_pm_v4, k_pm2_v2 its FP32
-//                  four-row and fused-pair forms
+//                  four-row and fused-pair forms, k_pm_p2 its FP64 form
+//                  with the row-pattern table
 //   k_pupd_vec     p = r + beta (p - omega q) (:244)
 //   k_eupd_vec     e = r - alpha q (:173)
 //   k_update_vec   x += alpha p + omega e; r = e - omega v; (r,r), (r^,r)
