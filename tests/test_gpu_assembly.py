@@ -1,7 +1,8 @@
 """GPU assembly against the oracle.
 
 clowns_assemble (init_matrix + cal_matrix_element + RHS on the device) must
-equal the oracle's restatement -- itself bit-exact against the flang build of
+equal the oracle's restatement (on the test grids and on configurations 1-3
+at full size, there after the device classification) -- itself bit-exact against the flang build of
 the reference routines -- bit for bit: AL, LL, D, B.  The fused
 clowns_solver_assemble must leave the same scaled/sorted operator as
 assemble -> set_matrix (PCGPME's scaling and sort), and its solve must match
@@ -34,16 +35,26 @@ def test_device_assembly_bit_exact(name):
     assert np.array_equal(B.cpu().numpy(), o["B"][:N])
 
 
-def test_device_assembly_config1():
-    g = configs.config1()
+@pytest.mark.parametrize("config", [1, 2, 3])
+def test_device_assembly_full_size(config):
+    """The benchmark configurations at full size (77,776 / 2.45M / 13.07M
+    rows): classification and row numbering on the GPU, then the assembly,
+    bit for bit against the oracle's."""
+    g = configs.CONFIGS[config]()
     o = build_oracle(g)
     N = o["N"]
-    dg = DeviceGrid(g)
+    sol = Solver(0)
+    dg = DeviceGrid.on_device(g, sol)
+    assert dg.N == N and not dg.last_fallback
+    assert np.array_equal(dg.row2cell.cpu().numpy(), o["row2cell"][:N])
     AL, LL, D, B = dg.assemble()
+    torch.cuda.synchronize()
+    print(f"configuration {config}: N={N}")
     assert np.array_equal(AL.cpu().numpy(), o["AL"][:, :N])
     assert np.array_equal(LL.cpu().numpy(), o["LL"][:, :N])
     assert np.array_equal(D.cpu().numpy(), o["D"][:N])
     assert np.array_equal(B.cpu().numpy(), o["B"][:N])
+    sol.close()
 
 
 @pytest.mark.parametrize("name", ["wavy32x24x16", "bathy28x24x16", "objects40x24x16"])
