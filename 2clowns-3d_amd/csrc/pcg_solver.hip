@@ -1019,7 +1019,6 @@ int run_bicgstab(clowns_solver *h, const Op<T> &A, VecSet<T> &V, Scal *sc, int i
 int check_nl(int NL) { return (NL == 6 || NL == 4) ? CLOWNS_OK : CLOWNS_EBADNL; }
 
 int build_ll16(clowns_solver *h);
-int derive_block_order(clowns_solver *h);
 int build_ll8(clowns_solver *h);
 int build_patterns(clowns_solver *h);
 int ensure_block_order(clowns_solver *h);
@@ -1303,10 +1302,9 @@ int load_matrix(clowns_solver *h, const double *AL, int NL, int lda, const int *
     h->have_t = false;
     h->raw_valid = keep_raw;
     h->will_rev_valid = false;
-    h->order_G = 0;  // a caller's operator: the order derive_block_order finds, or set_block_order's
+    h->order_G = 0;  // a caller's operator: natural order unless set_block_order
     h->order_pending = false;
     int rc2 = build_ll16(h);
-    if (!rc2) rc2 = derive_block_order(h);
     return rc2 ? rc2 : build_patterns(h);
 }
 
@@ -1360,86 +1358,6 @@ int build_ll16(clowns_solver *h) {
     h->ll16_ok = res[0] == 0;
     h->ll16_wide = res[1];
     return build_ll8(h);
-}
-
-// Block order for an operator given as arrays, without its grid (the
-// drop-in, set_matrix): the visit the grid's order makes -- chunks of grid
-// lines, plane by plane -- found from the operator.  In a k-major numbering
-// a row's z+ neighbour is about a plane of rows ahead, which the 16-bit
-// codes' far base records per 512-row block (the smallest offset >= 16384,
-// k_ll16_build), so block b links to the block a plane up.  Following the
-// links from every block no block links to gives columns of blocks through
-// the planes; groups of ORDER_CHUNK such columns (consecutive bottom blocks,
-// as the grid's chunks are consecutive lines) are visited plane by plane.
-// Measured on MI355X at 13.1M rows (FP64 solve of the assembled operator
-// given through set_matrix): 270.0 ms in the natural order, 263.6 ms in
-// this one, 247.0 ms in the grid's.  The columns keep the z neighbours
-// close (median 64 blocks back, as the grid's) but not the y neighbours of
-// a block at a column's edge (the grid's chunks are aligned plane to plane,
-// these drift where the planes' row counts differ).  Only the visiting
-// order changes: results are bit-identical.
-constexpr int ORDER_CHUNK = 64;
-int derive_block_order(clowns_solver *h) {
-    h->order_G = 0;
-    if (!h->kn.block_order || h->NL != 6 || h->N < ORDER_MIN_ROWS || !h->ll16_ok) return CLOWNS_OK;
-    const int N = h->N, nb = cdiv(N, 2 * BS), G = geo_of(N, 2).G;
-    static_assert(LL16_ROWS == 2 * BS, "one code block per row block");
-    std::vector<int2> base(nb);
-    hipStream_t s = h->stream;
-    HIPCHK(hipMemcpyAsync(base.data(), h->base16.p, (size_t)nb * sizeof(int2), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::vector<int> succ(nb, -1);
-    std::vector<char> pred(nb, 0), seen(nb, 0);
-    for (int b = 0; b < nb; ++b) {
-        const long up = base[b].x;  // 0x7fffffff: no offset >= 16384 in the block
-        if (up >= 16384 && up < 0x7fffffff) {
-            const long sb = ((long)b * (2 * BS) + BS + up) / (2 * BS);
-            if (sb > b && sb < nb) {
-                succ[b] = (int)sb;
-                pred[sb] = 1;
-            }
-        }
-    }
-    std::vector<int> ord;
-    ord.reserve(G);
-    std::vector<std::vector<int>> cols;
-    auto flush = [&]() {
-        for (size_t t = 0;; ++t) {
-            bool any = false;
-            for (const auto &c : cols)
-                if (t < c.size()) {
-                    ord.push_back(c[t]);
-                    any = true;
-                }
-            if (!any) break;
-        }
-        cols.clear();
-    };
-    for (int r = 0; r < nb; ++r) {
-        if (pred[r] || seen[r]) continue;
-        std::vector<int> c;
-        for (int b = r; b >= 0 && !seen[b]; b = succ[b]) {
-            seen[b] = 1;
-            c.push_back(b);
-        }
-        cols.push_back(std::move(c));
-        if ((int)cols.size() == ORDER_CHUNK) flush();
-    }
-    flush();
-    for (int b = 0; b < nb; ++b)
-        if (!seen[b]) ord.push_back(b);  // on a link cycle (cannot happen: links go up)
-    for (int b = nb; b < G; ++b) ord.push_back(b);
-    if (h->kn.verbose) {
-        int roots = 0;
-        for (int b = 0; b < nb; ++b) roots += !pred[b];
-        fprintf(stderr, "clowns_pcg: N=%d block order from the operator: %d blocks in %d columns\n", N, nb, roots);
-    }
-    int rc;
-    if ((rc = h->order.ensure(G))) return rc;
-    HIPCHK(hipMemcpyAsync(h->order.p, ord.data(), (size_t)G * sizeof(int), hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
-    h->order_G = G;
-    return CLOWNS_OK;
 }
 
 // The 8-bit form (Ix8), built beside the 16-bit one (which stays the
