@@ -162,11 +162,13 @@ class Solver:
         dict fuse64, fuse32 (1 fused PM pairs), stream64, stream32 (bit 0
         streamed PM stages, bit 1 streamed SpMV), index64, index32 (bits of
         the neighbour-index codes: 8, 16 or 32), pmpat64 (1: FP64 PM stages
-        with the row-pattern table); -1 where undecided."""
+        with the row-pattern table), spmv32 (FP32 SpMV: 0 column-major, 1
+        row-major copies, 2 row-major in two chunks a wavefront, plus 4 with
+        the row-pattern table); -1 where undecided."""
         out = np.zeros(8, np.int32)
         L.check(L.lib().clowns_solver_tuning(self._h, ctypes.c_void_p(out.ctypes.data)), "clowns_solver_tuning")
-        return dict(zip(("fuse64", "fuse32", "stream64", "stream32", "index64", "index32", "pmpat64"),
-                        (int(v) for v in out[:7])))
+        return dict(zip(("fuse64", "fuse32", "stream64", "stream32", "index64", "index32", "pmpat64", "spmv32"),
+                        (int(v) for v in out)))
 
     def solve_will(self, b, out=None, eps=1e-10, iters=1000):
         """PCGPME_WILL on the resident operator (set_matrix): zero start, EPS

@@ -5,7 +5,8 @@
 //   k_scale_sort   row scaling + ghost indices + per-row exchange sort (:77-112)
 //   k_spmv_v2      AXSUB, y = A x in sorted slot order (:323-352), two rows
 //                  a thread; k_spmv_rows32 its FP32 form from row-major
-//                  copies (mixed inner solve)
+//                  copies (mixed inner solve), k_spmv_rows32_p2 that form
+//                  in two chunks a wavefront (8-bit codes)
 //   k_pm_v2        one stage of LDUSUB (:279-318); the last stage carries the
 //                  dot products that follow it ((r^,q) :157-160,
 //                  (e,v),(v,v) :186-191); k_pm_v4, k_pm2_v2 its FP32
@@ -1302,6 +1303,113 @@ k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restri
 #pragma unroll
         for (int k = 0; k < 6; ++k) acc = acc + ALr[(size_t)6 * i + k] * x[LL.col((size_t)k * lda).one(i)];
         y[i] = acc;
+    }
+}
+
+// k_spmv_rows32 with 8-bit codes in the two-chunk form: a wavefront takes
+// 256 rows of one 512-row code block as two chunks of 128 (two rows a lane),
+// and issues both chunks' pattern ids, codes and x before the first chunk's
+// gathers; a workgroup takes two code blocks (grid: geo_half of
+// k_spmv_rows32's).  Same per-row arithmetic, so y is bit-identical.
+template <bool ST, bool PAT>
+__global__ void __launch_bounds__(BS)
+k_spmv_rows32_p2(const Scal *__restrict__ sc, int N, int lda, const float *__restrict__ ALr,
+                 const uint8_t *__restrict__ Lr, Ix8 LL, const float *__restrict__ D, const int *__restrict__ nonunit,
+                 const float *__restrict__ x, float *__restrict__ y, const uint8_t *__restrict__ pid,
+                 const double *__restrict__ ptab) {
+    static_assert(2 * BS == Ix8::ROWS, "half a workgroup is one code block");
+    const int done = sc->done;
+    const bool unit = *nonunit == 0;
+    __shared__ float tab[PAT ? PAT_MAX * PAT_W : 1];
+    __shared__ int T[2][256];
+    double tv = 0.0;
+    if (PAT && threadIdx.x < PAT_MAX * PAT_W) tv = ptab[threadIdx.x];
+    if (done) return;
+    const int q = threadIdx.x >> 6, lane = threadIdx.x & 63, h = q >> 1;
+    const int g = logical_block();
+    const int i0 = (2 * g + h) * Ix8::ROWS + (q & 1) * 256 + 2 * lane;
+    bool full[2];
+    int cr[2][2][6];
+    int pp[2] = {0, 0};
+    float2 xv[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int i = i0 + 128 * c;
+        full[c] = i + 1 < N;
+        if (full[c]) {
+            if (PAT) pp[c] = ldv<ST>(reinterpret_cast<const uint16_t *>(pid + i));
+            row_codes<ST>(Lr, i, cr[c]);
+            xv[c] = ld2(x + i);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const long r0 = (long)(2 * g + b) * Ix8::ROWS;
+        T[b][threadIdx.x] = r0 < N ? Ix8::tab_entry(threadIdx.x, LL.base[2 * g + b]) : 0;
+    }
+    if (PAT && threadIdx.x < PAT_MAX * PAT_W) tab[threadIdx.x] = (float)tv;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int i = i0 + 128 * c;
+        if (full[c]) {
+            const int p[2] = {PAT ? (pp[c] & 255) : PAT_NONE, PAT ? (pp[c] >> 8) : PAT_NONE};
+            float ar[2][6];
+            if (PAT && p[0] < PAT_MAX && p[1] < PAT_MAX) {
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) ar[r][k] = tab[p[r] * PAT_W + k];
+            } else {
+                const f4v *ap = reinterpret_cast<const f4v *>(ALr + (size_t)6 * i);
+                const f4v a0 = ldv<ST>(ap), a1 = ldv<ST>(ap + 1), a2 = ldv<ST>(ap + 2);
+                const float av[12] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k)
+                        ar[r][k] = (PAT && p[r] < PAT_MAX) ? tab[p[r] * PAT_W + k] : av[6 * r + k];
+            }
+            float acc0 = xv[c].x, acc1 = xv[c].y;
+            if (!unit) {
+                const float2 d = lds2<ST>(D + i);
+                acc0 = xv[c].x * d.x;
+                acc1 = xv[c].y * d.y;
+            }
+            int j[2][6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const Ix8 cc = LL.col((size_t)k * lda);
+                j[0][k] = cc.near_t(cr[c][0][k], i, T[h]) + 1;
+                j[1][k] = cc.near_t(cr[c][1][k], i + 1, T[h]) + 1;
+                if (cr[c][0][k] == Ix8::WIDE) j[0][k] = cc.L32[i];
+                if (cr[c][1][k] == Ix8::WIDE) j[1][k] = cc.L32[i + 1];
+            }
+            float gv[2][6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const f2u v = *reinterpret_cast<const f2u *>(x + (j[0][k] - 1));
+                gv[0][k] = v.x;
+                gv[1][k] = v.y;
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if (j[1][k] != j[0][k] + 1) gv[1][k] = x[j[1][k] - 1];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                acc0 = acc0 + ar[0][k] * gv[0][k];
+                acc1 = acc1 + ar[1][k] * gv[1][k];
+            }
+            float2 o;
+            o.x = acc0;
+            o.y = acc1;
+            sts2<ST>(y + i, o);
+        } else if (i < N) {
+            float acc = unit ? x[i] : x[i] * D[i];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc = acc + ALr[(size_t)6 * i + k] * x[LL.col((size_t)k * lda).one(i)];
+            y[i] = acc;
+        }
     }
 }
 

@@ -595,10 +595,11 @@ def kernel_names(tune, precision):
         st = tune["stream64"]
         pm, sp = ("true" if st > 0 and st & 1 else "false"), ("true" if st > 0 and st & 2 else "false")
         ix = "Ix%d" % tune["index64"]
-        names = {"spmv": f"k_spmv_v2<double,6,0,{ix},{sp}>", "pm_stage": f"k_pm_v2<double,1,0,{ix},{pm},double>",
+        pat = "true" if tune.get("spmv32", -1) >= 0 and tune["spmv32"] & 4 else "false"
+        names = {"spmv": f"k_spmv_v2<double,6,0,{ix},{sp},{pat}>", "pm_stage": f"k_pm_v2<double,1,0,{ix},{pm},double>",
                  "pm_pair": f"k_pm2_v2<double,{ix},{pm},double>", "pm_dot": f"k_pm_v2<double,2,DOT,{ix},{pm},double>",
                  "p_update": "k_pupd_vec<double>", "e_update": "k_eupd_vec<double>",
-                 "xr_update": "k_update_vec<double>", "residual_spmv": f"k_spmv_v2<double,6,3,{ix},{sp}>"}
+                 "xr_update": "k_update_vec<double>", "residual_spmv": f"k_spmv_v2<double,6,3,{ix},{sp},{pat}>"}
         if tune.get("pmpat64") == 1:
             names["pm_stage"] = f"k_pm_p2<1,0,{ix},{pm}>"
             names["pm_dot"] = f"k_pm_p2<2,DOT,{ix},{pm}>"
@@ -608,7 +609,15 @@ def kernel_names(tune, precision):
     ix = "Ix%d" % tune["index32"]
     codes = "unsigned char" if ix == "Ix8" else "short"
     c = "_Float16" if os.environ.get("CLOWNS_PM_HALF", "1") != "0" else "float"
-    return {"spmv": f"k_spmv_rows32<{ix},{codes},{sp}>", "pm_stage": f"k_pm_v4<1,0,{ix},{pm},{c}>",
+    form = tune.get("spmv32", 1)
+    pat = "true" if form >= 0 and form & 4 else "false"
+    if form >= 0 and (form & 3) == 2:
+        spmv = f"k_spmv_rows32_p2<{sp},{pat}>"
+    elif form >= 0 and (form & 3) == 0:
+        spmv = f"k_spmv_v2<float,6,0,{ix},{sp},{pat}>"
+    else:
+        spmv = f"k_spmv_rows32<{ix},{codes},{sp},{pat}>"
+    return {"spmv": spmv, "pm_stage": f"k_pm_v4<1,0,{ix},{pm},{c}>",
             "pm_pair": f"k_pm2_v2<float,{ix},{pm},{c}>", "pm_dot": f"k_pm_v4<2,DOT,{ix},{pm},{c}>",
             "p_update": "k_pupd_vec<float>", "e_update": "k_eupd_vec<float>", "xr_update": "k_update_vec<float>"}
 
