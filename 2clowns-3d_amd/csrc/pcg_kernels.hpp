@@ -324,6 +324,10 @@ struct Ix32 {
     typedef NV2<int>::t Raw;
     typedef int Base;
     __device__ __forceinline__ Base blk(int) const { return 0; }
+    // With the code block's base passed in (see Ix16::pair_b).
+    __device__ __forceinline__ Base blk_at(int, int) const { return 0; }
+    template <bool ST = false> __device__ __forceinline__ int2 pair_b(int i, Base) const { return pair<ST>(i); }
+    template <bool ST = false> __device__ __forceinline__ int4 quad_b(int i, Base) const { return quad<ST>(i); }
     template <bool ST = false> __device__ __forceinline__ Raw raw(int i) const { return ldn2<ST>(L + i); }
     __device__ __forceinline__ int2 dec1(Raw v, int, Base) const { return make_int2(v.x, v.y); }
     static constexpr bool TABLE = false;
@@ -367,6 +371,23 @@ struct Ix16 {
         return make_int4(dec(v.x, i, b), dec(v.y, i + 1, b), dec(v.z, i + 2, b), dec(v.w, i + 3, b));
     }
     __device__ __forceinline__ int one(int i) const { return dec(L[i], i, base[i / LL16_ROWS]); }
+    // pair() and quad() with the code block's base passed in: a kernel whose
+    // workgroup (or wavefront) rows lie in one code block reads it once, as
+    // a scalar load at a uniform block index, instead of a per-lane load of
+    // the same element that the stream loads must wait behind.  blk_at
+    // reads block b's base when the block holds rows (b ROWS < N).
+    __device__ __forceinline__ int2 blk_at(int b, int n) const {
+        return (long)b * LL16_ROWS < n ? base[b] : make_int2(0, 0);
+    }
+    template <bool ST = false> __device__ __forceinline__ int2 pair_b(int i, int2 b) const {
+        const NV2<short>::t v = ldn2<ST>(L + i);
+        return make_int2(dec(v.x, i, b), dec(v.y, i + 1, b));
+    }
+    template <bool ST = false> __device__ __forceinline__ int4 quad_b(int i, int2 b) const {
+        typedef short s4 __attribute__((ext_vector_type(4)));
+        const s4 v = ldv<ST>(reinterpret_cast<const s4 *>(L + i));
+        return make_int4(dec(v.x, i, b), dec(v.y, i + 1, b), dec(v.z, i + 2, b), dec(v.w, i + 3, b));
+    }
     // Phased form for kernels that load several slots' codes before using
     // any (k_spmv_v2): raw() the pair's codes, dec1() their indices 1-based
     // (clamped to [1, N + 1]).  A wide slot's 32-bit index is loaded into
@@ -505,6 +526,19 @@ struct Ix8 {
                          dec(v >> 24, i + 3, b));
     }
     __device__ __forceinline__ int one(int i) const { return dec(L[i], i, base[i / LL8_ROWS]); }
+    // With the code block's base passed in, as Ix16's.
+    __device__ __forceinline__ int4 blk_at(int b, int n) const {
+        return (long)b * LL8_ROWS < n ? base[b] : make_int4(0, 0, 0, 0);
+    }
+    template <bool ST = false> __device__ __forceinline__ int2 pair_b(int i, int4 b) const {
+        const uint16_t v = ldv<ST>(reinterpret_cast<const uint16_t *>(L + i));
+        return make_int2(dec(v & 255, i, b), dec(v >> 8, i + 1, b));
+    }
+    template <bool ST = false> __device__ __forceinline__ int4 quad_b(int i, int4 b) const {
+        const unsigned v = ldv<ST>(reinterpret_cast<const unsigned *>(L + i));
+        return make_int4(dec(v & 255, i, b), dec((v >> 8) & 255, i + 1, b), dec((v >> 16) & 255, i + 2, b),
+                         dec(v >> 24, i + 3, b));
+    }
     // Phased form, as Ix16's.
     typedef uint16_t Raw;
     typedef int4 Base;
@@ -654,16 +688,17 @@ k_pm_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
     const int lb = row_block(order);
     if (done) return;
     const int i = (lb * BS + threadIdx.x) * 2;
+    const auto bb = l0.blk_at(lb, N);  // the row block is the code block; one base array for all columns
     double s0 = 0.0, s1 = 0.0;
     if (i + 1 < N) {
         const auto c0 = ldc2<T, ST>(a0 + i);
-        const int2 j0 = l0.template pair<ST>(i);
+        const int2 j0 = l0.template pair_b<ST>(i, bb);
         const auto xi = ld2(in + i);
         T o0 = xi.x - c0.x * in[j0.x];
         T o1 = xi.y - c0.y * in[j0.y];
         if (NT == 2) {
             const auto c1 = ldc2<T, ST>(a1 + i);
-            const int2 j1 = l1.template pair<ST>(i);
+            const int2 j1 = l1.template pair_b<ST>(i, bb);
             o0 = o0 - c1.x * in[j1.x];
             o1 = o1 - c1.y * in[j1.y];
         }
@@ -806,10 +841,12 @@ k_pm_p2(const Scal *__restrict__ sc, int N, int G, const double *__restrict__ a0
     __syncthreads();
     if (done) return;
     const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int blk = 2 * logical_block() + (q >> 1);
+    // wavefront-uniform: the code block's base is one scalar load
+    const int blk = __builtin_amdgcn_readfirstlane(2 * logical_block() + (q >> 1));
     const bool live = blk < G;
     const int rb = live ? (order ? order[blk] : blk) : 0;
     const int i0 = rb * (2 * BS) + (q & 1) * (4 * 64) + 2 * lane;
+    const auto bb = l0.blk_at(rb, N);  // one base array for all columns
     bool full[2];
     uint16_t pp[2];
     typename IX::Raw r0[2], r1[2];
@@ -821,7 +858,7 @@ k_pm_p2(const Scal *__restrict__ sc, int N, int G, const double *__restrict__ a0
         full[c] = live && i + 1 < N;
         if (full[c]) {
             pp[c] = ldv<ST>(reinterpret_cast<const uint16_t *>(pid + i));
-            b0[c] = l0.blk(i);
+            b0[c] = bb;
             r0[c] = l0.template raw<ST>(i);
             if (NT == 2) r1[c] = l1.template raw<ST>(i);
             xv[c] = ld2(in + i);
@@ -950,6 +987,8 @@ k_pm_v4(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0, con
     if (sc->done) return;
     const int lb = logical_block();
     const int i = (lb * BS + threadIdx.x) * 4;
+    // a wavefront's 256 rows lie in one code block: its base is one scalar load
+    const auto bb = l0.blk_at(__builtin_amdgcn_readfirstlane(i / 512), N);
     double s0 = 0.0, s1 = 0.0;
     auto dots = [&](float o, float wi) {
         if (DOT == 1 || DOT == 2) s0 += (double)wi * (double)o;
@@ -961,11 +1000,11 @@ k_pm_v4(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0, con
     int4 j0, j1;
     if (full) {
         c0 = ldc4<ST>(a0 + i);
-        j0 = l0.template quad<ST>(i);
+        j0 = l0.template quad_b<ST>(i, bb);
         x = *reinterpret_cast<const f4v *>(in + i);
         if (NT == 2) {
             c1 = ldc4<ST>(a1 + i);
-            j1 = l1.template quad<ST>(i);
+            j1 = l1.template quad_b<ST>(i, bb);
         }
     }
     if (full) {
@@ -1014,8 +1053,10 @@ k_pm2_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
          const C *__restrict__ a1, IX l1, const T *__restrict__ in, T *__restrict__ out,
          const int *__restrict__ order) {
     const int done = sc->done;  // read together with the block order
-    const int i = (row_block(order) * BS + threadIdx.x) * 2;
+    const int rb = row_block(order);
+    const int i = (rb * BS + threadIdx.x) * 2;
     if (done) return;
+    const auto bb = l0.blk_at(rb, N);  // the row block is the code block
     // stage 1 at row r, and at its stage-2 neighbour m (a ghost m reads 0)
     auto one = [&](T c0, int jj0, T c1, int m, T xr) -> T {
         const T o1i = xr - c0 * in[jj0];
@@ -1025,9 +1066,9 @@ k_pm2_v2(const Scal *__restrict__ sc, int N, const C *__restrict__ a0, IX l0,
     };
     if (i + 1 < N) {
         const auto c0 = ldc2<T, ST>(a0 + i);
-        const int2 j0 = l0.template pair<ST>(i);
+        const int2 j0 = l0.template pair_b<ST>(i, bb);
         const auto c1 = ldc2<T, ST>(a1 + i);
-        const int2 j1 = l1.template pair<ST>(i);
+        const int2 j1 = l1.template pair_b<ST>(i, bb);
         const auto xi = ld2(in + i);
         typename V2<T>::t o;
         o.x = one(c0.x, j0.x, c1.x, j1.x, xi.x);
@@ -1056,19 +1097,19 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
           const double *__restrict__ ptab = nullptr) {
     static_assert(MODE == 0 || MODE == 3, "y = A x or y = v1 - A x");
     static_assert(!PAT || NL == 6, "patterns of six-slot rows");
-    const int done = sc->done;
-    const int i = (row_block(order) * BS + threadIdx.x) * 2;
+    const int rb = row_block(order);
+    const int i = (rb * BS + threadIdx.x) * 2;
     __shared__ T tab[PAT ? PAT_MAX * PAT_W : 1];
-    double tv = 0.0;  // converted when staged, so nothing waits for it here
-    if (PAT && threadIdx.x < PAT_MAX * PAT_W) tv = ptab[threadIdx.x];
-    if (done) return;
+    // The row streams are issued before the run's done flag is read: a
+    // launch queued past convergence reads them and returns unused, every
+    // other launch has them in flight while the flag and the table load.
     const bool full = i + 1 < N;
     typename IX::Raw raw[NL];
     typename IX::Base bb;
     typename V2<T>::t xv;
     int p0 = PAT_NONE, p1 = PAT_NONE;
     if (full) {
-        bb = LL.blk(i);
+        if constexpr (!IX::TABLE) bb = LL.blk_at(rb, N);  // the row block is the code block
 #pragma unroll
         for (int k = 0; k < NL; ++k) raw[k] = LL.col((size_t)k * lda).template raw<ST>(i);
         if (PAT) {
@@ -1078,12 +1119,16 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
         }
         xv = ld2(x + i);
     }
+    const int done = sc->done;
+    double tv = 0.0;  // converted when staged, so nothing waits for it here
+    if (PAT && threadIdx.x < PAT_MAX * PAT_W) tv = ptab[threadIdx.x];
+    if (done) return;
     // Ix8: the block's decode table (its 2 * BS rows are one code block)
     __shared__ int DT[IX::TABLE ? 256 : 1];
     if constexpr (IX::TABLE) {
+        // the row block is the code block (uniform: one scalar load of its base)
         static_assert(2 * BS == IX::ROWS, "a block is one code block");
-        const int blk = (int)(i / IX::ROWS);
-        DT[threadIdx.x] = (long)blk * IX::ROWS < N ? IX::tab_entry(threadIdx.x, LL.base[blk]) : 0;
+        DT[threadIdx.x] = (long)rb * IX::ROWS < N ? IX::tab_entry(threadIdx.x, LL.base[rb]) : 0;
     }
     if (PAT || IX::TABLE) {
         if (PAT && threadIdx.x < PAT_MAX * PAT_W) tab[threadIdx.x] = (T)tv;
@@ -1214,7 +1259,8 @@ k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restri
     double tv = 0.0;  // converted when staged, so nothing waits for it here
     if (PAT && threadIdx.x < PAT_MAX * PAT_W) tv = ptab[threadIdx.x];
     if (done) return;
-    const int i = (logical_block() * BS + threadIdx.x) * 2;
+    const int lb = logical_block();  // the row block is the code block (2 BS rows)
+    const int i = (lb * BS + threadIdx.x) * 2;
     const bool full = i + 1 < N;
     int cr[2][6];
     int p[2] = {PAT_NONE, PAT_NONE};
@@ -1227,15 +1273,14 @@ k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restri
             p[1] = pp >> 8;
         }
         row_codes<ST>(Lr, i, cr);
-        b = LL.base[i / IX::ROWS];
+        if constexpr (!IX::TABLE) b = LL.blk_at(lb, N);
         xv = ld2(x + i);
     }
     // Ix8: the block's decode table (its 2 * BS rows are one code block)
     __shared__ int T[IX::TABLE ? 256 : 1];
     if constexpr (IX::TABLE) {
         static_assert(2 * BS == IX::ROWS, "a block is one code block");
-        const int blk = (int)(i / IX::ROWS);
-        T[threadIdx.x] = (long)blk * IX::ROWS < N ? IX::tab_entry(threadIdx.x, LL.base[blk]) : 0;
+        T[threadIdx.x] = (long)lb * IX::ROWS < N ? IX::tab_entry(threadIdx.x, LL.base[lb]) : 0;
     }
     if (PAT || IX::TABLE) {
         if (PAT && threadIdx.x < PAT_MAX * PAT_W) tab[threadIdx.x] = (float)tv;

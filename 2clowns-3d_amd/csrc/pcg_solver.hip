@@ -1538,9 +1538,14 @@ Op<float> op32(const clowns_solver *h) {
 // reference's zeroed work arrays do (pcgpme_para.F90:59-75): a gather through
 // a padding slot (LL = 0 -> N+1+mod(I,64), :80) then adds AL * 0 even where a
 // caller's AL is not 0 there.  Rows past N keep whatever a larger earlier
-// solve left, so the ghost rows are cleared whenever N changes.
+// solve left, so the ghost rows are cleared whenever N changes.  VEC_SLACK
+// entries more follow the ghost rows: the FP32 SpMV gathers a row pair's two
+// neighbours with one 8-byte load (k_spmv_rows32), which through the last
+// ghost row reads the entry after it (its value is not used: the second row
+// of that pair has another neighbour and loads it on its own).
+constexpr int VEC_SLACK = 4;
 template <typename T> int ensure_vecs(clowns_solver *h, VecSet<T> &V, int N) {
-    int rc = V.ensure((size_t)n_pad(N));
+    int rc = V.ensure((size_t)n_pad(N) + VEC_SLACK);
     if (rc) return rc;
     if (V.padN != N) {
         T *all[] = {V.r.p, V.rh.p, V.p0.p, V.q.p, V.e.p, V.v.p, V.y.p, V.t.p, V.x.p};
@@ -1855,11 +1860,11 @@ extern "C" int clowns_solver_solve(clowns_solver *h, const double *d_b, double *
 // ----------------------------------------------------- allocation-free solve
 // The work memory of clowns_solver_solve_noalloc: a file-scope device array
 // (every GPU the library runs on holds its own copy until the process ends)
-// for the runs' vectors (nine FP64 and nine FP32 ones of N + 64 entries),
-// the scaled right-hand side, the dot partials and the run scalars, for up
-// to NOALLOC_MAX_ROWS rows: 58 MiB.
+// for the runs' vectors (nine FP64 and nine FP32 ones of N + 64 entries and
+// their slack, each 256-byte aligned), the scaled right-hand side, the dot
+// partials and the run scalars, for up to NOALLOC_MAX_ROWS rows: 58 MiB.
 constexpr int NOALLOC_MAX_ROWS = 1 << 19;
-constexpr size_t NA_VEC = (size_t)NOALLOC_MAX_ROWS + PAD;                  // entries per vector
+constexpr size_t NA_VEC = (size_t)NOALLOC_MAX_ROWS + PAD + 64;             // entries per vector
 constexpr size_t NA_G1 = (((size_t)NOALLOC_MAX_ROWS + BS - 1) / BS + 7) / 8 * 8;  // largest grid
 constexpr size_t NA_V64 = 0;                                                 // 9 FP64 vectors + bs
 constexpr size_t NA_V32 = NA_V64 + 10 * NA_VEC * sizeof(double);             // 9 FP32 vectors
