@@ -1097,12 +1097,13 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
           const double *__restrict__ ptab = nullptr) {
     static_assert(MODE == 0 || MODE == 3, "y = A x or y = v1 - A x");
     static_assert(!PAT || NL == 6, "patterns of six-slot rows");
+    const int done = sc->done;
     const int rb = row_block(order);
     const int i = (rb * BS + threadIdx.x) * 2;
     __shared__ T tab[PAT ? PAT_MAX * PAT_W : 1];
-    // The row streams are issued before the run's done flag is read: a
-    // launch queued past convergence reads them and returns unused, every
-    // other launch has them in flight while the flag and the table load.
+    double tv = 0.0;  // converted when staged, so nothing waits for it here
+    if (PAT && threadIdx.x < PAT_MAX * PAT_W) tv = ptab[threadIdx.x];
+    if (done) return;
     const bool full = i + 1 < N;
     typename IX::Raw raw[NL];
     typename IX::Base bb;
@@ -1119,10 +1120,6 @@ k_spmv_v2(const Scal *__restrict__ sc, int N, int lda, const T *__restrict__ AL,
         }
         xv = ld2(x + i);
     }
-    const int done = sc->done;
-    double tv = 0.0;  // converted when staged, so nothing waits for it here
-    if (PAT && threadIdx.x < PAT_MAX * PAT_W) tv = ptab[threadIdx.x];
-    if (done) return;
     // Ix8: the block's decode table (its 2 * BS rows are one code block)
     __shared__ int DT[IX::TABLE ? 256 : 1];
     if constexpr (IX::TABLE) {
