@@ -1348,28 +1348,35 @@ k_spmv_rows32(const Scal *__restrict__ sc, int N, int lda, const float *__restri
     }
 }
 
-// k_spmv_rows32 with 8-bit codes in the two-chunk form: a wavefront takes
-// 256 rows of one 512-row code block as two chunks of 128 (two rows a lane),
-// and issues both chunks' pattern ids, codes and x before the first chunk's
-// gathers; a workgroup takes two code blocks (grid: geo_half of
-// k_spmv_rows32's).  Same per-row arithmetic, so y is bit-identical.
-template <bool ST, bool PAT>
+// k_spmv_rows32 in the two-chunk form: a wavefront takes 256 rows of one
+// 512-row code block as two chunks of 128 (two rows a lane), and issues both
+// chunks' pattern ids, codes and x before the first chunk's gathers; a
+// workgroup takes two code blocks (grid: geo_half of k_spmv_rows32's).  IX:
+// Ix8 with Lr8 (the blocks' decode tables in LDS) or Ix16 with Lr16 (the
+// blocks' bases).  Same per-row arithmetic, so y is bit-identical.
+template <typename IX, typename CT, bool ST, bool PAT>
 __global__ void __launch_bounds__(BS)
 k_spmv_rows32_p2(const Scal *__restrict__ sc, int N, int lda, const float *__restrict__ ALr,
-                 const uint8_t *__restrict__ Lr, Ix8 LL, const float *__restrict__ D, const int *__restrict__ nonunit,
+                 const CT *__restrict__ Lr, IX LL, const float *__restrict__ D, const int *__restrict__ nonunit,
                  const float *__restrict__ x, float *__restrict__ y, const uint8_t *__restrict__ pid,
                  const double *__restrict__ ptab) {
-    static_assert(2 * BS == Ix8::ROWS, "half a workgroup is one code block");
+    static_assert(2 * BS == IX::ROWS, "half a workgroup is one code block");
     const int done = sc->done;
     const bool unit = *nonunit == 0;
     __shared__ float tab[PAT ? PAT_MAX * PAT_W : 1];
-    __shared__ int T[2][256];
+    __shared__ int T[IX::TABLE ? 2 : 1][IX::TABLE ? 256 : 1];
     double tv = 0.0;
     if (PAT && threadIdx.x < PAT_MAX * PAT_W) tv = ptab[threadIdx.x];
     if (done) return;
     const int q = threadIdx.x >> 6, lane = threadIdx.x & 63, h = q >> 1;
     const int g = logical_block();
-    const int i0 = (2 * g + h) * Ix8::ROWS + (q & 1) * 256 + 2 * lane;
+    const int i0 = (2 * g + h) * IX::ROWS + (q & 1) * 256 + 2 * lane;
+    // Ix16: the wavefront's code block base (one of the workgroup's two)
+    std::remove_cv_t<std::remove_reference_t<decltype(*LL.base)>> bw;
+    if constexpr (!IX::TABLE) {
+        const auto b0 = LL.blk_at(2 * g, N), b1 = LL.blk_at(2 * g + 1, N);
+        bw = h ? b1 : b0;
+    }
     bool full[2];
     int cr[2][2][6];
     int pp[2] = {0, 0};
@@ -1384,13 +1391,15 @@ k_spmv_rows32_p2(const Scal *__restrict__ sc, int N, int lda, const float *__res
             xv[c] = ld2(x + i);
         }
     }
+    if constexpr (IX::TABLE) {
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        const long r0 = (long)(2 * g + b) * Ix8::ROWS;
-        T[b][threadIdx.x] = r0 < N ? Ix8::tab_entry(threadIdx.x, LL.base[2 * g + b]) : 0;
+        for (int b = 0; b < 2; ++b) {
+            const long r0 = (long)(2 * g + b) * IX::ROWS;
+            T[b][threadIdx.x] = r0 < N ? IX::tab_entry(threadIdx.x, LL.base[2 * g + b]) : 0;
+        }
     }
     if (PAT && threadIdx.x < PAT_MAX * PAT_W) tab[threadIdx.x] = (float)tv;
-    __syncthreads();
+    if (PAT || IX::TABLE) __syncthreads();
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const int i = i0 + 128 * c;
@@ -1421,11 +1430,16 @@ k_spmv_rows32_p2(const Scal *__restrict__ sc, int N, int lda, const float *__res
             int j[2][6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                const Ix8 cc = LL.col((size_t)k * lda);
-                j[0][k] = cc.near_t(cr[c][0][k], i, T[h]) + 1;
-                j[1][k] = cc.near_t(cr[c][1][k], i + 1, T[h]) + 1;
-                if (cr[c][0][k] == Ix8::WIDE) j[0][k] = cc.L32[i];
-                if (cr[c][1][k] == Ix8::WIDE) j[1][k] = cc.L32[i + 1];
+                const IX cc = LL.col((size_t)k * lda);
+                if constexpr (IX::TABLE) {
+                    j[0][k] = cc.near_t(cr[c][0][k], i, T[h]) + 1;
+                    j[1][k] = cc.near_t(cr[c][1][k], i + 1, T[h]) + 1;
+                } else {
+                    j[0][k] = cc.near1(cr[c][0][k], i, bw);
+                    j[1][k] = cc.near1(cr[c][1][k], i + 1, bw);
+                }
+                if (cr[c][0][k] == IX::WIDE) j[0][k] = cc.L32[i];
+                if (cr[c][1][k] == IX::WIDE) j[1][k] = cc.L32[i + 1];
             }
             float gv[2][6];
 #pragma unroll

@@ -219,11 +219,11 @@ struct Knobs {
     // FP32 kernels keep 16-bit codes.  CLOWNS_SPMV_ROWS = 0 keeps the
     // column-major operator.
     bool rows32 = true;
-    // With 8-bit codes, k_spmv_rows32 in the two-chunk form
-    // (k_spmv_rows32_p2): both chunks' pattern ids, codes and x in flight
-    // before the first chunk's gathers.  Bit-identical.  Measured on MI355X
-    // at 13.1M rows: 66.4 -> 58.7 us (cached), 75.5 -> 69.3 us (streamed).
-    // CLOWNS_SPMV32_P2 = 0 keeps one chunk.
+    // k_spmv_rows32 in the two-chunk form (k_spmv_rows32_p2), with 8- or
+    // 16-bit codes: both chunks' pattern ids, codes and x in flight before
+    // the first chunk's gathers.  Bit-identical.  Measured on MI355X at 13.1M
+    // rows, 8-bit codes: 66.4 -> 58.7 us (cached), 75.5 -> 69.3 us
+    // (streamed).  CLOWNS_SPMV32_P2 = 0 keeps one chunk.
     bool spmv32_p2 = true;
     // The FP32 inner solve of the mixed form applies the PM preconditioner
     // with half-precision coefficients (k_to_f16): the outer FP64 loop
@@ -699,17 +699,15 @@ void launch_spmv_v2_st(hipStream_t s, const Scal *sc, const Op<T> &A, IX ix, int
 template <typename IX, typename CT, bool ST>
 void launch_rows32_st(hipStream_t s, int G2, const Scal *sc, const Op<float> &A, IX ix, const CT *Lr,
                       const float *x, float *y) {
-    if constexpr (std::is_same<IX, Ix8>::value) {
-        if (A.p2_32) {
-            const int W = geo_half(G2);
-            if (A.pid)
-                klaunch(k_spmv_rows32_p2<ST, true>, W, BS, s, sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y,
-                        A.pid, A.ptab);
-            else
-                klaunch(k_spmv_rows32_p2<ST, false>, W, BS, s, sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y,
-                        nullptr, nullptr);
-            return;
-        }
+    if (A.p2_32) {
+        const int W = geo_half(G2);
+        if (A.pid)
+            klaunch(k_spmv_rows32_p2<IX, CT, ST, true>, W, BS, s, sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x,
+                    y, A.pid, A.ptab);
+        else
+            klaunch(k_spmv_rows32_p2<IX, CT, ST, false>, W, BS, s, sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x,
+                    y, nullptr, nullptr);
+        return;
     }
     if (A.pid)
         klaunch(k_spmv_rows32<IX, CT, ST, true>, G2, BS, s, sc, A.N, A.lda, A.ALr, Lr, ix, A.D, A.nonunit, x, y, A.pid,
@@ -2963,7 +2961,7 @@ extern "C" int clowns_solver_tuning(clowns_solver *h, int32_t out[8]) {
     out[5] = f[1] < 0 ? -1 : (f[1] & 4 ? 8 : (h->ll16_ok ? 16 : 32));
     out[6] = f[0] < 0 ? -1 : (f[0] & 8 ? 1 : 0);
     out[7] = f[1] < 0 ? -1
-                      : (!h->have_rows ? 0 : ((f[1] & 4) && h->have_rows8 && h->kn.spmv32_p2 ? 2 : 1)) |
+                      : (!h->have_rows ? 0 : (h->kn.spmv32_p2 ? 2 : 1)) |
                             (h->pat_ok ? 4 : 0);
     return CLOWNS_OK;
 }

@@ -612,7 +612,7 @@ def kernel_names(tune, precision):
     form = tune.get("spmv32", 1)
     pat = "true" if form >= 0 and form & 4 else "false"
     if form >= 0 and (form & 3) == 2:
-        spmv = f"k_spmv_rows32_p2<{sp},{pat}>"
+        spmv = f"k_spmv_rows32_p2<{ix},{codes},{sp},{pat}>"
     elif form >= 0 and (form & 3) == 0:
         spmv = f"k_spmv_v2<float,6,0,{ix},{sp},{pat}>"
     else:

@@ -460,7 +460,7 @@ int clowns_solver_vof_time_kernel(clowns_solver *h, const clowns_grid_dev *g,
  * or 32); out[6] 1 if the FP64 PM stages read pattern rows' coefficients
  * from the row-pattern table (two 128-row chunks a wavefront), else 0;
  * out[7] the FP32 SpMV's form: 0 column-major, 1 row-major copies, 2
- * row-major in two 128-row chunks a wavefront (8-bit codes), plus 4 when it
+ * row-major in two 128-row chunks a wavefront, plus 4 when it
  * reads pattern rows' coefficients from the row-pattern table.  -1 where
  * not yet decided.  No counterpart in the reference (reporting). */
 int clowns_solver_tuning(clowns_solver *h, int32_t out[8]);
