@@ -89,11 +89,13 @@ def parse():
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--pmc-run", default=None, metavar="DIR",
-                    help="one FP64 and one mixed solve plus the copy kernel, for rocprofv3 --pmc (row count to DIR)")
+                    help="solves in every kernel form plus the copy kernel, for rocprofv3 --pmc (row count to DIR); "
+                         "with --will, PCGPME_WILL solves")
     ap.add_argument("--traffic", nargs=2, default=None, metavar=("FETCH_DIR", "WRITE_DIR"),
-                    help="write profiles/traffic_config<n>.json from two rocprofv3 --pmc runs of --pmc-run")
+                    help="write profiles/traffic_config<n>.json (traffic_will<NXxNY>.json with --will) from two "
+                         "rocprofv3 --pmc runs of --pmc-run")
     ap.add_argument("--profile-kernels", action="store_true",
-                    help="--will/--vof: only run the timed kernel loops (for rocprofv3 --pmc)")
+                    help="--vof: only run the timed kernel loops (for rocprofv3 --pmc)")
     ap.add_argument("--reference-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--trace-summary", default=None, metavar="CSV",
                     help="print a markdown summary of a rocprofv3 --kernel-trace CSV (working launches)")
@@ -109,6 +111,9 @@ def parse():
 # (24 each) plus the two dot vectors read by the last stage of each (16),
 # p update (32), x/r update with the dots (56).
 WILL_ITER_BYTES = 56 + 72 + 8 * 24 + 16 + 32 + 56
+# per-launch algorithmic bytes per row of PCGPME_WILL's kernel classes (NL = 4)
+WILL_CLASS_BYTES = {"spmv": 56, "pm_stage": 24, "pm_pair": 32, "pm_dot": 32, "p_update": 32, "e_update": 24,
+                    "xr_update": 56}
 
 
 def will_cpu_baseline(s, threads, eps, iters):
@@ -176,14 +181,31 @@ def bench_will(args):
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     sol = Solver(0)
     sol.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
-    if args.profile_kernels:
+    if args.pmc_run:
+        # solves in each kernel form the solver may pick for this operator
+        # (streamed or cached arrays, 8- or 16-bit codes, single or fused
+        # PM stages), then the copy kernel the counters are scaled by
+        tunes = []
+        for st, ll8, fuse in itertools.product(("0", "1", "2", "3"), ("0", "1"), ("0", "1")):
+            env = {"CLOWNS_STREAM": st, "CLOWNS_LL8": ll8, "CLOWNS_FUSE_PM": fuse}
+            os.environ.update(env)
+            try:
+                s2 = Solver(0)
+            finally:
+                for k in env:
+                    del os.environ[k]
+            s2.set_matrix(t(s["AL"].T), t(s["LL"].T), t(s["D"]))
+            s2.solve_will(t(s["B"]), eps=args.eps, iters=1000)
+            tunes.append(s2.tuning())
+            s2.close()
+            progress(f"counted forms {tunes[-1]}")
         sol.solve_will(t(s["B"]), eps=args.eps, iters=1000)
-        ix = "Ix%d" % sol.index_bits()
-        st = sol.tuning()["stream64"]
-        sp, sm = ("true" if st > 0 and st & 2 else "false"), ("true" if st > 0 and st & 1 else "false")
-        res = {k: sol.time_kernel(i, 10) * 1e3 for k, i in
-               ((f"k_spmv_v2<double,4,0,{ix},{sp}>", 0), (f"k_pm_v2<double,1,0,{ix},{sm},double>", 1), ("k_copy<double>", 4))}
-        print(json.dumps({"rows": N, "avg_us": res}))
+        sol.time_kernel(4, 3)
+        torch.cuda.synchronize()
+        os.makedirs(args.pmc_run, exist_ok=True)
+        with open(os.path.join(args.pmc_run, "rows.json"), "w") as fh:
+            json.dump({"rows": N, "forms": tunes}, fh)
+        print(json.dumps({"rows": N, "forms": tunes}))
         return
     rng = np.random.default_rng(7)
     # a new right-hand side per step: the base PSI scaled, as the surface evolves
@@ -205,24 +227,22 @@ def bench_will(args):
         t1 = time.perf_counter()
         ier = clowns3d.pcgpme_will(c["AL"], 4, c["NA"], c["LL"], c["D"], N, 64, c["B"], args.eps, 5.0, 1000)
         t_drop = time.perf_counter() - t1
-    # dominant kernel, HIP events on the solver's stream
-    # per iteration: 2 SpMV and two NL = 4 PM applications, as 8 single
-    # stages or (fused) 2 pairs + 4 single stages, whichever the solver picked
-    t_spmv = sol.time_kernel(0, 50)
-    t_pm = sol.time_kernel(1, 50)
-    t_pair = sol.time_kernel(5, 50)
+    # the dominant kernel class of a solve, from the in-solve profile (each
+    # dispatch stamped by a pair of HIP events on the solver's stream)
+    sol.kernel_profile(True)
+    sol.solve_will(rhs[0], out, eps=args.eps, iters=1000)
+    prof = sol.kernel_profile(False)["fp64"]
     t_copy = sol.time_kernel(4, 50)
     ix = "Ix%d" % sol.index_bits()
     tune = sol.tuning()
     st = tune["stream64"]
     sp, sm = ("true" if st > 0 and st & 2 else "false"), ("true" if st > 0 and st & 1 else "false")
-    cands = [(2 * t_spmv, f"k_spmv_v2<double,4,0,{ix},{sp}>", t_spmv, 56 * N)]
-    if tune["fuse64"] == 1:
-        cands.append((2 * t_pair, f"k_pm2_v2<double,{ix},{sm},double>", t_pair, ALG_BYTES["pm_pair"] * N))
-        cands.append((4 * t_pm, f"k_pm_v2<double,1,0,{ix},{sm},double>", t_pm, ALG_BYTES["pm_stage"] * N))
-    else:
-        cands.append((8 * t_pm, f"k_pm_v2<double,1,0,{ix},{sm},double>", t_pm, ALG_BYTES["pm_stage"] * N))
-    _, kname, kms, kbytes = max(cands)
+    names = {"spmv": f"k_spmv_v2<double,4,0,{ix},{sp},false>", "pm_stage": f"k_pm_v2<double,1,0,{ix},{sm},double>",
+             "pm_pair": f"k_pm2_v2<double,{ix},{sm},double>", "pm_dot": f"k_pm_v2<double,1,DOT,{ix},{sm},double>",
+             "p_update": "k_pupd_vec<double>", "e_update": "k_eupd_vec<double>", "xr_update": "k_update_vec<double>"}
+    cls = max((c for c in WILL_CLASS_BYTES if prof[c][0] > 0), key=lambda c: prof[c][1])
+    n_l, ms_l = prof[cls]
+    kname, kms, kbytes = names[cls], ms_l / n_l, WILL_CLASS_BYTES[cls] * N
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = None
     pmc = os.path.join(ROOT, "profiles", f"traffic_will{nx}x{ny}.json")
@@ -246,8 +266,9 @@ def bench_will(args):
                    "solve_alg_frac": solve_gbs / PEAK_HBM_GBS,
                    "drop_in_ms_host_arrays": t_drop * 1e3, "drop_in_ier": ier},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "kernel": kname,
-                     "alg_bytes_per_launch": kbytes, "avg_us": kms * 1e3,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "kernel": kname, "kernel_class": cls,
+                     "launches": n_l, "alg_bytes_per_launch": kbytes, "avg_us": kms * 1e3,
+                     "in_solve_us": {c: round(v[1] * 1e3 / v[0], 2) for c, v in prof.items() if v[0] > 0},
                      "copy_GBps": 16 * N / (t_copy * 1e-3) / 1e9},
         "cpu_baseline": cpu,
     }
@@ -804,8 +825,8 @@ def main():
         fetch_dir, write_dir = args.traffic
         with open(os.path.join(fetch_dir, "rows.json")) as fh:
             rows = json.load(fh)["rows"]
-        out = traffic_from_pmc(fetch_dir, write_dir, rows,
-                               os.path.join(ROOT, "profiles", f"traffic_config{args.config}.json"))
+        name = f"traffic_will{args.will.lower()}.json" if args.will else f"traffic_config{args.config}.json"
+        out = traffic_from_pmc(fetch_dir, write_dir, rows, os.path.join(ROOT, "profiles", name))
         print(json.dumps(out))
         return
     if args.will:
