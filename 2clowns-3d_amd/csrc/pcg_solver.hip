@@ -318,6 +318,7 @@ struct clowns_solver {
     int bs_padN = -1;      // row count whose ghost rows of bs were last zeroed
     DevBuf<double> stage_b, stage_x;  // clowns_pcgpme host staging
     struct Stager *stager = nullptr;  // pinned chunks for the drop-in's host arrays (ensure_stager)
+    struct Stager *stager_out = nullptr;  // ... and for its outputs copied during the solve
     DevBuf<double> part;   // per-block partial sums
     DevBuf<int> flag;      // device error flag
     DevBuf<int> will_flags;  // PCGPME_WILL output pass: per-round change flags / counts
@@ -1061,6 +1062,9 @@ struct Stager {
     char *pdst = nullptr;           // device-to-host chunk waiting in pin[pb]
     size_t plen = 0;
     int pb = -1;
+    bool pooled = true;             // host copies over the host threads (else the calling thread's)
+    hipStream_t cs = nullptr;       // a copy stream of its own (default priority), or null
+    hipEvent_t ready = nullptr;     // with cs: what cs waits for before its first copy
 };
 
 namespace {
@@ -1163,42 +1167,60 @@ bool any_nonzero(const double *x, size_t n) {
     return found.load();
 }
 
-int ensure_stager(clowns_solver *h) {
-    if (h->stager) return CLOWNS_OK;
+void free_stager(Stager *st) {
+    for (int b = 0; b < 2; ++b) {
+        if (st->busy[b]) (void)hipEventSynchronize(st->ev[b]);
+        if (st->pin[b]) (void)hipHostFree(st->pin[b]);
+        if (st->ev[b]) (void)hipEventDestroy(st->ev[b]);
+    }
+    if (st->ready) (void)hipEventDestroy(st->ready);
+    if (st->cs) (void)hipStreamDestroy(st->cs);
+    delete st;
+}
+
+// The stager in `slot`, made on first use.  pooled: host copies over the
+// host threads; own_stream: a copy stream of its own (for copies that run
+// beside the solver's stream).
+int ensure_stager(Stager *&slot, bool pooled, bool own_stream) {
+    if (slot) return CLOWNS_OK;
     Stager *st = new (std::nothrow) Stager();
     if (!st) return CLOWNS_ENOMEM;
-    for (int b = 0; b < 2; ++b) {
-        if (hipHostMalloc((void **)&st->pin[b], Stager::CHUNK) != hipSuccess ||
-            hipEventCreateWithFlags(&st->ev[b], hipEventDisableTiming) != hipSuccess) {
-            for (int c = 0; c < 2; ++c) {
-                if (st->pin[c]) (void)hipHostFree(st->pin[c]);
-                if (st->ev[c]) (void)hipEventDestroy(st->ev[c]);
-            }
-            delete st;
-            return CLOWNS_ENOMEM;
-        }
+    st->pooled = pooled;
+    bool ok = true;
+    for (int b = 0; b < 2 && ok; ++b)
+        ok = hipHostMalloc((void **)&st->pin[b], Stager::CHUNK) == hipSuccess &&
+             hipEventCreateWithFlags(&st->ev[b], hipEventDisableTiming) == hipSuccess;
+    if (ok && own_stream)
+        ok = hipStreamCreateWithFlags(&st->cs, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&st->ready, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        free_stager(st);
+        return CLOWNS_ENOMEM;
     }
-    h->stager = st;
+    slot = st;
     return CLOWNS_OK;
 }
 
 void release_stager(clowns_solver *h) {
-    Stager *st = h->stager;
-    if (!st) return;
-    for (int b = 0; b < 2; ++b) {
-        if (st->busy[b]) (void)hipEventSynchronize(st->ev[b]);
-        (void)hipHostFree(st->pin[b]);
-        (void)hipEventDestroy(st->ev[b]);
-    }
-    delete st;
-    h->stager = nullptr;
+    for (Stager **slot : {&h->stager, &h->stager_out})
+        if (*slot) {
+            free_stager(*slot);
+            *slot = nullptr;
+        }
+}
+
+void stage_copy(const Stager *st, char *dst, const char *src, size_t n) {
+    if (st->pooled)
+        host_pool().copy(dst, src, n);
+    else
+        std::memcpy(dst, src, n);
 }
 
 // Drains the device-to-host chunk still in a pinned buffer.
 int stage_finish(Stager *st) {
     if (st->pb < 0) return CLOWNS_OK;
     HIPCHK(hipEventSynchronize(st->ev[st->pb]));
-    host_pool().copy(st->pdst, st->pin[st->pb], st->plen);
+    stage_copy(st, st->pdst, st->pin[st->pb], st->plen);
     st->pb = -1;
     return CLOWNS_OK;
 }
@@ -1211,7 +1233,7 @@ int stage_h2d(Stager *st, hipStream_t s, void *dst, const void *src, size_t n) {
         const size_t len = std::min(Stager::CHUNK, n - off);
         const int b = st->next++ & 1;
         if (st->busy[b]) HIPCHK(hipEventSynchronize(st->ev[b]));
-        host_pool().copy(st->pin[b], (const char *)src + off, len);
+        stage_copy(st, st->pin[b], (const char *)src + off, len);
         HIPCHK(hipMemcpyAsync((char *)dst + off, st->pin[b], len, hipMemcpyHostToDevice, s));
         HIPCHK(hipEventRecord(st->ev[b], s));
         st->busy[b] = true;
@@ -2021,15 +2043,37 @@ extern "C" void clowns_pcgpme(double *AL, const int32_t *NL_, const int32_t *NA_
     const auto t0 = clk::now();
     const bool warm = any_nonzero(X, (size_t)N);
     hipStream_t s = h->stream;
-    int rc = ensure_stager(h);
-    Stager *st = h->stager;
+    int rc = ensure_stager(h->stager, true, false);
+    if (!rc) rc = ensure_stager(h->stager_out, false, true);
+    Stager *st = h->stager, *so = h->stager_out;
     const auto t1 = clk::now();
     if (!rc) rc = load_matrix(h, AL, NL, NA, LL, D, N, *PARM, hipMemcpyHostToDevice, false, st);
     const auto t2 = clk::now();
+    // The reference leaves AL, D scaled and LL sorted (pcgpme_para.F90:77-112),
+    // final once the matrix is set up: a host thread copies them out on the
+    // out-stager's stream while this one runs the solve (the solve only reads
+    // them).  It is joined before any return.
+    const size_t ld = (size_t)h->lda, n8 = (size_t)N * sizeof(double);
+    std::thread out_thread;
+    int rc_out = CLOWNS_OK;
+    if (!rc) {
+        if (hipEventRecord(so->ready, s) != hipSuccess || hipStreamWaitEvent(so->cs, so->ready, 0) != hipSuccess)
+            rc = CLOWNS_EHIP;
+        else
+            out_thread = std::thread([h, so, AL, LL, D, NA, N, NL, ld, n8, &rc_out] {
+                int r = set_device(h);
+                if (!r) r = stage_d2h_cols(so, so->cs, AL, NA, h->AL.p, ld, N, NL, sizeof(double));
+                if (!r) r = stage_d2h_cols(so, so->cs, LL, NA, h->LL.p, ld, N, NL, sizeof(int));
+                if (!r) r = stage_d2h(so, so->cs, D, h->D.p, n8);
+                if (!r) r = stage_finish(so);
+                if (!r && hipStreamSynchronize(so->cs) != hipSuccess) r = CLOWNS_EHIP;
+                rc_out = r;
+            });
+    }
     if (!rc) rc = h->stage_b.ensure(N);
     if (!rc) rc = h->stage_x.ensure(N);
-    if (!rc) rc = stage_h2d(st, s, h->stage_b.p, B, (size_t)N * sizeof(double));
-    if (!rc && warm) rc = stage_h2d(st, s, h->stage_x.p, X, (size_t)N * sizeof(double));
+    if (!rc) rc = stage_h2d(st, s, h->stage_b.p, B, n8);
+    if (!rc && warm) rc = stage_h2d(st, s, h->stage_x.p, X, n8);
     clowns_info info{};
     if (!rc) {
         Op<double> A = op64(h);
@@ -2037,17 +2081,15 @@ extern "C" void clowns_pcgpme(double *AL, const int32_t *NL_, const int32_t *NA_
     }
     const auto t3 = clk::now();
     if (!rc) {
-        // the reference leaves AL, D scaled, LL sorted and B the final
-        // scaled residual (pcgpme_para.F90:77-112,250-257)
-        const size_t ld = (size_t)h->lda, n8 = (size_t)N * sizeof(double);
-        if ((rc = stage_d2h_cols(st, s, AL, NA, h->AL.p, ld, N, NL, sizeof(double))) ||
-            (rc = stage_d2h_cols(st, s, LL, NA, h->LL.p, ld, N, NL, sizeof(int))) ||
-            (rc = stage_d2h(st, s, D, h->D.p, n8)) || (rc = stage_d2h(st, s, B, h->v64.r.p, n8)) ||
-            (rc = stage_d2h(st, s, X, h->stage_x.p, n8)) || (rc = stage_finish(st))) {
+        // B the final scaled residual, X the solution (:250-257)
+        if ((rc = stage_d2h(st, s, B, h->v64.r.p, n8)) || (rc = stage_d2h(st, s, X, h->stage_x.p, n8)) ||
+            (rc = stage_finish(st))) {
         } else if (hipStreamSynchronize(s) != hipSuccess) {
             rc = CLOWNS_EHIP;
         }
     }
+    if (out_thread.joinable()) out_thread.join();
+    if (!rc) rc = rc_out;
     if (h->kn.verbose) {
         const auto ms = [](clk::time_point a, clk::time_point b) {
             return std::chrono::duration<double, std::milli>(b - a).count();
@@ -2224,7 +2266,7 @@ extern "C" void clowns_pcgpme_will(const double *AL, const int32_t *NL_, const i
     hipStream_t s = h->stream;
     // AL, LL and D are inputs only (the reference scales copies,
     // pcgpme_2DH.f90:56-76); keep_raw holds the caller-order LL.
-    int rc = ensure_stager(h);
+    int rc = ensure_stager(h->stager, true, false);
     Stager *st = h->stager;
     if (!rc) rc = load_matrix(h, AL, NL, NA, LL, D, N, *PARM, hipMemcpyHostToDevice, true, st);
     if (!rc) rc = h->stage_b.ensure(N);
