@@ -961,7 +961,7 @@ def main():
             boundary["ratio_to_resident"] = boundary["s_per_solve"] / t64
         del sysm
         if not args.no_cpu:
-            # the host's cores, up to the 16 a one-GPU box grants a process
+            # the host's cores, up to the 16 a one-GPU host grants a process
             threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
             gx = {"fp64": x64.cpu().numpy()}
             if x32 is not None:

@@ -7,7 +7,7 @@ the flang build of pcgpme_para.F90 at one thread (test_oracle_pcgpme.py); its
 spread over 1, 2, 4 and 8 threads is measured on the same system with the
 flang build itself (the C oracle where that was not built).  The bar:
   * iteration count inside the reference's own range widened by 10%;
-  * relative L2 distance from the reference's X (the north_star figure) within
+  * relative L2 distance from the reference's X (the 1e-9 target) within
     max(1e-9, 2x the reference's thread spread);
   * distance from a tightly converged solution (EPS = 2e-15, stagnation exit)
     no worse than twice the reference's own.

@@ -64,7 +64,7 @@ def test_converged_solution(shape):
     """At the 1e-10 stopping rule the Bi-CGSTAB trajectory depends on the
     summation order of its dot products: the reference itself, run at 1, 2, 4
     and 8 OpenMP threads, ends at different iteration counts and different X.
-    So the GPU solve is judged against a tightly converged solution
+    So the GPU solve is compared with a tightly converged solution
     (EPS = 2e-15, stagnation exit) and against that spread: its error must be
     within 1e-9 relative L2 or no worse than 2x the reference's worst, and its
     iteration count inside the reference's range widened by 10%."""
