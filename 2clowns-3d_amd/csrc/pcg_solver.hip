@@ -25,6 +25,7 @@
 #include <mutex>
 #include <new>
 #include <sched.h>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -1075,8 +1076,14 @@ namespace {
 // hold g_mu.
 class HostPool {
 public:
-    explicit HostPool(int n) : nt_(std::max(1, n)) {
-        for (int t = 1; t < nt_; ++t) th_.emplace_back([this, t] { loop(t); });
+    explicit HostPool(int n) {
+        // as many threads as the system gives (the calling thread is part 0)
+        int made = 1;
+        try {
+            for (; made < std::max(1, n); ++made) th_.emplace_back([this, made] { loop(made); });
+        } catch (const std::system_error &) {
+        }
+        nt_ = made;
     }
     ~HostPool() {
         {
@@ -1130,7 +1137,7 @@ private:
             if (--pending_ == 0) done_.notify_one();
         }
     }
-    const int nt_;
+    int nt_ = 1;
     std::vector<std::thread> th_;
     std::mutex mu_;
     std::condition_variable cv_, done_;
@@ -2056,19 +2063,25 @@ extern "C" void clowns_pcgpme(double *AL, const int32_t *NL_, const int32_t *NA_
     const size_t ld = (size_t)h->lda, n8 = (size_t)N * sizeof(double);
     std::thread out_thread;
     int rc_out = CLOWNS_OK;
+    auto copy_out = [h, so, AL, LL, D, NA, N, NL, ld, n8]() -> int {
+        int r = set_device(h);
+        if (!r) r = stage_d2h_cols(so, so->cs, AL, NA, h->AL.p, ld, N, NL, sizeof(double));
+        if (!r) r = stage_d2h_cols(so, so->cs, LL, NA, h->LL.p, ld, N, NL, sizeof(int));
+        if (!r) r = stage_d2h(so, so->cs, D, h->D.p, n8);
+        if (!r) r = stage_finish(so);
+        if (!r && hipStreamSynchronize(so->cs) != hipSuccess) r = CLOWNS_EHIP;
+        return r;
+    };
     if (!rc) {
-        if (hipEventRecord(so->ready, s) != hipSuccess || hipStreamWaitEvent(so->cs, so->ready, 0) != hipSuccess)
+        if (hipEventRecord(so->ready, s) != hipSuccess || hipStreamWaitEvent(so->cs, so->ready, 0) != hipSuccess) {
             rc = CLOWNS_EHIP;
-        else
-            out_thread = std::thread([h, so, AL, LL, D, NA, N, NL, ld, n8, &rc_out] {
-                int r = set_device(h);
-                if (!r) r = stage_d2h_cols(so, so->cs, AL, NA, h->AL.p, ld, N, NL, sizeof(double));
-                if (!r) r = stage_d2h_cols(so, so->cs, LL, NA, h->LL.p, ld, N, NL, sizeof(int));
-                if (!r) r = stage_d2h(so, so->cs, D, h->D.p, n8);
-                if (!r) r = stage_finish(so);
-                if (!r && hipStreamSynchronize(so->cs) != hipSuccess) r = CLOWNS_EHIP;
-                rc_out = r;
-            });
+        } else {
+            try {
+                out_thread = std::thread([&rc_out, &copy_out] { rc_out = copy_out(); });
+            } catch (const std::system_error &) {
+                rc_out = copy_out();  // no thread to be had: the same copies, before the solve
+            }
+        }
     }
     if (!rc) rc = h->stage_b.ensure(N);
     if (!rc) rc = h->stage_x.ensure(N);
