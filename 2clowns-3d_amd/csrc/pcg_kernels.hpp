@@ -6,7 +6,7 @@
 //   k_spmv_v2      AXSUB, y = A x in sorted slot order (:323-352), two rows
 //                  a thread; k_spmv_rows32 its FP32 form from row-major
 //                  copies (mixed inner solve), k_spmv_rows32_p2 that form
-//                  in two chunks a wavefront (8-bit codes)
+//                  in two chunks a wavefront
 //   k_pm_v2        one stage of LDUSUB (:279-318); the last stage carries the
 //                  dot products that follow it ((r^,q) :157-160,
 //                  (e,v),(v,v) :186-191); k_pm_v4, k_pm2_v2 its FP32

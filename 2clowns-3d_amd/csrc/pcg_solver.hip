@@ -423,7 +423,7 @@ template <typename T> struct Op {
     const int *order = nullptr;   // row-block order of the gather kernels, or null
     const _Float16 *ALh = nullptr;  // FP32 inner solve: half-precision PM coefficients, or null
     bool v4 = false;                // FP32 single PM stages with four rows a thread (k_pm_v4)
-    bool p2_32 = false;             // FP32 SpMV with 8-bit codes as k_spmv_rows32_p2
+    bool p2_32 = false;             // FP32 row-major SpMV as k_spmv_rows32_p2 (two chunks a wavefront)
     const float *ALr = nullptr;     // FP32 SpMV: row-major coefficients and codes, or null
     const short *Lr16 = nullptr;
     const uint8_t *Lr8 = nullptr;   // ... 8-bit codes (with LL8), or null
